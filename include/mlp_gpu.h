@@ -1,0 +1,157 @@
+/*
+ * mlp_gpu.h -- C ABI of the MI355X Minimum Latency Problem local-search engine.
+ *
+ * Plain C types only (no HIP, no torch).  Every call returns an int status
+ * (MLP_OK == 0, negative on error; mlp_ctx_error() holds the message) and never
+ * throws or exits.  A context is externally synchronised: one host thread at a
+ * time per context.  Device work is queued on the context's HIP stream; calls
+ * that hand results back to host memory synchronise that stream first.
+ *
+ * Model.  An *instance* holds n client coordinates (client 0 = depot) and the
+ * distance rounding mode.  A *slot* is a device-resident solution of one
+ * instance: a tour of N = n + 1 positions [0, perm(1..n-1), 0], its ADS (the
+ * prefix arrays of the reference's auxiliary data structure), the per-row best
+ * move of each neighborhood, and the last merged move set.  Neighborhoods:
+ * 0 swap, 1 2-opt, 2/3/4 or-opt-1/2/3 (ids as mlads.h:144-176 MLMoveId).
+ * Row maps: 0 = the reference's ADS kernel grids, 1 = its ODC kernel grids.
+ *
+ * Reference interfaces these entry points replace (LabIC-UFF/wamca-gpu-mi):
+ *   MLProblem::load distances           mlproblem.cpp:153-166  -> mlp_instance_create
+ *   MLSolution::update / ldsUpdate      mlsolution.cpp:215-228, 419-486
+ *                                                              -> mlp_set_tours (device ADS)
+ *   MLKernel::sendSolution              mlkernel.h:229-235     -> mlp_set_tours
+ *   MLKernel{Swap,2Opt,OrOpt}::launchKernel
+ *                                       mlkswap.cu:1038-1064, mlk2opt.cu:705-723,
+ *                                       mlkoropt.cu:863-884    -> mlp_eval
+ *   MLKernel::recvResult / bestMove     mlkernel.h:240-244, mlkernel.cu:617-647
+ *                                                              -> mlp_get_rows / mlp_get_moves64
+ *   MLKernel::mergeGPU + applyMove      mlkernel.cu:254-384, mlkswap.cu:1066-1106,
+ *                                       mlk2opt.cu:725-768, mlkoropt.cu:886-950
+ *                                                              -> mlp_merge_apply
+ *   WAMCAExperiment::runKernel chain    WamcaExperiment.hpp:292-365, 368-581
+ *                                                              -> mlp_dvnd
+ *   MLSolution::ldsUpdate ADS buffer    mlads.h:38-74 layout   -> mlp_ads_materialize
+ */
+#ifndef MLP_GPU_H_
+#define MLP_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MLP_OK = 0,
+  MLP_EINVAL = -1,   /* bad argument (size, id, tour not a permutation ...) */
+  MLP_ENOMEM = -2,   /* device or host allocation failed / admission refused */
+  MLP_EDEVICE = -3,  /* HIP runtime error */
+  MLP_ERANGE = -4,   /* value outside the representable range (N, delta) */
+  MLP_ESTATE = -5    /* call out of order (e.g. rows read before an eval) */
+};
+
+enum { MLP_NB_SWAP = 0, MLP_NB_2OPT = 1, MLP_NB_OROPT1 = 2, MLP_NB_OROPT2 = 3, MLP_NB_OROPT3 = 4 };
+enum { MLP_ROWMAP_ADS = 0, MLP_ROWMAP_ODC = 1 };
+
+/* Largest supported tour size N (positions).  Per-row keys pack the row-local
+ * index in 20 bits and the delta in the remaining 43. */
+#define MLP_MAX_N 16384
+
+typedef struct mlp_ctx mlp_ctx;
+
+/* A merged move as returned to the host (the reference's MLMove with an int64
+ * cost, mlads.h:211-216). */
+typedef struct {
+  int64_t cost;
+  int32_t id;
+  int32_t i;
+  int32_t j;
+  int32_t row;
+} mlp_move;
+
+/* Context: device selection, stream, kernels. */
+int mlp_ctx_create(int device, mlp_ctx** out);
+int mlp_ctx_destroy(mlp_ctx* ctx);
+const char* mlp_ctx_error(const mlp_ctx* ctx);
+const char* mlp_strerror(int status);
+int mlp_sync(mlp_ctx* ctx);
+int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* name, int32_t name_len);
+
+/* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5). */
+int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                        int32_t round, int32_t* inst);
+int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst);
+/* Device-computed client-indexed distance matrix (n*n int32) -- used to check
+ * the device distance function against the host one. */
+int mlp_instance_weights(mlp_ctx* ctx, int32_t inst, int32_t* w_out);
+
+/* HBM bytes a slot of an n-client instance holds (admission control). */
+int64_t mlp_slot_footprint(int32_t n);
+
+/* Slots. */
+int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot);
+int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot);
+
+/* Upload tours (count slots, tours concatenated, each N ints) and build the
+ * ADS on the device.  Each tour must start and end with 0 and visit every
+ * client once. */
+int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32_t* tours);
+int mlp_get_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* tours);
+int mlp_get_latency(mlp_ctx* ctx, const int32_t* slots, int32_t count, int64_t* latency);
+
+/* Evaluate the neighborhoods in nb_mask (bit nb) for every slot: one per-row
+ * best move per row of the row map (ties -> lowest row-local index).
+ * persistent != 0 drains a device work queue of (slot, neighborhood, row
+ * block) items with a persistent grid instead of one workgroup per item. */
+int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+             int32_t rowmap, int32_t persistent);
+
+/* Per-row results of the last eval of (slot, nb): cost (INT64_MAX for a row
+ * without moves), i, j.  *rows receives the row count; arrays hold MLP_MAX_N. */
+int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t* i, int32_t* j,
+                 int32_t* rows);
+
+/* Same rows in the reference's 64-bit wire format MLMove64 (mlads.h:190-206):
+ * low word id:4 | i:14 | j:14, high word int32 cost.  Returns MLP_ERANGE if a
+ * cost does not fit in int32 or N exceeds 16384. */
+int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int32_t* rows);
+
+/* Device greedy merge of the last eval of nb (stable sort by cost, keep an
+ * improving move iff no earlier kept move of the same neighborhood conflicts)
+ * and application of the kept moves; the ADS is rebuilt for the new tour. */
+int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb);
+
+/* The moves kept by the last mlp_merge_apply of (slot, nb), in sorted order. */
+int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int32_t cap,
+                    int32_t* count);
+
+/* DVND to a local optimum: rounds of {eval, merge, apply} over swap, 2-opt,
+ * or-opt-1/2/3 until a round applies no move (or max_rounds, 0 = unbounded).
+ * rounds_out (per slot, may be NULL) receives the rounds each slot ran. */
+int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
+             int32_t max_rounds, int32_t* rounds_out);
+
+/* The reference's ADS buffer (mlads.h:38-74) for the slot's current tour, built
+ * on the device: 128-byte header, coords, [client|weight], T, C[N][N] (uint32
+ * with the reference's wrap-around).  bytes = 128 + 4*N*(N+3). */
+int mlp_ads_materialize(mlp_ctx* ctx, int32_t slot, uint32_t* host_out, int64_t bytes);
+
+/* Device-side distance function over a range of squared lengths s (for
+ * exhaustive checks against the host): out[q] = d(s0 + q). */
+int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32_t* out);
+/* Same for an explicit list of squared lengths. */
+int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out);
+
+/* Timing of the last eval launch on the context stream (ms, HIP events). */
+int mlp_last_eval_ms(mlp_ctx* ctx, float* ms);
+/* Per-launch eval timing: between begin and end every mlp_eval launch is
+ * bracketed by its own pair of HIP events (up to capacity launches); end
+ * synchronises and returns the summed kernel time and the launch count. */
+int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity);
+int mlp_timing_end(mlp_ctx* ctx, double* total_ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLP_GPU_H_ */

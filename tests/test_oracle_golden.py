@@ -1,0 +1,124 @@
+"""Pins the CPU oracle to every known answer the reference holds.
+
+* tests/mtrand.Test.cpp:16-47           -- MT19937 raw outputs and %10 runs
+* tests/mlproblem.Test.cpp:27           -- berlin52 problem size 53
+* tests/mlsolution.Test.cpp:59,73,79     -- random(seed 0, alpha .5) -> 494309
+* tests/WamcaExperiment.Test.cpp:131     -- one ODC swap pass + merge -> 367403
+* results/*.txt (Appendix-B fixtures)    -- 22 start tours and latencies, and
+  for SWAP / 2OPT / OROPT1: global best, improving rows, merged (sum, count)
+"""
+import numpy as np
+import pytest
+
+from helpers import load_tsplib_coords, results_logs
+from oracle import oracle as O
+
+
+def test_mt19937_raw_seed0():
+    m = O.MT(0)
+    assert [m.rand() for _ in range(5)] == [2357136044, 2546248239, 3071714933, 3626093760,
+                                            2588848963]
+
+
+@pytest.mark.parametrize("seed,expect", [(0, [4, 9, 3, 0, 3]), (1, [5, 9, 4, 8, 3])])
+def test_mt19937_mod10(seed, expect):
+    m = O.MT(seed)
+    assert [m.rand() % 10 for _ in range(5)] == expect
+    m = O.MT(seed)
+    assert m.rand() % 10 == expect[0]
+
+
+def test_mt19937_matches_numpy_legacy():
+    # numpy's legacy RandomState(seed) is init_genrand MT19937 as well
+    rs = np.random.RandomState(12345)
+    ref = rs.randint(0, 2**32, size=2000, dtype=np.uint64)
+    m = O.MT(12345)
+    assert [m.rand() for _ in range(2000)] == [int(v) for v in ref]
+
+
+def test_randmax_closed_form():
+    m1, m2 = O.MT(7), O.MT(7)
+    for mx in [0, 1, 2, 9, 99, 1000, 2**31, 2**32 - 2]:
+        for _ in range(50):
+            u = m2.rand()
+            assert m1.randmax(mx) == int(((mx + 1.0) * u) / 4294967296.0)
+
+
+def _berlin():
+    xs, ys = load_tsplib_coords("01_berlin52.tsp")
+    return O.weights(xs, ys, 0)
+
+
+def test_berlin52_size_and_start_latency():
+    W = _berlin()
+    assert W.shape[0] + 1 == 53
+    t = O.random_tour(O.MT(0), W, 0.5)
+    assert len(t) == 53 and t[0] == 0 and t[-1] == 0
+    assert sorted(t[:-1].tolist()) == list(range(52))
+    assert O.latency(W, t) == 494309
+    assert O.latency_u32(W, t) == 494309
+
+
+def test_berlin52_odc_swap_pass():
+    W = _berlin()
+    t = O.random_tour(O.MT(0), W, 0.5)
+    t2, count, s = O.step(0, 1, W, t)
+    assert O.latency(W, t2) == 367403
+    assert O.latency(W, t) + s == 367403
+    # the ADS row partition gives a different merged set: the test pins ODC rows
+    t3, _, _ = O.step(0, 0, W, t)
+    assert O.latency(W, t3) != 367403
+
+
+def _logged():
+    out = []
+    for log in results_logs():
+        for sol in log["solutions"]:
+            out.append((log, sol))
+    return out
+
+
+_COORDS = {}
+
+
+def _W(inst):
+    if inst not in _COORDS:
+        xs, ys = load_tsplib_coords(inst)
+        _COORDS[inst] = O.weights(xs, ys, 0)
+    return _COORDS[inst]
+
+
+@pytest.mark.parametrize("log", results_logs(), ids=lambda l: l["log"])
+def test_logged_start_tours(log):
+    """The RNG seeded as the log says regenerates each logged tour, and the
+    truncated-distance latency equals the logged uint32 latency."""
+    W = _W(log["instance"])
+    rng = O.MT(log["seed"])
+    for sol in log["solutions"]:
+        t = O.random_tour(rng, W, 0.5)
+        assert t.tolist() == sol["tour"]
+        assert O.latency_u32(W, t) == sol["latency"]
+
+
+CASES = [(log["log"], sol["index"], name)
+         for log, sol in _logged() for name in sol["kernels"]]
+
+
+@pytest.mark.parametrize("logname,idx,kname", CASES)
+def test_logged_partial_kernels(logname, idx, kname):
+    log = next(l for l in results_logs() if l["log"] == logname)
+    sol = next(s for s in log["solutions"] if s["index"] == idx)
+    k = sol["kernels"][kname]
+    if "partial" not in k or "best" not in k["partial"]:
+        pytest.skip("log has no partial section for this kernel")
+    p = k["partial"]
+    W = _W(log["instance"])
+    t = np.array(sol["tour"], np.int32)
+    r = O.eval_rows(k["nb"], 0, W, t)
+    # global best: first record after a stable sort by cost
+    b = int(np.argsort(r["cost"], kind="stable")[0])
+    assert int(r["cost"][b]) == p["best"][2]
+    assert [int(r["i"][b]), int(r["j"][b])] == p["best"][:2]
+    assert [int((r["cost"] < 0).sum()), len(r["cost"])] == p["graph"]
+    kept, s = O.merge(k["nb"], r["cost"], r["i"], r["j"])
+    assert [s, len(kept)] == p["merge_gpu"]

@@ -1,0 +1,74 @@
+// mlp_internal.h -- data layout shared by the HIP kernels and the host runtime.
+//
+// A slot is one device-resident solution.  Its ADS is kept in O(N) prefix form
+// (the closed form of the reference's O(N^2) ADS, mlsolution.cpp:419-486):
+//   xy[p]  packed tour-ordered coordinates (x << 16 | y) of client tour[p]
+//   w[p]   edge weight d(tour[p-1], tour[p]), w[0] = 0       (mlsolution.cpp:224-226)
+//   T[p]   prefix duration  sum_{q<=p} w[q]                  (ADS "T" row)
+//   Q[p]   prefix moment    sum_{q<=p} q * w[q]
+// C[i][j] of the reference equals (S[j]-S[i]) - (j-i) T[i] with S the prefix
+// of T; the move costs below only need T and Q.  Both are kept as int64 and as
+// their low 32 bits (int32 path, exact while every |delta| < 2^31).
+#ifndef MLP_INTERNAL_H_
+#define MLP_INTERNAL_H_
+
+#include <stdint.h>
+
+#define MLP_NB_COUNT 5
+#define MLP_KEY_SHIFT 20                       // row-local index bits in a key
+#define MLP_KEY_MASK ((1LL << MLP_KEY_SHIFT) - 1)
+#define MLP_KEY_NONE INT64_MAX                 // row without a move
+#define MLP_MERGE_MAX_ROWS 8192                // LDS-resident merge capacity
+
+// info[] words of a slot
+enum {
+  MLP_INFO_LATENCY = 0,   // int64 latency of the current tour
+  MLP_INFO_TTOT = 1,      // T[N-1]
+  MLP_INFO_PATH32 = 2,    // 1 if the int32 path is exact for this tour
+  MLP_INFO_ROUND_MOVES = 3,  // moves applied since the last round reset
+  MLP_INFO_TOTAL_MOVES = 4,  // moves applied since the tour was set
+  MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
+  MLP_INFO_WORDS = 16
+};
+
+struct SlotDev {
+  int32_t N;      // positions (n + 1)
+  int32_t n;      // clients
+  int32_t round;  // 0 trunc, 1 nint
+  int32_t maxd;   // upper bound of any distance in the instance
+  const int32_t* xs;  // instance coordinates (client indexed)
+  const int32_t* ys;
+  int32_t* tour;  // [N]
+  int32_t* xy;    // [N]
+  int32_t* w;     // [N]
+  int32_t* t32;   // [N]
+  int32_t* q32;   // [N]
+  int64_t* t64;   // [N]
+  int64_t* q64;   // [N]
+  int64_t* info;  // [MLP_INFO_WORDS]
+  int64_t* rows[MLP_NB_COUNT];      // per-row best keys (delta << 20 | ctx)
+  int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
+  int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
+};
+
+// (slot, neighborhood) segment of an eval launch: rows of the row map are cut
+// into blocks of MLP_EVAL_ROWS_PER_WG rows; blk0 is the first block index.
+#define MLP_EVAL_ROWS_PER_WG 4
+struct EvalSeg {
+  int32_t slot;
+  int32_t nb;
+  int32_t rows;
+  int32_t blk0;
+};
+
+#define MLP_SEGS_INLINE 40
+struct EvalSegsInline {
+  EvalSeg s[MLP_SEGS_INLINE];
+};
+
+#define MLP_SLOTS_INLINE 128
+struct SlotListInline {
+  int32_t id[MLP_SLOTS_INLINE];
+};
+
+#endif  // MLP_INTERNAL_H_

@@ -1,0 +1,717 @@
+// mlp_kernels.hip -- gfx950 kernels of the MLP local search.
+//
+//   k_prep          tour -> device ADS (xy, w, T, Q, latency)  [MLSolution::update
+//                   + ldsUpdate, mlsolution.cpp:215-228, 419-486]
+//   k_eval          per-row best move of swap / 2-opt / or-opt-k over a
+//                   (slot, neighborhood, row block) work list, one wave per row,
+//                   O(1) integer cost per move, wave argmin on (delta, index)
+//                   [kernelSwap / kernel2Opt / kernelOrOpt, mlkswap.cu:54-332,
+//                   mlk2opt.cu:41-267, mlkoropt.cu:48-366]
+//   k_merge_apply   stable sort of the improving rows, greedy non-overlapping
+//                   subset, parallel application, ADS rebuild
+//                   [MLKernel::mergeGPU mlkernel.cu:254-384 + testSwap/test2Opt/
+//                   testOrOpt :73-170 + applyMove mlkswap.cu:1066, mlk2opt.cu:725,
+//                   mlkoropt.cu:886]
+//   k_ads_*         the reference's materialised ADS buffer (mlads.h:38-74)
+//
+// Integer arithmetic throughout; the only floating-point step is the distance
+// d = int(sqrtf(float(dx^2 + dy^2)) + dr) with an IEEE correctly rounded
+// square root (mlproblem.cpp:159-162).
+#include <hip/hip_runtime.h>
+
+#include "mlp_internal.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// distance
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int dist_s(int s, float dr) {
+  const float f = (float)s;           // v_cvt_f32_i32, round to nearest
+  const float r = __builtin_sqrtf(f);  // correctly rounded (checked exhaustively)
+  return (int)(r + dr);               // RN add, truncating convert
+}
+
+__device__ __forceinline__ int dist_xy(int a, int b, float dr) {
+  const int dx = (a >> 16) - (b >> 16);
+  const int dy = (a & 0xffff) - (b & 0xffff);
+  return dist_s(__mul24(dx, dx) + __mul24(dy, dy), dr);
+}
+
+// ---------------------------------------------------------------------------
+// row maps (the reference's launch grids and thread -> move maps, tour = 1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int nb_k(int nb) { return nb >= 2 ? nb - 1 : 0; }
+
+__device__ __forceinline__ int rows_of(int nb, int rowmap, int N) {
+  if (nb == 0) return rowmap == 0 ? N - 4 : ((N + 1) >> 1) - 1;  // mlkswap.cu:991,1003
+  if (nb == 1) return (N + 1) >> 1;                              // mlk2opt.cu:661
+  return N - nb_k(nb) - 2;                                       // mlkoropt.cu:819
+}
+
+// Valid row-local index range [lo, hi) of a row.
+__device__ __forceinline__ void ctx_range(int nb, int rowmap, int N, int row, int& lo, int& hi) {
+  if (nb == 0) {
+    if (rowmap == 0) { lo = row + 3; hi = N - 1; }  // j = ctx in [i + 2, N - 2]
+    else { lo = 0; hi = N - 4; }
+  } else if (nb == 1) {
+    lo = 0; hi = N - 3;
+  } else {
+    lo = 0;
+    hi = rowmap == 0 ? N - nb_k(nb) - 1 : N - nb_k(nb) - 2;
+  }
+}
+
+// (row, ctx) -> (i, j); ctx must lie in ctx_range.
+__device__ __forceinline__ void move_ij(int nb, int rowmap, int N, int row, int ctx, int& i, int& j) {
+  if (nb == 0) {
+    if (rowmap == 0) { i = row + 1; j = ctx; }  // mlkswap.cu:205-206
+    else if (ctx >= row) { i = row + 1; j = ctx + 3; }  // mlkswap.cu:766-768
+    else { i = N - row - 3; j = N - ctx - 2; }
+  } else if (nb == 1) {  // mlk2opt.cu:170-174
+    if (ctx >= row) { i = ctx - row + 1; j = ctx + 2; }
+    else { i = row - ctx; j = N - ctx - 2; }
+  } else {
+    const int k = nb_k(nb);
+    if (rowmap == 0) {  // mlkoropt.cu:187-247
+      i = ctx > row ? row + 1 : row + 2;
+      j = ctx + 1;
+    } else if (ctx >= row) {  // mlkoropt.cu:697-723
+      i = ctx - row + 1; j = ctx + 2;
+    } else {
+      i = N - ctx - k - 1; j = row - ctx;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// O(1) move deltas from the prefix ADS.  Each equals L(apply(tour)) - L(tour)
+// (the reference's ADS concatenation value, mlkswap.cu:210-278 ...).  P32
+// computes modulo 2^32, exact when the tour's bound puts every |delta| < 2^31.
+// ---------------------------------------------------------------------------
+template <bool P32>
+struct Arith;
+template <>
+struct Arith<true> {
+  using U = uint32_t;
+  __device__ static int64_t fin(U v) { return (int64_t)(int32_t)v; }
+  __device__ static U T(const SlotDev& s, int p) { return (U)s.t32[p]; }
+  __device__ static U Q(const SlotDev& s, int p) { return (U)s.q32[p]; }
+};
+template <>
+struct Arith<false> {
+  using U = int64_t;
+  __device__ static int64_t fin(U v) { return v; }
+  __device__ static U T(const SlotDev& s, int p) { return s.t64[p]; }
+  __device__ static U Q(const SlotDev& s, int p) { return s.q64[p]; }
+};
+
+// swap(i, j), j >= i + 2 (ODC positional form, mlkswap.cu:770-785)
+template <bool P32>
+__device__ __forceinline__ int64_t delta_swap(const SlotDev& s, int i, int j, float dr) {
+  using A = Arith<P32>;
+  using U = typename A::U;
+  const int N = s.N;
+  const int a = s.xy[i - 1], xi = s.xy[i], b = s.xy[i + 1];
+  const int jm = s.xy[j - 1], xj = s.xy[j], jp = s.xy[j + 1];
+  const int t1 = __mul24(N - i, dist_xy(a, xj, dr) - s.w[i]);
+  const int t2 = __mul24(N - i - 1, dist_xy(xj, b, dr) - s.w[i + 1]);
+  const int t3 = __mul24(N - j, dist_xy(jm, xi, dr) - s.w[j]);
+  const int t4 = __mul24(N - j - 1, dist_xy(xi, jp, dr) - s.w[j + 1]);
+  return A::fin((U)t1 + (U)t2 + (U)t3 + (U)t4);
+}
+
+// 2-opt(i, j): reverse [i..j] (mlk2opt.cu:565-571 with the internal sum in
+// closed form: sum_{p=i+1}^{j} (2p - i - j - 1) w[p] = 2(Q[j]-Q[i]) - (i+j+1)(T[j]-T[i]))
+template <bool P32>
+__device__ __forceinline__ int64_t delta_2opt(const SlotDev& s, int i, int j, float dr) {
+  using A = Arith<P32>;
+  using U = typename A::U;
+  const int N = s.N;
+  const int t1 = __mul24(N - i, dist_xy(s.xy[i - 1], s.xy[j], dr) - s.w[i]);
+  const int t2 = __mul24(N - j - 1, dist_xy(s.xy[i], s.xy[j + 1], dr) - s.w[j + 1]);
+  const U inner = (U)2 * (A::Q(s, j) - A::Q(s, i)) - (U)(i + j + 1) * (A::T(s, j) - A::T(s, i));
+  return A::fin((U)t1 + (U)t2 + inner);
+}
+
+// or-opt-k(i, j): segment [i..i+k-1] moved to start at j (mlkoropt.cu:704-736)
+template <bool P32>
+__device__ __forceinline__ int64_t delta_oropt(const SlotDev& s, int k, int i, int j, float dr) {
+  using A = Arith<P32>;
+  using U = typename A::U;
+  const int N = s.N;
+  if (i < j) {
+    const int t1 = __mul24(N - i, dist_xy(s.xy[i - 1], s.xy[i + k], dr) - s.w[i]);
+    const int t2 = __mul24(N - j - k, dist_xy(s.xy[i + k - 1], s.xy[j + k], dr) - s.w[j + k]);
+    const int t3 = __mul24(N - j, dist_xy(s.xy[j + k - 1], s.xy[i], dr)) -
+                   __mul24(N - i - k, s.w[i + k]);
+    const U seg = A::T(s, i + k - 1) - A::T(s, i);
+    const U blk = A::T(s, j + k - 1) - A::T(s, i + k);
+    return A::fin((U)t1 + (U)t2 + (U)t3 + (U)(i - j) * seg + (U)k * blk);
+  } else {
+    const int t1 = __mul24(N - j, dist_xy(s.xy[j - 1], s.xy[i], dr) - s.w[j]);
+    const int t2 = __mul24(N - i - k, dist_xy(s.xy[i - 1], s.xy[i + k], dr) - s.w[i + k]);
+    const int t3 = __mul24(N - j - k, dist_xy(s.xy[i + k - 1], s.xy[j], dr)) -
+                   __mul24(N - i, s.w[i]);
+    const U seg = A::T(s, i + k - 1) - A::T(s, i);
+    const U blk = A::T(s, i - 1) - A::T(s, j);
+    return A::fin((U)t1 + (U)t2 + (U)t3 + (U)(i - j) * seg - (U)k * blk);
+  }
+}
+
+__device__ __forceinline__ int64_t wave_min64(int64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int64_t o = __shfl_xor((long long)v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float round_dr(int round) { return round ? 0.5F : 0.0F; }
+
+// One row: lanes stride over the row-local index, keep the smallest key
+// (delta << 20 | ctx) -> lowest delta, then lowest index; wave reduction.
+template <int KIND, bool P32>
+__device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane) {
+  const int N = s.N;
+  const float dr = round_dr(s.round);
+  int lo, hi;
+  ctx_range(nb, rowmap, N, row, lo, hi);
+  int64_t best = MLP_KEY_NONE;
+  const int k = nb_k(nb);
+  for (int ctx = lo + lane; ctx < hi; ctx += kWave) {
+    int i, j;
+    move_ij(nb, rowmap, N, row, ctx, i, j);
+    int64_t d;
+    if (KIND == 0) d = delta_swap<P32>(s, i, j, dr);
+    else if (KIND == 1) d = delta_2opt<P32>(s, i, j, dr);
+    else d = delta_oropt<P32>(s, k, i, j, dr);
+    const int64_t key = d * (int64_t)(1 << MLP_KEY_SHIFT) + ctx;
+    best = key < best ? key : best;
+  }
+  best = wave_min64(best);
+  if (lane == 0) s.rows[nb][row] = best;
+}
+
+template <bool P32>
+__device__ void eval_row_dispatch(const SlotDev& s, int nb, int rowmap, int row, int lane) {
+  if (nb == 0) eval_row<0, P32>(s, nb, rowmap, row, lane);
+  else if (nb == 1) eval_row<1, P32>(s, nb, rowmap, row, lane);
+  else eval_row<2, P32>(s, nb, rowmap, row, lane);
+}
+
+__device__ __forceinline__ EvalSeg find_seg(const EvalSeg* segs, int nsegs, int item) {
+  int lo = 0, hi = nsegs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].blk0 <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return segs[lo];
+}
+
+// Inline (kernel-argument) segment table: constant indices only, so the
+// argument stays in the kernarg segment instead of being copied to scratch.
+__device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, int nsegs, int item) {
+  EvalSeg sg = inl.s[0];
+#pragma unroll
+  for (int q = 1; q < MLP_SEGS_INLINE; ++q)
+    if (q < nsegs && inl.s[q].blk0 <= item) sg = inl.s[q];
+  return sg;
+}
+
+__device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
+                          int rowmap) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int row = (item - sg.blk0) * MLP_EVAL_ROWS_PER_WG + wave;
+  if (row >= sg.rows) return;
+  const SlotDev& s = table[sg.slot];
+  if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane);
+  else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane);
+}
+
+__device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t* dev, int q) {
+  if (dev) return dev[q];
+  int v = inl.id[0];
+#pragma unroll
+  for (int t = 1; t < MLP_SLOTS_INLINE; ++t)
+    if (t == q) v = inl.id[t];
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// ADS prefix build for one slot by one workgroup.  tour may live in LDS.
+// ---------------------------------------------------------------------------
+__device__ void prep_wg(const SlotDev& s, const int32_t* tour, int64_t* sh /* >= 2*kBlock+4 */) {
+  const int N = s.N, tid = threadIdx.x;
+  const float dr = round_dr(s.round);
+  const int per = (N + kBlock - 1) / kBlock;
+  const int p0 = tid * per, p1 = min(N, p0 + per);
+  int64_t tsum = 0, qsum = 0;
+  for (int p = p0; p < p1; ++p) {
+    const int c = tour[p];
+    const int xy = (s.xs[c] << 16) | s.ys[c];
+    int w = 0;
+    if (p > 0) {
+      const int cp = tour[p - 1];
+      w = dist_xy((s.xs[cp] << 16) | s.ys[cp], xy, dr);
+    }
+    s.xy[p] = xy;
+    s.w[p] = w;
+    tsum += w;
+    qsum += (int64_t)p * w;
+  }
+  sh[tid] = tsum;
+  sh[kBlock + tid] = qsum;
+  __syncthreads();
+  if (tid == 0) {  // exclusive scan of the per-thread totals
+    int64_t at = 0, aq = 0;
+    for (int t = 0; t < kBlock; ++t) {
+      const int64_t vt = sh[t], vq = sh[kBlock + t];
+      sh[t] = at;
+      sh[kBlock + t] = aq;
+      at += vt;
+      aq += vq;
+    }
+    sh[2 * kBlock] = at;  // T[N-1]
+  }
+  __syncthreads();
+  int64_t T = sh[tid], Q = sh[kBlock + tid], lsum = 0;
+  for (int p = p0; p < p1; ++p) {
+    const int c = tour[p];
+    int w = 0;
+    if (p > 0) {
+      const int cp = tour[p - 1];
+      w = dist_xy((s.xs[cp] << 16) | s.ys[cp], (s.xs[c] << 16) | s.ys[c], dr);
+    }
+    T += w;
+    Q += (int64_t)p * w;
+    s.t64[p] = T;
+    s.q64[p] = Q;
+    s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
+    s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
+    lsum += T;  // L = sum_p T[p]
+  }
+  __syncthreads();
+  sh[tid] = lsum;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int64_t L = sh[0];
+    const int64_t Ttot = sh[2 * kBlock];
+    // every |delta| <= N (Ttot + 6 maxd) + 3 Ttot (see the delta forms)
+    const int64_t bound = (int64_t)N * (Ttot + 6 * (int64_t)s.maxd) + 3 * Ttot;
+    s.info[MLP_INFO_LATENCY] = L;
+    s.info[MLP_INFO_TTOT] = Ttot;
+    s.info[MLP_INFO_PATH32] = bound < 2147483647LL ? 1 : 0;
+  }
+  __syncthreads();
+}
+
+}  // namespace
+
+// ===========================================================================
+// kernels
+// ===========================================================================
+
+__global__ __launch_bounds__(kBlock) void k_prep(const SlotDev* __restrict__ table,
+                                                 SlotListInline list, const int32_t* list_dev) {
+  __shared__ int64_t sh[2 * kBlock + 4];
+  const int id = list_at(list, list_dev, blockIdx.x);
+  const SlotDev& s = table[id];
+  prep_wg(s, s.tour, sh);
+  if (threadIdx.x == 0) {
+    s.info[MLP_INFO_ROUND_MOVES] = 0;
+    s.info[MLP_INFO_TOTAL_MOVES] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_round_reset(const SlotDev* __restrict__ table,
+                                                        SlotListInline list,
+                                                        const int32_t* list_dev, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) {
+    const int id = list_at(list, list_dev, t);
+    table[id].info[MLP_INFO_ROUND_MOVES] = 0;
+  }
+}
+
+// One workgroup per work item (persistent == nullptr) or a persistent grid
+// draining the item counter.
+__global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ table,
+                                                 EvalSegsInline inl, const EvalSeg* segs_dev,
+                                                 int nsegs, int total, int rowmap,
+                                                 int* counter) {
+  if (!counter) {
+    const int item = blockIdx.x;
+    const EvalSeg sg = segs_dev ? find_seg(segs_dev, nsegs, item) : find_seg_inline(inl, nsegs, item);
+    eval_item(table, sg, item, rowmap);
+    return;
+  }
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const EvalSeg sg = segs_dev ? find_seg(segs_dev, nsegs, item) : find_seg_inline(inl, nsegs, item);
+    eval_item(table, sg, item, rowmap);
+  }
+}
+
+namespace {
+
+// Conflict predicates of one neighborhood (mlkernel.cu:51-71).
+__device__ __forceinline__ bool conflicts(int nb, int k, int i1, int j1, int i2, int j2) {
+  if (nb == 0)
+    return !((abs(i1 - i2) > 1) && (abs(i1 - j2) > 1) && (abs(j1 - i2) > 1) && (abs(j1 - j2) > 1));
+  if (nb == 1) return !((j1 < i2 - 1) || (i1 > j2 + 1) || (j2 < i1 - 1) || (i2 > j1 + 1));
+  const int mn1 = min(i1, j1), mx1 = max(i1, j1), mn2 = min(i2, j2), mx2 = max(i2, j2);
+  return !((mx1 + k < mn2) || (mn1 > mx2 + k));
+}
+
+__device__ __forceinline__ bool bits_any(const uint32_t* bits, int a, int b) {
+  const int wa = a >> 5, wb = b >> 5;
+  for (int w = wa; w <= wb; ++w) {
+    uint32_t m = 0xffffffffu;
+    if (w == wa) m &= 0xffffffffu << (a & 31);
+    if (w == wb) m &= 0xffffffffu >> (31 - (b & 31));
+    if (bits[w] & m) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void bits_set(uint32_t* bits, int a, int b) {
+  const int wa = a >> 5, wb = b >> 5;
+  for (int w = wa; w <= wb; ++w) {
+    uint32_t m = 0xffffffffu;
+    if (w == wa) m &= 0xffffffffu << (a & 31);
+    if (w == wb) m &= 0xffffffffu >> (31 - (b & 31));
+    atomicOr(&bits[w], m);
+  }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+}  // namespace
+
+// One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits.
+__global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
+                                                        SlotListInline list,
+                                                        const int32_t* list_dev, int nb,
+                                                        int rowmap, int pcap, int ncap) {
+  extern __shared__ int64_t lds[];
+  __shared__ int s_m, s_kept;
+  __shared__ int64_t sh[2 * kBlock + 4];
+  const int id = list_at(list, list_dev, blockIdx.x);
+  const SlotDev& s = table[id];
+  const int N = s.N, tid = threadIdx.x;
+  const int R = rows_of(nb, rowmap, N);
+  const int k = nb_k(nb);
+  int64_t* keys = lds;
+  int32_t* told = (int32_t*)(lds + pcap);
+  int32_t* tnew = told + ncap;
+  uint32_t* bits = (uint32_t*)(tnew + ncap);
+  const int nwords = (N + 32) / 32 + 1;
+
+  if (tid == 0) s_m = 0;
+  for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
+  __syncthreads();
+  // 1. improving rows -> (cost, row) keys
+  const int64_t* rk = s.rows[nb];
+  for (int r = tid; r < R; r += kBlock) {
+    const int64_t key = rk[r];
+    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
+      const int at = atomicAdd(&s_m, 1);
+      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + r;
+    }
+  }
+  __syncthreads();
+  const int M = s_m;
+  if (M == 0) {
+    if (tid == 0) s.info[MLP_INFO_KEPT0 + nb] = 0;
+    return;
+  }
+  int P = 1;
+  while (P < M) P <<= 1;
+  for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
+  __syncthreads();
+  // 2. bitonic sort by (cost, row) == stable sort by cost of the row list
+  for (int kk = 2; kk <= P; kk <<= 1) {
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      for (int t = tid; t < P; t += kBlock) {
+        const int u = t ^ jj;
+        if (u > t) {
+          const int64_t a = keys[t], b = keys[u];
+          const bool up = (t & kk) == 0;
+          if (up ? (a > b) : (a < b)) {
+            keys[t] = b;
+            keys[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // 3. greedy filter, 64 candidates per step on wave 0
+  if (tid < kWave) {
+    const int lane = tid;
+    int kept_total = 0;
+    for (int base = 0; base < M; base += kWave) {
+      const int t = base + lane;
+      const bool valid = t < M;
+      int i = 0, j = 0, row = 0;
+      int64_t cost = 0;
+      if (valid) {
+        const int64_t sk = keys[t];
+        cost = sk >> MLP_KEY_SHIFT;
+        row = (int)(sk & MLP_KEY_MASK);
+        const int ctx = (int)(rk[row] & MLP_KEY_MASK);
+        move_ij(nb, rowmap, N, row, ctx, i, j);
+      }
+      bool pre = valid;
+      if (valid) {
+        if (nb == 0) {
+          pre = !(((bits[i >> 5] >> (i & 31)) & 1u) || ((bits[j >> 5] >> (j & 31)) & 1u));
+        } else {
+          const int a = nb == 1 ? i : min(i, j);
+          const int b = nb == 1 ? j + 1 : max(i, j) + k;
+          pre = !bits_any(bits, a, b);
+        }
+      }
+      uint64_t cm = 0;
+      for (int q = 0; q < kWave; ++q) {
+        const int iq = __shfl(i, q, kWave), jq = __shfl(j, q, kWave);
+        if (q < lane && conflicts(nb, k, iq, jq, i, j)) cm |= 1ull << q;
+      }
+      const uint64_t pre_mask = __ballot(pre);
+      uint64_t kept = 0;
+      for (int q = 0; q < kWave; ++q) {
+        if ((pre_mask >> q) & 1ull) {
+          const uint64_t cq = readlane64(cm, q);
+          if ((cq & kept) == 0) kept |= 1ull << q;
+        }
+      }
+      const bool mine = (kept >> lane) & 1ull;
+      if (mine) {
+        if (nb == 0) {
+          bits_set(bits, i - 1, i + 1);
+          bits_set(bits, j - 1, j + 1);
+        } else {
+          const int a = nb == 1 ? i : min(i, j);
+          const int b = nb == 1 ? j + 1 : max(i, j) + k;
+          bits_set(bits, a, b);
+        }
+        const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
+        keys[q] = ((int64_t)i << 32) | (uint32_t)j;  // q <= t: already consumed
+        s.kept_cost[nb][q] = cost;
+        s.kept_ij[nb][3 * q + 0] = i;
+        s.kept_ij[nb][3 * q + 1] = j;
+        s.kept_ij[nb][3 * q + 2] = row;
+      }
+      kept_total += __popcll(kept);
+    }
+    if (lane == 0) s_kept = kept_total;
+  }
+  __syncthreads();
+  const int K = s_kept;
+  // 4. apply the kept moves (disjoint regions) through a position -> source map
+  for (int p = tid; p < N; p += kBlock) {
+    const int c = s.tour[p];
+    told[p] = c;
+    tnew[p] = c;
+  }
+  __syncthreads();
+  for (int q = 0; q < K; ++q) {
+    const int64_t v = keys[q];
+    const int i = (int)(v >> 32), j = (int)(uint32_t)v;
+    if (nb == 0) {
+      if (tid == 0) {
+        tnew[i] = told[j];
+        tnew[j] = told[i];
+      }
+    } else if (nb == 1) {
+      for (int p = i + tid; p <= j; p += kBlock) tnew[p] = told[i + j - p];
+    } else if (i < j) {
+      for (int p = i + tid; p < j + k; p += kBlock) tnew[p] = p < j ? told[p + k] : told[i + p - j];
+    } else {
+      for (int p = j + tid; p < i + k; p += kBlock) tnew[p] = p < j + k ? told[i + p - j] : told[p - k];
+    }
+  }
+  __syncthreads();
+  for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
+  prep_wg(s, tnew, sh);
+  if (tid == 0) {
+    s.info[MLP_INFO_KEPT0 + nb] = K;
+    s.info[MLP_INFO_ROUND_MOVES] += K;
+    s.info[MLP_INFO_TOTAL_MOVES] += K;
+  }
+}
+
+// The reference's ADS buffer: header, coords, [client|w], T, C (uint32 wrap).
+// S (prefix of T, uint32) is produced first by k_ads_prefix into scratch.
+__global__ void k_ads_prefix(const SlotDev* __restrict__ table, int id, uint32_t* S) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const SlotDev& s = table[id];
+    uint32_t acc = 0;
+    for (int p = 0; p < s.N; ++p) {
+      acc += (uint32_t)s.t32[p];
+      S[p] = acc;
+    }
+  }
+}
+
+__global__ void k_ads_fill(const SlotDev* __restrict__ table, int id, const uint32_t* S,
+                           uint32_t* out) {
+  const SlotDev& s = table[id];
+  const int N = s.N;
+  const int64_t total = (int64_t)N * N;
+  uint32_t* hdr = out;
+  uint32_t* coords = out + 32;
+  uint32_t* sol = coords + N;
+  uint32_t* T = sol + N;
+  uint32_t* C = T + N;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (g < 32) {
+    uint32_t v = 0;
+    if (g == 0) v = (uint32_t)(128 + 4 * (int64_t)N * (N + 3));
+    else if (g == 1 || g == 2) v = (uint32_t)N;
+    else if (g == 3) v = (uint32_t)(uint64_t)s.info[MLP_INFO_LATENCY];
+    else if (g == 4) v = 1;
+    else if (g == 5) v = (uint32_t)s.round;
+    hdr[g] = v;
+  }
+  for (int64_t p = g; p < N; p += stride) {
+    coords[p] = (uint32_t)s.xy[p];
+    sol[p] = p == 0 ? (uint32_t)s.tour[0] : (((uint32_t)s.tour[p] << 16) | ((uint32_t)s.w[p] & 0xffffu));
+    T[p] = (uint32_t)s.t32[p];
+  }
+  for (int64_t e = g; e < total; e += stride) {
+    const int i = (int)(e / N), j = (int)(e % N);
+    uint32_t v;
+    const uint32_t Ti = (uint32_t)s.t32[i];
+    if (i < j) v = (S[j] - S[i]) - (uint32_t)(j - i) * Ti;
+    else if (i > j) v = (uint32_t)(i - j) * Ti - (S[i - 1] - (j > 0 ? S[j - 1] : 0u));
+    else v = 0;
+    C[e] = v;
+  }
+}
+
+__global__ void k_dist_table(int s0, int count, int round, int32_t* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) out[q] = dist_s(s0 + q, round_dr(round));
+}
+
+__global__ void k_dist_list(const int32_t* s, int count, int round, int32_t* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) out[q] = dist_s(s[q], round_dr(round));
+}
+
+__global__ void k_weights(const int32_t* xs, const int32_t* ys, int n, int round, int32_t* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (int64_t)n * n) {
+    const int a = (int)(e / n), b = (int)(e % n);
+    out[e] = dist_xy((xs[a] << 16) | ys[a], (xs[b] << 16) | ys[b], round_dr(round));
+  }
+}
+
+__global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* ids, int count,
+                              int word, int64_t* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[t] = table[ids[t]].info[word];
+}
+
+// ===========================================================================
+// host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
+// ===========================================================================
+#include "mlp_launch.h"
+
+hipError_t mlp_launch_prep(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                           const int32_t* list_dev, int count) {
+  hipLaunchKernelGGL(k_prep, dim3(count), dim3(kBlock), 0, st, table, inl, list_dev);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_round_reset(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                  const int32_t* list_dev, int count) {
+  hipLaunchKernelGGL(k_round_reset, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     table, inl, list_dev, count);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsInline& inl,
+                           const EvalSeg* segs_dev, int nsegs, int total, int rowmap,
+                           int* counter, int grid) {
+  hipLaunchKernelGGL(k_eval, dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs, total,
+                     rowmap, counter);
+  return hipGetLastError();
+}
+
+size_t mlp_merge_lds_bytes(int pcap, int ncap) {
+  const int nwords = (ncap + 32) / 32 + 1;
+  return (size_t)pcap * 8 + (size_t)ncap * 8 + (size_t)nwords * 4;
+}
+
+hipError_t mlp_merge_setup(size_t max_dyn_lds) {
+  return hipFuncSetAttribute((const void*)k_merge_apply,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+}
+
+hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                  const int32_t* list_dev, int count, int nb, int rowmap,
+                                  int pcap, int ncap) {
+  hipLaunchKernelGGL(k_merge_apply, dim3(count), dim3(kBlock), mlp_merge_lds_bytes(pcap, ncap),
+                     st, table, inl, list_dev, nb, rowmap, pcap, ncap);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, uint32_t* S,
+                          uint32_t* out) {
+  hipLaunchKernelGGL(k_ads_prefix, dim3(1), dim3(64), 0, st, table, id, S);
+  const int64_t total = (int64_t)N * N;
+  int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_ads_fill, dim3(grid), dim3(kBlock), 0, st, table, id, S, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out) {
+  hipLaunchKernelGGL(k_dist_table, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s0,
+                     count, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_dist_list(hipStream_t st, const int32_t* s, int count, int round,
+                                int32_t* out) {
+  hipLaunchKernelGGL(k_dist_list, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s,
+                     count, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* ys, int n,
+                              int round, int32_t* out) {
+  const int64_t total = (int64_t)n * n;
+  hipLaunchKernelGGL(k_weights, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     st, xs, ys, n, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_gather_info(hipStream_t st, const SlotDev* table, const int32_t* ids,
+                                  int count, int word, int64_t* out) {
+  hipLaunchKernelGGL(k_gather_info, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     table, ids, count, word, out);
+  return hipGetLastError();
+}

@@ -1,0 +1,850 @@
+// mlp_runtime.cpp -- host runtime behind the C ABI in include/mlp_gpu.h.
+//
+// Owns the HIP stream, the instance and slot tables, the device copy of the
+// slot table read by the kernels, and the caches of device-side work lists.
+// Every entry point returns a status and records a message; nothing throws
+// across the ABI and nothing calls exit().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "mlp_gpu.h"
+#include "mlp_internal.h"
+#include "mlp_launch.h"
+
+namespace {
+
+struct Instance {
+  bool alive = false;
+  int n = 0, round = 0, maxd = 0;
+  int32_t* xs = nullptr;
+  int32_t* ys = nullptr;
+};
+
+struct Slot {
+  bool alive = false;
+  int inst = -1;
+  int N = 0;
+  void* mem = nullptr;
+  SlotDev dev{};
+  int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
+  bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
+};
+
+struct DevList {
+  void* p = nullptr;
+};
+
+int rows_of(int nb, int rowmap, int N) {
+  if (nb == 0) return rowmap == 0 ? N - 4 : ((N + 1) / 2) - 1;
+  if (nb == 1) return (N + 1) / 2;
+  return N - (nb - 1) - 2;
+}
+
+void move_ij(int nb, int rowmap, int N, int row, int ctx, int& i, int& j) {
+  if (nb == 0) {
+    if (rowmap == 0) { i = row + 1; j = ctx; }
+    else if (ctx >= row) { i = row + 1; j = ctx + 3; }
+    else { i = N - row - 3; j = N - ctx - 2; }
+  } else if (nb == 1) {
+    if (ctx >= row) { i = ctx - row + 1; j = ctx + 2; }
+    else { i = row - ctx; j = N - ctx - 2; }
+  } else {
+    const int k = nb - 1;
+    if (rowmap == 0) { i = ctx > row ? row + 1 : row + 2; j = ctx + 1; }
+    else if (ctx >= row) { i = ctx - row + 1; j = ctx + 2; }
+    else { i = N - ctx - k - 1; j = row - ctx; }
+  }
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Byte layout of one slot's device block.
+struct SlotLayout {
+  size_t tour, xy, w, t32, q32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
+      kij[MLP_NB_COUNT], total;
+};
+
+SlotLayout slot_layout(int N) {
+  SlotLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes, 256);
+    return at;
+  };
+  L.tour = take(4 * (size_t)N);
+  L.xy = take(4 * (size_t)N);
+  L.w = take(4 * (size_t)N);
+  L.t32 = take(4 * (size_t)N);
+  L.q32 = take(4 * (size_t)N);
+  L.t64 = take(8 * (size_t)N);
+  L.q64 = take(8 * (size_t)N);
+  L.info = take(8 * MLP_INFO_WORDS);
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    const size_t r = (size_t)std::max(1, std::max(rows_of(nb, 0, N), rows_of(nb, 1, N)));
+    L.rows[nb] = take(8 * r);
+    L.kc[nb] = take(8 * r);
+    L.kij[nb] = take(12 * r);
+  }
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+
+struct mlp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int64_t hbm = 0;
+  char name[256] = {0};
+  std::string err;
+  std::vector<Instance> insts;
+  std::vector<Slot> slots;
+  SlotDev* d_table = nullptr;
+  int table_cap = 0;
+  int* d_counter = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool eval_timed = false;
+  int32_t* h_stage = nullptr;  // pinned staging for tour uploads
+  size_t h_stage_cap = 0;
+  int64_t* d_i64 = nullptr;  // small device scratch for gathers
+  int64_t* h_i64 = nullptr;  // pinned
+  size_t i64_cap = 0;
+  std::unordered_map<std::string, DevList> lists;
+  size_t max_merge_lds = 0;
+  // per-launch timing of k_eval (HIP events on the context stream)
+  bool timing = false;
+  std::vector<hipEvent_t> tev;  // pairs
+  size_t tcount = 0;
+};
+
+namespace {
+
+int fail(mlp_ctx* c, int st, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return st;
+}
+
+#define HCK(call)                                                                             \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(ctx, MLP_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+bool slot_ok(mlp_ctx* ctx, int s) {
+  return s >= 0 && s < (int)ctx->slots.size() && ctx->slots[s].alive;
+}
+
+// Device copy of an int32 list, uploaded once per distinct content.
+int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) {
+  std::string key((const char*)v, (size_t)count * 4);
+  auto it = ctx->lists.find(key);
+  if (it != ctx->lists.end()) {
+    *out = (const int32_t*)it->second.p;
+    return MLP_OK;
+  }
+  if (ctx->lists.size() >= 8192) {
+    HCK(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->lists) hipFree(kv.second.p);
+    ctx->lists.clear();
+  }
+  void* p = nullptr;
+  HCK(hipMalloc(&p, std::max<size_t>(4, (size_t)count * 4)));
+  HCK(hipMemcpy(p, v, (size_t)count * 4, hipMemcpyHostToDevice));
+  ctx->lists.emplace(std::move(key), DevList{p});
+  *out = (const int32_t*)p;
+  return MLP_OK;
+}
+
+// Slot-id list for a per-slot launch: inline kernel argument when small.
+int slot_list(mlp_ctx* ctx, const int32_t* slots, int count, SlotListInline& inl,
+              const int32_t** dev) {
+  *dev = nullptr;
+  if (count <= MLP_SLOTS_INLINE) {
+    for (int q = 0; q < count; ++q) inl.id[q] = slots[q];
+    return MLP_OK;
+  }
+  return device_list(ctx, slots, count, dev);
+}
+
+int ensure_i64(mlp_ctx* ctx, size_t count) {
+  if (count <= ctx->i64_cap) return MLP_OK;
+  HCK(hipStreamSynchronize(ctx->stream));
+  if (ctx->d_i64) hipFree(ctx->d_i64);
+  if (ctx->h_i64) hipHostFree(ctx->h_i64);
+  ctx->d_i64 = nullptr;
+  ctx->h_i64 = nullptr;
+  ctx->i64_cap = 0;
+  const size_t cap = std::max<size_t>(count, 1024);
+  HCK(hipMalloc(&ctx->d_i64, cap * 8));
+  HCK(hipHostMalloc(&ctx->h_i64, cap * 8, hipHostMallocDefault));
+  ctx->i64_cap = cap;
+  return MLP_OK;
+}
+
+int check_slots(mlp_ctx* ctx, const int32_t* slots, int count) {
+  if (!ctx) return MLP_EINVAL;
+  if (count < 0 || (count > 0 && !slots)) return fail(ctx, MLP_EINVAL, "bad slot list");
+  for (int q = 0; q < count; ++q)
+    if (!slot_ok(ctx, slots[q])) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slots[q]);
+  return MLP_OK;
+}
+
+int grow_table(mlp_ctx* ctx, int need) {
+  if (need <= ctx->table_cap) return MLP_OK;
+  int cap = std::max(64, ctx->table_cap);
+  while (cap < need) cap *= 2;
+  SlotDev* t = nullptr;
+  HCK(hipStreamSynchronize(ctx->stream));
+  HCK(hipMalloc(&t, sizeof(SlotDev) * (size_t)cap));
+  HCK(hipMemset(t, 0, sizeof(SlotDev) * (size_t)cap));
+  if (ctx->d_table) {
+    HCK(hipMemcpy(t, ctx->d_table, sizeof(SlotDev) * (size_t)ctx->table_cap,
+                  hipMemcpyDeviceToDevice));
+    HCK(hipFree(ctx->d_table));
+  }
+  ctx->d_table = t;
+  ctx->table_cap = cap;
+  return MLP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mlp_strerror(int st) {
+  switch (st) {
+    case MLP_OK: return "ok";
+    case MLP_EINVAL: return "invalid argument";
+    case MLP_ENOMEM: return "out of memory";
+    case MLP_EDEVICE: return "HIP runtime error";
+    case MLP_ERANGE: return "value out of supported range";
+    case MLP_ESTATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+const char* mlp_ctx_error(const mlp_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int mlp_ctx_create(int device, mlp_ctx** out) {
+  if (!out) return MLP_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MLP_EDEVICE;
+  if (device < 0 || device >= ndev) return MLP_EINVAL;
+  mlp_ctx* ctx = new (std::nothrow) mlp_ctx();
+  if (!ctx) return MLP_ENOMEM;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+      hipEventCreate(&ctx->ev1) != hipSuccess || hipMalloc(&ctx->d_counter, 64) != hipSuccess) {
+    delete ctx;
+    return MLP_EDEVICE;
+  }
+  ctx->cu_count = prop.multiProcessorCount;
+  ctx->hbm = (int64_t)prop.totalGlobalMem;
+  snprintf(ctx->name, sizeof ctx->name, "%s", prop.gcnArchName);
+  // dynamic LDS of the merge kernel: up to the device's per-block limit
+  int lds_attr = 0;
+  if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+      hipSuccess)
+    lds_attr = 64 * 1024;
+  size_t lds = (size_t)lds_attr;
+  if (lds > 160 * 1024) lds = 160 * 1024;
+  ctx->max_merge_lds = lds > 8192 ? lds - 8192 : lds;
+  if (mlp_merge_setup(ctx->max_merge_lds) != hipSuccess) ctx->max_merge_lds = 64 * 1024 - 8192;
+  *out = ctx;
+  return MLP_OK;
+}
+
+int mlp_ctx_destroy(mlp_ctx* ctx) {
+  if (!ctx) return MLP_EINVAL;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& s : ctx->slots)
+    if (s.mem) hipFree(s.mem);
+  for (auto& in : ctx->insts) {
+    if (in.xs) hipFree(in.xs);
+    if (in.ys) hipFree(in.ys);
+  }
+  for (auto& kv : ctx->lists) hipFree(kv.second.p);
+  if (ctx->d_table) hipFree(ctx->d_table);
+  if (ctx->d_counter) hipFree(ctx->d_counter);
+  if (ctx->d_i64) hipFree(ctx->d_i64);
+  if (ctx->h_i64) hipHostFree(ctx->h_i64);
+  if (ctx->h_stage) hipHostFree(ctx->h_stage);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  for (hipEvent_t e : ctx->tev) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MLP_OK;
+}
+
+int mlp_sync(mlp_ctx* ctx) {
+  if (!ctx) return MLP_EINVAL;
+  HCK(hipStreamSynchronize(ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* name,
+                    int32_t name_len) {
+  if (!ctx) return MLP_EINVAL;
+  if (cu_count) *cu_count = ctx->cu_count;
+  if (hbm_bytes) *hbm_bytes = ctx->hbm;
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", ctx->name);
+  return MLP_OK;
+}
+
+int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                        int32_t round, int32_t* inst) {
+  if (!ctx || !inst) return MLP_EINVAL;
+  if (!xs || !ys || n < 2) return fail(ctx, MLP_EINVAL, "need at least 2 clients");
+  if (n + 1 > MLP_MAX_N) return fail(ctx, MLP_ERANGE, "n = %d exceeds %d", n, MLP_MAX_N - 1);
+  if (round != 0 && round != 1) return fail(ctx, MLP_EINVAL, "round must be 0 or 1");
+  int xmin = INT32_MAX, xmax = INT32_MIN, ymin = INT32_MAX, ymax = INT32_MIN;
+  for (int c = 0; c < n; ++c) {
+    if (xs[c] < 0 || ys[c] < 0 || xs[c] > 32767 || ys[c] > 32767)
+      return fail(ctx, MLP_ERANGE, "coordinate of client %d outside [0, 32767]", c);
+    xmin = std::min(xmin, xs[c]);
+    xmax = std::max(xmax, xs[c]);
+    ymin = std::min(ymin, ys[c]);
+    ymax = std::max(ymax, ys[c]);
+  }
+  Instance in;
+  in.n = n;
+  in.round = round;
+  const double dx = xmax - xmin, dy = ymax - ymin;
+  in.maxd = (int)std::ceil(std::sqrt(dx * dx + dy * dy)) + 1;
+  HCK(hipSetDevice(ctx->device));
+  HCK(hipMalloc(&in.xs, 4 * (size_t)n));
+  HCK(hipMalloc(&in.ys, 4 * (size_t)n));
+  HCK(hipMemcpy(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice));
+  HCK(hipMemcpy(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice));
+  in.alive = true;
+  for (size_t q = 0; q < ctx->insts.size(); ++q)
+    if (!ctx->insts[q].alive) {
+      ctx->insts[q] = in;
+      *inst = (int32_t)q;
+      return MLP_OK;
+    }
+  ctx->insts.push_back(in);
+  *inst = (int32_t)ctx->insts.size() - 1;
+  return MLP_OK;
+}
+
+int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
+  if (!ctx) return MLP_EINVAL;
+  if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
+    return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
+  for (auto& s : ctx->slots)
+    if (s.alive && s.inst == inst)
+      return fail(ctx, MLP_ESTATE, "instance %d still has slots", inst);
+  HCK(hipStreamSynchronize(ctx->stream));
+  Instance& in = ctx->insts[inst];
+  hipFree(in.xs);
+  hipFree(in.ys);
+  in = Instance();
+  return MLP_OK;
+}
+
+int mlp_instance_weights(mlp_ctx* ctx, int32_t inst, int32_t* w_out) {
+  if (!ctx || !w_out) return MLP_EINVAL;
+  if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
+    return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
+  const Instance& in = ctx->insts[inst];
+  const size_t bytes = 4 * (size_t)in.n * in.n;
+  int32_t* d = nullptr;
+  HCK(hipMalloc(&d, bytes));
+  hipError_t e = mlp_launch_weights(ctx->stream, in.xs, in.ys, in.n, in.round, d);
+  if (e == hipSuccess) e = hipMemcpyAsync(w_out, d, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "weights: %s", hipGetErrorString(e));
+  return MLP_OK;
+}
+
+int64_t mlp_slot_footprint(int32_t n) {
+  if (n < 2 || n + 1 > MLP_MAX_N) return -1;
+  return (int64_t)slot_layout(n + 1).total;
+}
+
+int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
+  if (!ctx || !slot) return MLP_EINVAL;
+  if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
+    return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
+  const Instance& in = ctx->insts[inst];
+  const int N = in.n + 1;
+  const SlotLayout L = slot_layout(N);
+  int id = -1;
+  for (size_t q = 0; q < ctx->slots.size(); ++q)
+    if (!ctx->slots[q].alive) {
+      id = (int)q;
+      break;
+    }
+  if (id < 0) {
+    id = (int)ctx->slots.size();
+    ctx->slots.emplace_back();
+  }
+  int st = grow_table(ctx, id + 1);
+  if (st) return st;
+  Slot s;
+  s.inst = inst;
+  s.N = N;
+  if (hipMalloc(&s.mem, L.total) != hipSuccess)
+    return fail(ctx, MLP_ENOMEM, "slot allocation of %zu bytes failed", L.total);
+  HCK(hipMemset(s.mem, 0, L.total));
+  char* b = (char*)s.mem;
+  SlotDev& d = s.dev;
+  d.N = N;
+  d.n = in.n;
+  d.round = in.round;
+  d.maxd = in.maxd;
+  d.xs = in.xs;
+  d.ys = in.ys;
+  d.tour = (int32_t*)(b + L.tour);
+  d.xy = (int32_t*)(b + L.xy);
+  d.w = (int32_t*)(b + L.w);
+  d.t32 = (int32_t*)(b + L.t32);
+  d.q32 = (int32_t*)(b + L.q32);
+  d.t64 = (int64_t*)(b + L.t64);
+  d.q64 = (int64_t*)(b + L.q64);
+  d.info = (int64_t*)(b + L.info);
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    d.rows[nb] = (int64_t*)(b + L.rows[nb]);
+    d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
+    d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
+  }
+  HCK(hipMemcpy(ctx->d_table + id, &d, sizeof(SlotDev), hipMemcpyHostToDevice));
+  s.alive = true;
+  ctx->slots[id] = s;
+  *slot = id;
+  return MLP_OK;
+}
+
+int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot) {
+  if (!ctx) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  HCK(hipStreamSynchronize(ctx->stream));
+  hipFree(ctx->slots[slot].mem);
+  ctx->slots[slot] = Slot();
+  return MLP_OK;
+}
+
+int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32_t* tours) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (count == 0) return MLP_OK;
+  if (!tours) return fail(ctx, MLP_EINVAL, "tours is NULL");
+  size_t total = 0;
+  std::vector<char> seen;
+  for (int q = 0; q < count; ++q) {
+    const Slot& s = ctx->slots[slots[q]];
+    const int32_t* t = tours + total;
+    const int N = s.N, n = N - 1;
+    if (t[0] != 0 || t[N - 1] != 0)
+      return fail(ctx, MLP_EINVAL, "tour %d must start and end at the depot 0", q);
+    seen.assign((size_t)n, 0);
+    for (int p = 1; p < N - 1; ++p) {
+      if (t[p] <= 0 || t[p] >= n || seen[(size_t)t[p]])
+        return fail(ctx, MLP_EINVAL, "tour %d is not a permutation (position %d)", q, p);
+      seen[(size_t)t[p]] = 1;
+    }
+    total += (size_t)N;
+  }
+  HCK(hipStreamSynchronize(ctx->stream));
+  if (total > ctx->h_stage_cap) {
+    if (ctx->h_stage) hipHostFree(ctx->h_stage);
+    ctx->h_stage = nullptr;
+    ctx->h_stage_cap = 0;
+    HCK(hipHostMalloc(&ctx->h_stage, total * 4, hipHostMallocDefault));
+    ctx->h_stage_cap = total;
+  }
+  memcpy(ctx->h_stage, tours, total * 4);
+  size_t off = 0;
+  for (int q = 0; q < count; ++q) {
+    Slot& s = ctx->slots[slots[q]];
+    HCK(hipMemcpyAsync(s.dev.tour, ctx->h_stage + off, 4 * (size_t)s.N, hipMemcpyHostToDevice,
+                       ctx->stream));
+    off += (size_t)s.N;
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      s.eval_rowmap[nb] = -1;
+      s.merged[nb] = false;
+    }
+  }
+  for (int q0 = 0; q0 < count; q0 += 65535) {
+    const int c = std::min(65535, count - q0);
+    SlotListInline inl;
+    const int32_t* dev = nullptr;
+    st = slot_list(ctx, slots + q0, c, inl, &dev);
+    if (st) return st;
+    HCK(mlp_launch_prep(ctx->stream, ctx->d_table, inl, dev, c));
+  }
+  return MLP_OK;
+}
+
+int mlp_get_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* tours) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (count && !tours) return fail(ctx, MLP_EINVAL, "tours is NULL");
+  HCK(hipStreamSynchronize(ctx->stream));
+  size_t off = 0;
+  for (int q = 0; q < count; ++q) {
+    const Slot& s = ctx->slots[slots[q]];
+    HCK(hipMemcpy(tours + off, s.dev.tour, 4 * (size_t)s.N, hipMemcpyDeviceToHost));
+    off += (size_t)s.N;
+  }
+  return MLP_OK;
+}
+
+static int gather_info(mlp_ctx* ctx, const int32_t* slots, int count, int word, int64_t* out) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (count == 0) return MLP_OK;
+  st = ensure_i64(ctx, (size_t)count);
+  if (st) return st;
+  const int32_t* dev = nullptr;
+  st = device_list(ctx, slots, count, &dev);
+  if (st) return st;
+  HCK(mlp_launch_gather_info(ctx->stream, ctx->d_table, dev, count, word, ctx->d_i64));
+  HCK(hipMemcpyAsync(ctx->h_i64, ctx->d_i64, 8 * (size_t)count, hipMemcpyDeviceToHost,
+                     ctx->stream));
+  HCK(hipStreamSynchronize(ctx->stream));
+  memcpy(out, ctx->h_i64, 8 * (size_t)count);
+  return MLP_OK;
+}
+
+int mlp_get_latency(mlp_ctx* ctx, const int32_t* slots, int32_t count, int64_t* latency) {
+  if (!latency && count) return fail(ctx, MLP_EINVAL, "latency is NULL");
+  return gather_info(ctx, slots, count, MLP_INFO_LATENCY, latency);
+}
+
+int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+             int32_t rowmap, int32_t persistent) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  std::vector<EvalSeg> segs;
+  segs.reserve((size_t)count * 5);
+  int total = 0;
+  for (int q = 0; q < count; ++q) {
+    Slot& s = ctx->slots[slots[q]];
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      if (!(nb_mask & (1u << nb))) continue;
+      s.eval_rowmap[nb] = rowmap;
+      s.merged[nb] = false;
+      const int R = rows_of(nb, rowmap, s.N);
+      if (R <= 0) continue;
+      segs.push_back(EvalSeg{slots[q], nb, R, total});
+      total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
+    }
+  }
+  if (segs.empty()) return MLP_OK;
+  EvalSegsInline inl;
+  const EvalSeg* dev = nullptr;
+  const int nsegs = (int)segs.size();
+  if (nsegs <= MLP_SEGS_INLINE) {
+    for (int q = 0; q < nsegs; ++q) inl.s[q] = segs[q];
+  } else {
+    st = device_list(ctx, (const int32_t*)segs.data(), nsegs * 4, (const int32_t**)&dev);
+    if (st) return st;
+  }
+  int* counter = nullptr;
+  int grid = total;
+  if (persistent) {
+    counter = ctx->d_counter;
+    HCK(hipMemsetAsync(counter, 0, 4, ctx->stream));
+    grid = std::min(total, ctx->cu_count * 8);
+  }
+  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
+  if (ctx->timing) {
+    const size_t q = ctx->tcount % (ctx->tev.size() / 2);
+    if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
+    e0 = ctx->tev[2 * q];
+    e1 = ctx->tev[2 * q + 1];
+    ctx->tcount++;
+  }
+  HCK(hipEventRecord(e0, ctx->stream));
+  HCK(mlp_launch_eval(ctx->stream, ctx->d_table, inl, dev, nsegs, total, rowmap, counter, grid));
+  HCK(hipEventRecord(e1, ctx->stream));
+  ctx->eval_timed = true;
+  return MLP_OK;
+}
+
+int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity) {
+  if (!ctx || capacity <= 0) return MLP_EINVAL;
+  HCK(hipStreamSynchronize(ctx->stream));
+  while (ctx->tev.size() < 2 * (size_t)capacity) {
+    hipEvent_t e;
+    HCK(hipEventCreate(&e));
+    ctx->tev.push_back(e);
+  }
+  ctx->tcount = 0;
+  ctx->timing = true;
+  return MLP_OK;
+}
+
+int mlp_timing_end(mlp_ctx* ctx, double* total_ms, int32_t* launches) {
+  if (!ctx) return MLP_EINVAL;
+  HCK(hipStreamSynchronize(ctx->stream));
+  double tot = 0;
+  for (size_t q = 0; q < ctx->tcount; ++q) {
+    float ms = 0;
+    HCK(hipEventElapsedTime(&ms, ctx->tev[2 * q], ctx->tev[2 * q + 1]));
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = (int32_t)ctx->tcount;
+  ctx->timing = false;
+  ctx->tcount = 0;
+  return MLP_OK;
+}
+
+int mlp_last_eval_ms(mlp_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return MLP_EINVAL;
+  if (!ctx->eval_timed) return fail(ctx, MLP_ESTATE, "no eval launched");
+  HCK(hipEventSynchronize(ctx->ev1));
+  HCK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return MLP_OK;
+}
+
+int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t* i, int32_t* j,
+                 int32_t* rows) {
+  if (!ctx) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  const Slot& s = ctx->slots[slot];
+  const int rm = s.eval_rowmap[nb];
+  if (rm < 0) return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not evaluated", nb, slot);
+  const int R = std::max(0, rows_of(nb, rm, s.N));
+  if (rows) *rows = R;
+  if (R == 0) return MLP_OK;
+  std::vector<int64_t> keys((size_t)R);
+  HCK(hipStreamSynchronize(ctx->stream));
+  HCK(hipMemcpy(keys.data(), s.dev.rows[nb], 8 * (size_t)R, hipMemcpyDeviceToHost));
+  for (int r = 0; r < R; ++r) {
+    const int64_t k = keys[(size_t)r];
+    int ii = 0, jj = 0;
+    int64_t c = INT64_MAX;
+    if (k != MLP_KEY_NONE) {
+      c = k >> MLP_KEY_SHIFT;
+      move_ij(nb, rm, s.N, r, (int)(k & MLP_KEY_MASK), ii, jj);
+    }
+    if (cost) cost[r] = c;
+    if (i) i[r] = ii;
+    if (j) j[r] = jj;
+  }
+  return MLP_OK;
+}
+
+int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int32_t* rows) {
+  if (!ctx || !moves) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  if (ctx->slots[slot].N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j)");
+  std::vector<int64_t> c(MLP_MAX_N);
+  std::vector<int32_t> ii(MLP_MAX_N), jj(MLP_MAX_N);
+  int R = 0;
+  int st = mlp_get_rows(ctx, slot, nb, c.data(), ii.data(), jj.data(), &R);
+  if (st) return st;
+  for (int r = 0; r < R; ++r) {
+    int64_t cost = c[(size_t)r];
+    int32_t c32;
+    uint32_t lo;
+    if (cost == INT64_MAX) {  // empty row: COST_INFTY, move word 0 (mlkswap.cu:175-176)
+      c32 = 0x7fffffff;
+      lo = 0;
+    } else {
+      if (cost < INT32_MIN || cost >= INT32_MAX)
+        return fail(ctx, MLP_ERANGE, "row %d cost %lld does not fit int32", r, (long long)cost);
+      c32 = (int32_t)cost;
+      lo = ((uint32_t)jj[(size_t)r] << 18) | ((uint32_t)ii[(size_t)r] << 4) | (uint32_t)nb;
+    }
+    moves[r] = ((uint64_t)(uint32_t)c32 << 32) | lo;
+  }
+  if (rows) *rows = R;
+  return MLP_OK;
+}
+
+int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  if (count == 0) return MLP_OK;
+  int rowmap = -1, pcap = 1, ncap = 1;
+  for (int q = 0; q < count; ++q) {
+    Slot& s = ctx->slots[slots[q]];
+    const int rm = s.eval_rowmap[nb];
+    if (rm < 0)
+      return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not evaluated", nb, slots[q]);
+    if (s.merged[nb])
+      return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d already merged", nb, slots[q]);
+    if (rowmap >= 0 && rm != rowmap)
+      return fail(ctx, MLP_EINVAL, "slots evaluated with different row maps");
+    rowmap = rm;
+    const int R = std::max(1, rows_of(nb, rm, s.N));
+    if (R > MLP_MERGE_MAX_ROWS)
+      return fail(ctx, MLP_ERANGE, "merge supports at most %d rows", MLP_MERGE_MAX_ROWS);
+    while (pcap < R) pcap <<= 1;
+    ncap = std::max(ncap, s.N);
+  }
+  if (mlp_merge_lds_bytes(pcap, ncap) > ctx->max_merge_lds)
+    return fail(ctx, MLP_ERANGE, "merge of N = %d exceeds LDS", ncap);
+  for (int q = 0; q < count; ++q) ctx->slots[slots[q]].merged[nb] = true;
+  for (int q0 = 0; q0 < count; q0 += 65535) {
+    const int c = std::min(65535, count - q0);
+    SlotListInline inl;
+    const int32_t* dev = nullptr;
+    st = slot_list(ctx, slots + q0, c, inl, &dev);
+    if (st) return st;
+    HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap));
+  }
+  return MLP_OK;
+}
+
+int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int32_t cap,
+                    int32_t* count) {
+  if (!ctx || !count) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  const Slot& s = ctx->slots[slot];
+  if (!s.merged[nb]) return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not merged", nb, slot);
+  HCK(hipStreamSynchronize(ctx->stream));
+  int64_t K = 0;
+  HCK(hipMemcpy(&K, s.dev.info + MLP_INFO_KEPT0 + nb, 8, hipMemcpyDeviceToHost));
+  *count = (int32_t)K;
+  const int take = (int)std::min<int64_t>(K, cap);
+  if (take > 0 && moves) {
+    std::vector<int64_t> c((size_t)take);
+    std::vector<int32_t> ij(3 * (size_t)take);
+    HCK(hipMemcpy(c.data(), s.dev.kept_cost[nb], 8 * (size_t)take, hipMemcpyDeviceToHost));
+    HCK(hipMemcpy(ij.data(), s.dev.kept_ij[nb], 12 * (size_t)take, hipMemcpyDeviceToHost));
+    for (int q = 0; q < take; ++q)
+      moves[q] = mlp_move{c[(size_t)q], nb, ij[3 * (size_t)q], ij[3 * (size_t)q + 1],
+                          ij[3 * (size_t)q + 2]};
+  }
+  return MLP_OK;
+}
+
+int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
+             int32_t max_rounds, int32_t* rounds_out) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  std::vector<int32_t> active(slots, slots + count);
+  std::vector<int> pos(ctx->slots.size(), -1);
+  for (int q = 0; q < count; ++q) {
+    pos[(size_t)slots[q]] = q;
+    if (rounds_out) rounds_out[q] = 0;
+  }
+  std::vector<int64_t> moved;
+  int round = 0;
+  while (!active.empty() && (max_rounds <= 0 || round < max_rounds)) {
+    const int c = (int)active.size();
+    SlotListInline inl;
+    const int32_t* dev = nullptr;
+    st = slot_list(ctx, active.data(), c, inl, &dev);
+    if (st) return st;
+    HCK(mlp_launch_round_reset(ctx->stream, ctx->d_table, inl, dev, c));
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      bool any = false;
+      for (int32_t s : active)
+        if (rows_of(nb, rowmap, ctx->slots[s].N) > 0) any = true;
+      if (!any) continue;
+      st = mlp_eval(ctx, active.data(), c, 1u << nb, rowmap, 0);
+      if (st) return st;
+      st = mlp_merge_apply(ctx, active.data(), c, nb);
+      if (st) return st;
+    }
+    ++round;
+    moved.assign((size_t)c, 0);
+    st = gather_info(ctx, active.data(), c, MLP_INFO_ROUND_MOVES, moved.data());
+    if (st) return st;
+    std::vector<int32_t> next;
+    for (int q = 0; q < c; ++q) {
+      if (rounds_out) rounds_out[pos[(size_t)active[(size_t)q]]] = round;
+      if (moved[(size_t)q] > 0) next.push_back(active[(size_t)q]);
+    }
+    active.swap(next);
+  }
+  return MLP_OK;
+}
+
+int mlp_ads_materialize(mlp_ctx* ctx, int32_t slot, uint32_t* host_out, int64_t bytes) {
+  if (!ctx || !host_out) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  const int N = ctx->slots[slot].N;
+  const int64_t need = 128 + 4 * (int64_t)N * (N + 3);
+  if (bytes < need) return fail(ctx, MLP_EINVAL, "ADS buffer needs %lld bytes", (long long)need);
+  uint32_t *d = nullptr, *S = nullptr;
+  HCK(hipMalloc(&d, (size_t)need));
+  if (hipMalloc(&S, 4 * (size_t)N) != hipSuccess) {
+    hipFree(d);
+    return fail(ctx, MLP_ENOMEM, "ADS scratch");
+  }
+  hipError_t e = mlp_launch_ads(ctx->stream, ctx->d_table, slot, N, S, d);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host_out, d, (size_t)need, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(S);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "ADS: %s", hipGetErrorString(e));
+  return MLP_OK;
+}
+
+int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32_t* out) {
+  if (!ctx || !out || count < 0 || s0 < 0) return MLP_EINVAL;
+  if ((int64_t)s0 + count > INT32_MAX) return fail(ctx, MLP_ERANGE, "s range");
+  int32_t* d = nullptr;
+  HCK(hipMalloc(&d, std::max<size_t>(4, 4 * (size_t)count)));
+  hipError_t e = mlp_launch_dist_table(ctx->stream, s0, count, round, d);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d, 4 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "dist table: %s", hipGetErrorString(e));
+  return MLP_OK;
+}
+
+int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out) {
+  if (!ctx || !out || !s || count < 0) return MLP_EINVAL;
+  for (int q = 0; q < count; ++q)
+    if (s[q] < 0) return fail(ctx, MLP_ERANGE, "negative squared length");
+  int32_t *d = nullptr, *ds = nullptr;
+  const size_t bytes = std::max<size_t>(4, 4 * (size_t)count);
+  HCK(hipMalloc(&d, bytes));
+  if (hipMalloc(&ds, bytes) != hipSuccess) {
+    hipFree(d);
+    return fail(ctx, MLP_ENOMEM, "dist list");
+  }
+  hipError_t e = hipMemcpyAsync(ds, s, 4 * (size_t)count, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = mlp_launch_dist_list(ctx->stream, ds, count, round, d);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d, 4 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  hipFree(ds);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "dist list: %s", hipGetErrorString(e));
+  return MLP_OK;
+}
+
+}  // extern "C"

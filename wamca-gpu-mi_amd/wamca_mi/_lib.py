@@ -1,0 +1,102 @@
+"""Loader for the in-tree HIP library ``wamca-gpu-mi_amd/lib/libwamca_mi.so``.
+
+The library is the product: there is no CPU fallback.  If it is missing the
+import fails loudly with the command that builds it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libwamca_mi.so")
+
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+I64P = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+U64P = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+class MLPError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class MoveRec(C.Structure):
+    _fields_ = [("cost", C.c_int64), ("id", C.c_int32), ("i", C.c_int32), ("j", C.c_int32),
+                ("row", C.c_int32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mlp_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mlp_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "mlp_ctx_error": (C.c_char_p, [C.c_void_p]),
+    "mlp_strerror": (C.c_char_p, [C.c_int]),
+    "mlp_sync": (C.c_int, [C.c_void_p]),
+    "mlp_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                  C.c_char_p, C.c_int32]),
+    "mlp_instance_create": (C.c_int, [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32)]),
+    "mlp_instance_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_instance_weights": (C.c_int, [C.c_void_p, C.c_int32, I32P]),
+    "mlp_slot_footprint": (C.c_int64, [C.c_int32]),
+    "mlp_slot_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "mlp_slot_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_set_tours": (C.c_int, [C.c_void_p, I32P, C.c_int32, I32P]),
+    "mlp_get_tours": (C.c_int, [C.c_void_p, I32P, C.c_int32, I32P]),
+    "mlp_get_latency": (C.c_int, [C.c_void_p, I32P, C.c_int32, I64P]),
+    "mlp_eval": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
+    "mlp_get_rows": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, I64P, I32P, I32P,
+                               C.POINTER(C.c_int32)]),
+    "mlp_get_moves64": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, U64P,
+                                  C.POINTER(C.c_int32)]),
+    "mlp_merge_apply": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32]),
+    "mlp_get_applied": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(MoveRec),
+                                  C.c_int32, C.POINTER(C.c_int32)]),
+    "mlp_dvnd": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, C.c_int32, I32P]),
+    "mlp_ads_materialize": (C.c_int, [C.c_void_p, C.c_int32, U32P, C.c_int64]),
+    "mlp_dist_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, I32P]),
+    "mlp_dist_list": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, I32P]),
+    "mlp_last_eval_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "mlp_timing_begin": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_timing_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    # host-side inputs (include/mlp_host.h)
+    "mlp_mt_first": (C.c_uint32, [C.c_uint32, C.c_int32]),
+    "mlp_gen_uniform": (C.c_int, [C.c_int32, C.c_uint32, C.c_int32, I32P, I32P]),
+    "mlp_gen_clustered": (C.c_int, [C.c_int32, C.c_uint32, C.c_int32, C.c_int32, C.c_double,
+                                    I32P, I32P]),
+    "mlp_gen_loguniform_n": (C.c_int32, [C.c_uint32, C.c_int32, C.c_int32, C.c_int32]),
+    "mlp_tsplib_load": (C.c_int, [C.c_char_p, I32P, I32P, C.c_int32, C.POINTER(C.c_int32)]),
+    "mlp_shift_nonnegative": (C.c_int, [C.c_int32, I32P, I32P]),
+    "mlp_tour_perm": (C.c_int, [C.c_uint32, C.c_int32, I32P]),
+    "mlp_tour_random": (C.c_int, [I32P, I32P, C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
+                                  C.c_float, I32P]),
+}
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`make -C wamca-gpu-mi_amd/csrc` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Every name the C ABI headers declare, for the export check."""
+    return sorted(_SIGS)

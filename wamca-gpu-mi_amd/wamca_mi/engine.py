@@ -1,0 +1,198 @@
+"""Python view of the C ABI (include/mlp_gpu.h): one Engine per HIP context.
+
+Thin glue for tests and the benchmark: every call goes straight to
+``libwamca_mi.so``; errors raise :class:`MLPError` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import I32P, MLPError, MoveRec, lib  # noqa: F401
+
+NB_NAMES = ("swap", "2opt", "oropt1", "oropt2", "oropt3")
+ROWMAP_ADS, ROWMAP_ODC = 0, 1
+
+
+def rows_of(nb: int, rowmap: int, N: int) -> int:
+    """Rows of a neighborhood's launch grid (mlkswap.cu:991,1003; mlk2opt.cu:661;
+    mlkoropt.cu:819)."""
+    if nb == 0:
+        return N - 4 if rowmap == 0 else (N + 1) // 2 - 1
+    if nb == 1:
+        return (N + 1) // 2
+    return N - (nb - 1) - 2
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        st = self._L.mlp_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise MLPError(st, self._L.mlp_strerror(st).decode())
+        self.h = h
+        self.N = {}  # slot -> N
+
+    def close(self):
+        if self.h:
+            self._L.mlp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, st):
+        if st != 0:
+            raise MLPError(st, self._L.mlp_ctx_error(self.h).decode())
+
+    # -- device -------------------------------------------------------------
+    def device_info(self):
+        cu = C.c_int32()
+        hbm = C.c_int64()
+        name = C.create_string_buffer(256)
+        self._ck(self._L.mlp_device_info(self.h, C.byref(cu), C.byref(hbm), name, 256))
+        return {"cus": cu.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
+
+    def sync(self):
+        self._ck(self._L.mlp_sync(self.h))
+
+    # -- instances / slots --------------------------------------------------
+    def instance(self, xs, ys, rnd=0) -> int:
+        xs = np.ascontiguousarray(xs, np.int32)
+        ys = np.ascontiguousarray(ys, np.int32)
+        out = C.c_int32()
+        self._ck(self._L.mlp_instance_create(self.h, xs, ys, len(xs), rnd, C.byref(out)))
+        self._n_of = getattr(self, "_n_of", {})
+        self._n_of[out.value] = len(xs)
+        return out.value
+
+    def destroy_instance(self, inst):
+        self._ck(self._L.mlp_instance_destroy(self.h, inst))
+
+    def weights(self, inst) -> np.ndarray:
+        n = self._n_of[inst]
+        w = np.empty(n * n, np.int32)
+        self._ck(self._L.mlp_instance_weights(self.h, inst, w))
+        return w.reshape(n, n)
+
+    def slot(self, inst) -> int:
+        out = C.c_int32()
+        self._ck(self._L.mlp_slot_create(self.h, inst, C.byref(out)))
+        self.N[out.value] = self._n_of[inst] + 1
+        return out.value
+
+    def slots(self, inst, count):
+        return [self.slot(inst) for _ in range(count)]
+
+    def destroy_slot(self, s):
+        self._ck(self._L.mlp_slot_destroy(self.h, s))
+        self.N.pop(s, None)
+
+    @staticmethod
+    def footprint(n) -> int:
+        return int(lib().mlp_slot_footprint(n))
+
+    # -- tours --------------------------------------------------------------
+    def set_tours(self, slots, tours):
+        slots = np.ascontiguousarray(slots, np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(t, np.int32) for t in tours]))
+        self._ck(self._L.mlp_set_tours(self.h, slots, len(slots), flat))
+
+    def set_tour(self, s, tour):
+        self.set_tours([s], [tour])
+
+    def get_tours(self, slots):
+        slots = np.ascontiguousarray(slots, np.int32)
+        tot = sum(self.N[int(s)] for s in slots)
+        out = np.empty(tot, np.int32)
+        self._ck(self._L.mlp_get_tours(self.h, slots, len(slots), out))
+        res, off = [], 0
+        for s in slots:
+            res.append(out[off:off + self.N[int(s)]].copy())
+            off += self.N[int(s)]
+        return res
+
+    def get_tour(self, s):
+        return self.get_tours([s])[0]
+
+    def latency(self, slots):
+        slots = np.ascontiguousarray(slots, np.int32)
+        out = np.empty(len(slots), np.int64)
+        self._ck(self._L.mlp_get_latency(self.h, slots, len(slots), out))
+        return out
+
+    # -- neighborhoods ------------------------------------------------------
+    def eval(self, slots, nb_mask=0x1F, rowmap=ROWMAP_ADS, persistent=False):
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_eval(self.h, slots, len(slots), nb_mask, rowmap, int(persistent)))
+
+    def rows(self, s, nb):
+        cap = 16384
+        cost = np.empty(cap, np.int64)
+        i = np.empty(cap, np.int32)
+        j = np.empty(cap, np.int32)
+        R = C.c_int32()
+        self._ck(self._L.mlp_get_rows(self.h, s, nb, cost, i, j, C.byref(R)))
+        r = R.value
+        return {"cost": cost[:r].copy(), "i": i[:r].copy(), "j": j[:r].copy()}
+
+    def moves64(self, s, nb):
+        out = np.empty(16384, np.uint64)
+        R = C.c_int32()
+        self._ck(self._L.mlp_get_moves64(self.h, s, nb, out, C.byref(R)))
+        return out[:R.value].copy()
+
+    def merge_apply(self, slots, nb):
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_merge_apply(self.h, slots, len(slots), nb))
+
+    def applied(self, s, nb):
+        cap = 16384
+        buf = (MoveRec * cap)()
+        cnt = C.c_int32()
+        self._ck(self._L.mlp_get_applied(self.h, s, nb, buf, cap, C.byref(cnt)))
+        return [(buf[q].cost, buf[q].id, buf[q].i, buf[q].j, buf[q].row) for q in range(cnt.value)]
+
+    def dvnd(self, slots, rowmap=ROWMAP_ADS, max_rounds=0):
+        slots = np.ascontiguousarray(slots, np.int32)
+        rounds = np.zeros(len(slots), np.int32)
+        self._ck(self._L.mlp_dvnd(self.h, slots, len(slots), rowmap, max_rounds, rounds))
+        return rounds
+
+    def ads(self, s):
+        N = self.N[s]
+        nbytes = 128 + 4 * N * (N + 3)
+        buf = np.empty(nbytes // 4, np.uint32)
+        self._ck(self._L.mlp_ads_materialize(self.h, s, buf, nbytes))
+        return buf
+
+    def dist_table(self, s0, count, rnd):
+        out = np.empty(count, np.int32)
+        self._ck(self._L.mlp_dist_table(self.h, s0, count, rnd, out))
+        return out
+
+    def dist_list(self, s, rnd):
+        s = np.ascontiguousarray(s, np.int32)
+        out = np.empty(len(s), np.int32)
+        self._ck(self._L.mlp_dist_list(self.h, s, len(s), rnd, out))
+        return out
+
+    def timing_begin(self, capacity=8192):
+        self._ck(self._L.mlp_timing_begin(self.h, capacity))
+
+    def timing_end(self):
+        """(summed eval kernel ms, launches) since timing_begin."""
+        tot = C.c_double()
+        cnt = C.c_int32()
+        self._ck(self._L.mlp_timing_end(self.h, C.byref(tot), C.byref(cnt)))
+        return tot.value, cnt.value
+
+    def last_eval_ms(self) -> float:
+        ms = C.c_float()
+        self._ck(self._L.mlp_last_eval_ms(self.h, C.byref(ms)))
+        return ms.value
