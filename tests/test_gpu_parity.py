@@ -279,3 +279,20 @@ def test_dvnd_batch_uneven_equals_alone(eng):
         assert int(eng.latency([s])[0]) == refs[q][1]
         assert np.array_equal(eng.get_tour(s), refs[q][0])
         assert int(rounds[q]) == refs[q][2]
+
+
+@pytest.mark.parametrize("n,span", [(700, 30000), (2500, 20000)])
+def test_rows_int64_path(eng, n, span):
+    """Coordinates spread over [0, span]^2 push the |delta| bound past 2^31:
+    the tours take the int64 path; results stay identical to the oracle."""
+    xs, ys = O.instance_uniform(n, 77, span)
+    W = O.weights(xs, ys, 1)
+    inst = eng.instance(xs, ys, 1)
+    s = eng.slot(inst)
+    t = O.perm_tour(78, n)
+    eng.set_tour(s, t)
+    assert int(eng.latency([s])[0]) == O.latency(W, t)
+    for rowmap in (0, 1):
+        eng.eval([s], 0x1F, rowmap)
+        for nb in range(5):
+            _check_rows(eng, s, W, t, nb, rowmap)

@@ -43,6 +43,8 @@ struct SlotDev {
   int32_t* w;     // [N]
   int32_t* t32;   // [N]
   int32_t* q32;   // [N]
+  int32_t* g32;   // [N] 2-opt i-side constant -(N-p) w[p] - Z[p]      (mod 2^32)
+  int32_t* h32;   // [N] 2-opt j-side constant Z[p] - (N-p-1) w[p+1]  (mod 2^32)
   int64_t* t64;   // [N]
   int64_t* q64;   // [N]
   int64_t* info;  // [MLP_INFO_WORDS]
@@ -50,6 +52,9 @@ struct SlotDev {
   int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
   int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
 };
+
+// Z[p] = 2 Q[p] - (2p + 1) T[p] turns the 2-opt internal sum into
+// Z[j] - Z[i] + (j - i)(T[i] + T[j]).
 
 // (slot, neighborhood) segment of an eval launch: rows of the row map are cut
 // into blocks of MLP_EVAL_ROWS_PER_WG rows; blk0 is the first block index.
@@ -61,9 +66,23 @@ struct EvalSeg {
   int32_t blk0;
 };
 
+// Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
+// the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
+#define MLP_TILE_ROWS 8
+#define MLP_2OPT_ROWS 4
+struct SweepSeg {
+  int32_t slot;
+  int32_t kind;
+  int32_t mask;  // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
+  int32_t blk0;
+};
+
 #define MLP_SEGS_INLINE 40
 struct EvalSegsInline {
   EvalSeg s[MLP_SEGS_INLINE];
+};
+struct SweepSegsInline {
+  SweepSeg s[MLP_SEGS_INLINE];
 };
 
 #define MLP_SLOTS_INLINE 128

@@ -295,6 +295,15 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, int64_t* sh /* >=
     s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
     s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
     lsum += T;  // L = sum_p T[p]
+    // 2-opt per-position constants (mod 2^32)
+    int wn = 0;
+    if (p + 1 < N) {
+      const int cn = tour[p + 1];
+      wn = dist_xy((s.xs[c] << 16) | s.ys[c], (s.xs[cn] << 16) | s.ys[cn], dr);
+    }
+    const uint32_t Z = 2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
+    s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w - Z);
+    s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)wn);
   }
   __syncthreads();
   sh[tid] = lsum;
@@ -635,6 +644,299 @@ __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* 
 }
 
 // ===========================================================================
+// k_sweep: full-neighborhood sweep of the ADS row map.
+//
+// kind 0 items own MLP_TILE_ROWS consecutive rows "by" of swap and of each
+// or-opt-k.  Every one of those moves reads distances d(tour[a], tour[c]) for
+// a in [by, by+k+1] only, so the workgroup computes the tour-ordered distance
+// vectors v_a[c] for a in [b0, b0+R+3] once per column chunk into LDS (uint16)
+// and all four neighborhoods' rows read them.  Per move:
+//   swap    delta = Cs(i) + Bs(j) + (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j]
+//                 + (N-j) v_i[j-1] + (N-j-1) v_i[j+1]
+//   or-opt  forward  RF(i) + CF(j) + (N-j-k) v_{i+k-1}[j+k] + (N-j) v_i[j+k-1] - j S(i)
+//           backward RB(i) + CB(j) + (N-j) v_i[j-1] + (N-j-k) v_{i+k-1}[j] - j S(i)
+// with row constants (Cs, RF, RB, S) and column constants (Bs, CF, CB) in
+// LDS.  These are the positional forms of the reference's ODC kernels
+// (mlkswap.cu:770-785, mlkoropt.cu:704-736) regrouped; each equals the ADS
+// concatenation value of mlkswap.cu:210-278 / mlkoropt.cu:187-300.
+// kind 1 items own MLP_2OPT_ROWS 2-opt rows (one per wave): each row is two
+// diagonals j - i = m, and delta = G[i] + H[j] + (N-i) e[i-1] + (N-j-1) e[i]
+// + m (T[i] + T[j]) with e[p] = d(tour[p], tour[p+m+1]) staged in LDS.
+// Tours whose |delta| bound does not fit int32 take the per-lane int64 path.
+// ===========================================================================
+namespace {
+
+constexpr int kR = MLP_TILE_ROWS;
+constexpr int kChunk = 1024;
+constexpr int kVecs = kR + 4;
+constexpr int kVecW = kChunk + 4;
+constexpr int kSegs = 4 * kR;
+constexpr int k2Chunk = 2048;
+
+struct TileLds {
+  uint16_t v[kVecs][kVecW];
+  int32_t cs[kChunk];
+  int32_t cf[3][kChunk];
+  int32_t cb[3][kChunk];
+  int64_t best[kSegs];
+};
+
+struct DiagLds {
+  uint16_t e[MLP_2OPT_ROWS][k2Chunk + 2];
+};
+
+union SweepLds {
+  TileLds t;
+  DiagLds d;
+};
+
+__device__ __forceinline__ uint32_t umul(uint32_t a, uint32_t b) { return __umul24(a, b); }
+
+__device__ __forceinline__ int64_t pair_key(int32_t d, int32_t c) {
+  return c < 0 ? MLP_KEY_NONE : (int64_t)d * (int64_t)(1 << MLP_KEY_SHIFT) + c;
+}
+
+__device__ void sweep_fixed(const SlotDev& s, int b0, int mask, TileLds& L) {
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const float dr = round_dr(s.round);
+  for (int q = tid; q < kSegs; q += kBlock) L.best[q] = MLP_KEY_NONE;
+  for (int q0 = 1; q0 < N - 1; q0 += kChunk) {
+    const int q1 = min(q0 + kChunk, N - 1);  // move j in [q0, q1)
+    const int cbase = q0 - 1;                 // LDS column 0 <-> tour position q0 - 1
+    const int cend = min(q1 + 3, N);
+    for (int c = cbase + tid; c < cend; c += kBlock) {
+      const int xc = s.xy[c];
+#pragma unroll
+      for (int va = 0; va < kVecs; ++va) {
+        const int a = b0 + va;
+        if (a < N) L.v[va][c - cbase] = (uint16_t)dist_xy(s.xy[a], xc, dr);
+      }
+    }
+    for (int j = q0 + tid; j < q1; j += kBlock) {
+      const int x = j - q0;
+      const uint32_t wj = (uint32_t)s.w[j];
+      const uint32_t wj1 = j + 1 < N ? (uint32_t)s.w[j + 1] : 0u;
+      L.cs[x] = (int32_t)(0u - umul(N - j, wj) - umul(N - j - 1, wj1));
+#pragma unroll
+      for (int k = 1; k <= 3; ++k) {
+        uint32_t cf = 0u;
+        if (j + k <= N - 1)
+          cf = 0u - umul(N - j - k, (uint32_t)s.w[j + k]) + (uint32_t)k * (uint32_t)s.t32[j + k - 1];
+        L.cf[k - 1][x] = (int32_t)cf;
+        L.cb[k - 1][x] = (int32_t)(0u - umul(N - j, wj) + (uint32_t)k * (uint32_t)s.t32[j]);
+      }
+    }
+    __syncthreads();
+    for (int sg = wave; sg < kSegs; sg += kBlock / kWave) {
+      const int g = sg / kR, by = b0 + sg % kR;
+      const int nb = g == 0 ? 0 : g + 1;
+      if (!((mask >> nb) & 1)) continue;
+      int32_t bd = INT32_MAX, bc = -1;
+      if (g == 0) {
+        if (by > N - 5) continue;
+        const int i = by + 1;
+        const int jlo = max(q0, i + 2), jhi = min(q1, N - 1);
+        if (jlo >= jhi) continue;
+        const uint32_t Cs = 0u - umul(N - i, (uint32_t)s.w[i]) - umul(N - i - 1, (uint32_t)s.w[i + 1]);
+        const uint16_t* va = L.v[i - 1 - b0];
+        const uint16_t* vb = L.v[i + 1 - b0];
+        const uint16_t* vi = L.v[i - b0];
+        const uint32_t a1 = (uint32_t)(N - i), a2 = (uint32_t)(N - i - 1);
+        for (int j = jlo + lane; j < jhi; j += kWave) {
+          const int x = j - cbase;
+          const uint32_t nj = (uint32_t)(N - j);
+          uint32_t d = Cs + (uint32_t)L.cs[j - q0];
+          d += umul(a1, va[x]) + umul(a2, vb[x]) + umul(nj, vi[x - 1]) + umul(nj - 1u, vi[x + 1]);
+          if ((int32_t)d < bd) {
+            bd = (int32_t)d;
+            bc = j;
+          }
+        }
+      } else {
+        const int k = g;
+        if (by > N - k - 3) continue;
+        // backward part of the row: i = by + 2, j in [1, by + 1]
+        {
+          const int i = by + 2;
+          const int jlo = max(q0, 1), jhi = min(q1, by + 2);
+          if (jlo < jhi) {
+            const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
+            const uint32_t dB = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
+            const uint32_t R = (uint32_t)(N - i - k) * (dB - (uint32_t)s.w[i + k]) -
+                               (uint32_t)(N - i) * (uint32_t)s.w[i] + (uint32_t)i * S -
+                               (uint32_t)k * (uint32_t)s.t32[i - 1];
+            const uint16_t* vi = L.v[i - b0];
+            const uint16_t* vk = L.v[i + k - 1 - b0];
+            const int32_t* cb = L.cb[k - 1];
+            for (int j = jlo + lane; j < jhi; j += kWave) {
+              const int x = j - cbase;
+              const uint32_t nj = (uint32_t)(N - j);
+              uint32_t d = R + (uint32_t)cb[j - q0];
+              d += umul(nj, vi[x - 1]) + umul(nj - (uint32_t)k, vk[x]) - umul((uint32_t)j, S);
+              if ((int32_t)d < bd) {
+                bd = (int32_t)d;
+                bc = j - 1;
+              }
+            }
+          }
+        }
+        // forward part: i = by + 1, j in [by + 2, N - k - 1]
+        {
+          const int i = by + 1;
+          const int jlo = max(q0, by + 2), jhi = min(q1, N - k);
+          if (jlo < jhi) {
+            const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
+            const uint32_t dF = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
+            const uint32_t R = (uint32_t)(N - i) * (dF - (uint32_t)s.w[i]) -
+                               (uint32_t)(N - i - k) * (uint32_t)s.w[i + k] + (uint32_t)i * S -
+                               (uint32_t)k * (uint32_t)s.t32[i + k];
+            const uint16_t* vi = L.v[i - b0];
+            const uint16_t* vk = L.v[i + k - 1 - b0];
+            const int32_t* cf = L.cf[k - 1];
+            for (int j = jlo + lane; j < jhi; j += kWave) {
+              const int x = j - cbase;
+              const uint32_t nj = (uint32_t)(N - j);
+              uint32_t d = R + (uint32_t)cf[j - q0];
+              d += umul(nj - (uint32_t)k, vk[x + k]) + umul(nj, vi[x + k - 1]) - umul((uint32_t)j, S);
+              if ((int32_t)d < bd) {
+                bd = (int32_t)d;
+                bc = j - 1;
+              }
+            }
+          }
+        }
+      }
+      const int64_t key = wave_min64(pair_key(bd, bc));
+      if (lane == 0 && key < L.best[sg]) L.best[sg] = key;
+    }
+    __syncthreads();
+  }
+  for (int sg = tid; sg < kSegs; sg += kBlock) {
+    const int g = sg / kR, by = b0 + sg % kR;
+    const int nb = g == 0 ? 0 : g + 1;
+    if (((mask >> nb) & 1) && by < rows_of(nb, 0, N)) s.rows[nb][by] = L.best[sg];
+  }
+}
+
+// int64 fallback of a fixed-i block: one wave per (neighborhood, row).
+__device__ void sweep_fixed_wide(const SlotDev& s, int b0, int mask) {
+  const int N = s.N, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int sg = wave; sg < kSegs; sg += kBlock / kWave) {
+    const int g = sg / kR, by = b0 + sg % kR;
+    const int nb = g == 0 ? 0 : g + 1;
+    if (!((mask >> nb) & 1) || by >= rows_of(nb, 0, N)) continue;
+    if (nb == 0) eval_row<0, false>(s, nb, 0, by, lane);
+    else eval_row<2, false>(s, nb, 0, by, lane);
+  }
+}
+
+__device__ void sweep_2opt(const SlotDev& s, int b0, DiagLds& L) {
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const float dr = round_dr(s.round);
+  const int by = b0 + wave;
+  const bool live = by < ((N + 1) >> 1);
+  uint16_t* e = L.e[wave];
+  int32_t bd = INT32_MAX, bc = -1;
+  for (int part = 0; part < 2; ++part) {
+    // part 0: ctx < by, m = N-2-by, i = by - ctx in [1, by]
+    // part 1: ctx >= by, m = by+1,  i = ctx - by + 1 in [1, N-3-by]
+    const int m = part == 0 ? N - 2 - by : by + 1;
+    const int imax = live ? (part == 0 ? by : N - 3 - by) : 0;
+    for (int i0 = 1; i0 <= N - 3; i0 += k2Chunk) {
+      const int i1 = min(i0 + k2Chunk - 1, imax);
+      if (i0 <= i1) {
+        for (int x = lane; x <= i1 - i0 + 1; x += kWave) {
+          const int p = i0 - 1 + x;
+          e[x] = (uint16_t)dist_xy(s.xy[p], s.xy[p + m + 1], dr);
+        }
+      }
+      __syncthreads();
+      if (i0 <= i1) {
+        for (int i = i0 + lane; i <= i1; i += kWave) {
+          const int j = i + m;
+          const int x = i - i0;
+          uint32_t d = (uint32_t)s.g32[i] + (uint32_t)s.h32[j];
+          d += umul((uint32_t)(N - i), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
+          d += (uint32_t)m * ((uint32_t)s.t32[i] + (uint32_t)s.t32[j]);
+          const int ctx = part == 0 ? by - i : i + by - 1;
+          if ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc)) {
+            bd = (int32_t)d;
+            bc = ctx;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t key = wave_min64(pair_key(bd, bc));
+  if (live && lane == 0) s.rows[1][by] = key;
+}
+
+__device__ void sweep_2opt_wide(const SlotDev& s, int b0) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int by = b0 + wave;
+  if (by < ((s.N + 1) >> 1)) eval_row<1, false>(s, 1, 0, by, lane);
+}
+
+__device__ __forceinline__ SweepSeg find_sweep(const SweepSeg* segs, int nsegs, int item) {
+  int lo = 0, hi = nsegs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].blk0 <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return segs[lo];
+}
+
+__device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl, int nsegs,
+                                                      int item) {
+  SweepSeg sg = inl.s[0];
+#pragma unroll
+  for (int q = 1; q < MLP_SEGS_INLINE; ++q)
+    if (q < nsegs && inl.s[q].blk0 <= item) sg = inl.s[q];
+  return sg;
+}
+
+__device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
+                           SweepLds& L) {
+  const SlotDev& s = table[sg.slot];
+  const int b = item - sg.blk0;
+  const bool p32 = s.info[MLP_INFO_PATH32] != 0;
+  if (sg.kind == 0) {
+    if (p32) sweep_fixed(s, b * kR, sg.mask, L.t);
+    else sweep_fixed_wide(s, b * kR, sg.mask);
+  } else {
+    if (p32) sweep_2opt(s, b * MLP_2OPT_ROWS, L.d);
+    else sweep_2opt_wide(s, b * MLP_2OPT_ROWS);
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ table,
+                                                  SweepSegsInline inl, const SweepSeg* segs_dev,
+                                                  int nsegs, int total, int* counter) {
+  __shared__ SweepLds L;
+  if (!counter) {
+    const int item = blockIdx.x;
+    const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
+    sweep_item(table, sg, item, L);
+    return;
+  }
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
+    sweep_item(table, sg, item, L);
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
 // ===========================================================================
 #include "mlp_launch.h"
@@ -657,6 +959,14 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
                            int* counter, int grid) {
   hipLaunchKernelGGL(k_eval, dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs, total,
                      rowmap, counter);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_sweep(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
+                            const SweepSeg* segs_dev, int nsegs, int total, int* counter,
+                            int grid) {
+  hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs, total,
+                     counter);
   return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Byte layout of one slot's device block.
 struct SlotLayout {
-  size_t tour, xy, w, t32, q32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
+  size_t tour, xy, w, t32, q32, g32, h32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
       kij[MLP_NB_COUNT], total;
 };
 
@@ -86,6 +86,8 @@ SlotLayout slot_layout(int N) {
   L.w = take(4 * (size_t)N);
   L.t32 = take(4 * (size_t)N);
   L.q32 = take(4 * (size_t)N);
+  L.g32 = take(4 * (size_t)N);
+  L.h32 = take(4 * (size_t)N);
   L.t64 = take(8 * (size_t)N);
   L.q64 = take(8 * (size_t)N);
   L.info = take(8 * MLP_INFO_WORDS);
@@ -426,6 +428,8 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   d.w = (int32_t*)(b + L.w);
   d.t32 = (int32_t*)(b + L.t32);
   d.q32 = (int32_t*)(b + L.q32);
+  d.g32 = (int32_t*)(b + L.g32);
+  d.h32 = (int32_t*)(b + L.h32);
   d.t64 = (int64_t*)(b + L.t64);
   d.q64 = (int64_t*)(b + L.q64);
   d.info = (int64_t*)(b + L.info);
@@ -538,21 +542,112 @@ int mlp_get_latency(mlp_ctx* ctx, const int32_t* slots, int32_t count, int64_t* 
   return gather_info(ctx, slots, count, MLP_INFO_LATENCY, latency);
 }
 
+namespace {
+
+// Records one eval launch between a pair of HIP events (timing ring or the
+// context's last-launch pair).
+int timed_launch(mlp_ctx* ctx, hipError_t (*launch)(mlp_ctx*, void*), void* arg) {
+  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
+  if (ctx->timing) {
+    if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
+    e0 = ctx->tev[2 * ctx->tcount];
+    e1 = ctx->tev[2 * ctx->tcount + 1];
+    ctx->tcount++;
+  }
+  HCK(hipEventRecord(e0, ctx->stream));
+  HCK(launch(ctx, arg));
+  HCK(hipEventRecord(e1, ctx->stream));
+  ctx->eval_timed = true;
+  return MLP_OK;
+}
+
+struct EvalArgs {
+  EvalSegsInline inl;
+  const EvalSeg* dev;
+  SweepSegsInline sinl;
+  const SweepSeg* sdev;
+  int nsegs, total, rowmap, grid;
+  int* counter;
+};
+
+hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
+  EvalArgs& a = *(EvalArgs*)p;
+  return mlp_launch_eval(ctx->stream, ctx->d_table, a.inl, a.dev, a.nsegs, a.total, a.rowmap,
+                         a.counter, a.grid);
+}
+
+hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
+  EvalArgs& a = *(EvalArgs*)p;
+  return mlp_launch_sweep(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total, a.counter,
+                          a.grid);
+}
+
+}  // namespace
+
 int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
              int32_t rowmap, int32_t persistent) {
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
   if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  for (int q = 0; q < count; ++q) {
+    Slot& s = ctx->slots[slots[q]];
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      if (nb_mask & (1u << nb)) {
+        s.eval_rowmap[nb] = rowmap;
+        s.merged[nb] = false;
+      }
+  }
+  EvalArgs a{};
+  if (rowmap == 0) {
+    // ADS row map: fixed-i row blocks (swap + or-opt) and 2-opt diagonal rows
+    std::vector<SweepSeg> segs;
+    segs.reserve((size_t)count * 2);
+    int total = 0;
+    const uint32_t fixed = nb_mask & 0x1du;
+    for (int q = 0; q < count; ++q) {
+      const Slot& s = ctx->slots[slots[q]];
+      if (fixed) {
+        int rmax = 0;
+        for (int nb : {0, 2, 3, 4})
+          if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
+        if (rmax > 0) {
+          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total});
+          total += (rmax + MLP_TILE_ROWS - 1) / MLP_TILE_ROWS;
+        }
+      }
+      if (nb_mask & 2u) {
+        const int r = rows_of(1, 0, s.N);
+        if (r > 0 && s.N >= 4) {
+          segs.push_back(SweepSeg{slots[q], 1, 2, total});
+          total += (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS;
+        }
+      }
+    }
+    if (segs.empty()) return MLP_OK;
+    a.nsegs = (int)segs.size();
+    a.total = total;
+    if (a.nsegs <= MLP_SEGS_INLINE) {
+      for (int q = 0; q < a.nsegs; ++q) a.sinl.s[q] = segs[(size_t)q];
+    } else {
+      st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * 4, (const int32_t**)&a.sdev);
+      if (st) return st;
+    }
+    a.grid = total;
+    if (persistent) {
+      a.counter = ctx->d_counter;
+      HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+      a.grid = std::min(total, ctx->cu_count * 3);
+    }
+    return timed_launch(ctx, launch_sweep_cb, &a);
+  }
   std::vector<EvalSeg> segs;
   segs.reserve((size_t)count * 5);
   int total = 0;
   for (int q = 0; q < count; ++q) {
-    Slot& s = ctx->slots[slots[q]];
+    const Slot& s = ctx->slots[slots[q]];
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
       if (!(nb_mask & (1u << nb))) continue;
-      s.eval_rowmap[nb] = rowmap;
-      s.merged[nb] = false;
       const int R = rows_of(nb, rowmap, s.N);
       if (R <= 0) continue;
       segs.push_back(EvalSeg{slots[q], nb, R, total});
@@ -560,35 +655,22 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
     }
   }
   if (segs.empty()) return MLP_OK;
-  EvalSegsInline inl;
-  const EvalSeg* dev = nullptr;
-  const int nsegs = (int)segs.size();
-  if (nsegs <= MLP_SEGS_INLINE) {
-    for (int q = 0; q < nsegs; ++q) inl.s[q] = segs[q];
+  a.nsegs = (int)segs.size();
+  a.total = total;
+  a.rowmap = rowmap;
+  if (a.nsegs <= MLP_SEGS_INLINE) {
+    for (int q = 0; q < a.nsegs; ++q) a.inl.s[q] = segs[(size_t)q];
   } else {
-    st = device_list(ctx, (const int32_t*)segs.data(), nsegs * 4, (const int32_t**)&dev);
+    st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * 4, (const int32_t**)&a.dev);
     if (st) return st;
   }
-  int* counter = nullptr;
-  int grid = total;
+  a.grid = total;
   if (persistent) {
-    counter = ctx->d_counter;
-    HCK(hipMemsetAsync(counter, 0, 4, ctx->stream));
-    grid = std::min(total, ctx->cu_count * 8);
+    a.counter = ctx->d_counter;
+    HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+    a.grid = std::min(total, ctx->cu_count * 8);
   }
-  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
-  if (ctx->timing) {
-    const size_t q = ctx->tcount % (ctx->tev.size() / 2);
-    if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
-    e0 = ctx->tev[2 * q];
-    e1 = ctx->tev[2 * q + 1];
-    ctx->tcount++;
-  }
-  HCK(hipEventRecord(e0, ctx->stream));
-  HCK(mlp_launch_eval(ctx->stream, ctx->d_table, inl, dev, nsegs, total, rowmap, counter, grid));
-  HCK(hipEventRecord(e1, ctx->stream));
-  ctx->eval_timed = true;
-  return MLP_OK;
+  return timed_launch(ctx, launch_eval_cb, &a);
 }
 
 int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity) {
