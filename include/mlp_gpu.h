@@ -142,6 +142,11 @@ int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32
 /* Same for an explicit list of squared lengths. */
 int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out);
 
+/* Sweep tile shape for mlp_eval with the ADS row map: -1 automatic (small
+ * tiles for launches of at most 8 tours, large tiles otherwise), 0 small
+ * (4 rows x 256 columns), 1 large (8 rows x 1024 columns). */
+int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
+
 /* Timing of the last eval launch on the context stream (ms, HIP events). */
 int mlp_last_eval_ms(mlp_ctx* ctx, float* ms);
 /* Per-launch eval timing: between begin and end every mlp_eval launch is

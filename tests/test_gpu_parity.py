@@ -296,3 +296,29 @@ def test_rows_int64_path(eng, n, span):
         eng.eval([s], 0x1F, rowmap)
         for nb in range(5):
             _check_rows(eng, s, W, t, nb, rowmap)
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("n", [9, 300, 1000, 2000])
+def test_tile_shapes_and_row_buffers(eng, tile, n):
+    """Both sweep tile shapes (column-split rows folded with atomicMin into
+    double-buffered row arrays) give the oracle's rows, across repeated
+    evaluations and alternating row maps."""
+    xs, ys = O.instance_clustered(n, 300 + n)
+    W = O.weights(xs, ys, 1)
+    inst = eng.instance(xs, ys, 1)
+    s = eng.slot(inst)
+    eng.set_tile(tile)
+    try:
+        for rep, (rowmap, tseed) in enumerate([(0, 1), (0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (0, 3)]):
+            t = O.perm_tour(tseed + n, n)
+            eng.set_tour(s, t)
+            eng.eval([s], 0x1F, rowmap)
+            for nb in range(5):
+                _check_rows(eng, s, W, t, nb, rowmap)
+            # a neighborhood subset leaves the others' buffers alone
+            eng.eval([s], 0b00101, rowmap)
+            for nb in (0, 2):
+                _check_rows(eng, s, W, t, nb, rowmap)
+    finally:
+        eng.set_tile(-1)

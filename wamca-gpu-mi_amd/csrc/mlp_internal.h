@@ -28,7 +28,8 @@ enum {
   MLP_INFO_ROUND_MOVES = 3,  // moves applied since the last round reset
   MLP_INFO_TOTAL_MOVES = 4,  // moves applied since the tour was set
   MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
-  MLP_INFO_WORDS = 16
+  MLP_INFO_PARITY0 = 16,  // row buffer (0/1) holding the last eval of nb (16 + nb)
+  MLP_INFO_WORDS = 24
 };
 
 struct SlotDev {
@@ -36,6 +37,7 @@ struct SlotDev {
   int32_t n;      // clients
   int32_t round;  // 0 trunc, 1 nint
   int32_t maxd;   // upper bound of any distance in the instance
+  int32_t rowcap; // entries per row buffer (rows[nb] holds two buffers)
   const int32_t* xs;  // instance coordinates (client indexed)
   const int32_t* ys;
   int32_t* tour;  // [N]
@@ -48,7 +50,10 @@ struct SlotDev {
   int64_t* t64;   // [N]
   int64_t* q64;   // [N]
   int64_t* info;  // [MLP_INFO_WORDS]
-  int64_t* rows[MLP_NB_COUNT];      // per-row best keys (delta << 20 | ctx)
+  // per-row best keys (delta << 20 | ctx), two buffers of rowcap entries: an
+  // eval of nb folds its partial results into buffer p with atomicMin and
+  // resets buffer p ^ 1 for the next eval (p recorded in info[PARITY0 + nb]).
+  int64_t* rows[MLP_NB_COUNT];
   int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
   int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
 };
@@ -64,7 +69,9 @@ struct EvalSeg {
   int32_t nb;
   int32_t rows;
   int32_t blk0;
+  int32_t parity;
 };
+#define MLP_EVALSEG_INTS 5
 
 // Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
 // the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
@@ -73,11 +80,17 @@ struct EvalSeg {
 struct SweepSeg {
   int32_t slot;
   int32_t kind;
-  int32_t mask;  // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
+  int32_t mask;    // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
   int32_t blk0;
+  int32_t ncols;   // column tiles per row block
+  int32_t parity;  // bit nb: row buffer written by this eval
 };
+#define MLP_SWEEPSEG_INTS 6
+// Tile shapes (rows x columns) of the sweep; 0 = small (single tours), 1 = large (batches).
+#define MLP_TILE_SMALL 0
+#define MLP_TILE_LARGE 1
 
-#define MLP_SEGS_INLINE 40
+#define MLP_SEGS_INLINE 32
 struct EvalSegsInline {
   EvalSeg s[MLP_SEGS_INLINE];
 };

@@ -176,7 +176,7 @@ __device__ __forceinline__ float round_dr(int round) { return round ? 0.5F : 0.0
 // One row: lanes stride over the row-local index, keep the smallest key
 // (delta << 20 | ctx) -> lowest delta, then lowest index; wave reduction.
 template <int KIND, bool P32>
-__device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane) {
+__device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane, int64_t* out) {
   const int N = s.N;
   const float dr = round_dr(s.round);
   int lo, hi;
@@ -194,14 +194,15 @@ __device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane
     best = key < best ? key : best;
   }
   best = wave_min64(best);
-  if (lane == 0) s.rows[nb][row] = best;
+  if (lane == 0) *out = best;
 }
 
 template <bool P32>
-__device__ void eval_row_dispatch(const SlotDev& s, int nb, int rowmap, int row, int lane) {
-  if (nb == 0) eval_row<0, P32>(s, nb, rowmap, row, lane);
-  else if (nb == 1) eval_row<1, P32>(s, nb, rowmap, row, lane);
-  else eval_row<2, P32>(s, nb, rowmap, row, lane);
+__device__ void eval_row_dispatch(const SlotDev& s, int nb, int rowmap, int row, int lane,
+                                  int64_t* out) {
+  if (nb == 0) eval_row<0, P32>(s, nb, rowmap, row, lane, out);
+  else if (nb == 1) eval_row<1, P32>(s, nb, rowmap, row, lane, out);
+  else eval_row<2, P32>(s, nb, rowmap, row, lane, out);
 }
 
 __device__ __forceinline__ EvalSeg find_seg(const EvalSeg* segs, int nsegs, int item) {
@@ -228,10 +229,21 @@ __device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, 
                           int rowmap) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int row = (item - sg.blk0) * MLP_EVAL_ROWS_PER_WG + wave;
-  if (row >= sg.rows) return;
   const SlotDev& s = table[sg.slot];
-  if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane);
-  else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane);
+  const int par = sg.parity;
+  int64_t* cur = s.rows[sg.nb] + (par ? s.rowcap : 0);
+  int64_t* oth = s.rows[sg.nb] + (par ? 0 : s.rowcap);
+  if (item == sg.blk0) {  // first block: clear the tails of both buffers, record parity
+    for (int r = sg.rows + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
+      cur[r] = MLP_KEY_NONE;
+      oth[r] = MLP_KEY_NONE;
+    }
+    if (threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + sg.nb] = par;
+  }
+  if (row >= sg.rows) return;
+  if (lane == 0) oth[row] = MLP_KEY_NONE;
+  if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
+  else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane, cur + row);
 }
 
 __device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t* dev, int q) {
@@ -439,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
   __syncthreads();
   // 1. improving rows -> (cost, row) keys
-  const int64_t* rk = s.rows[nb];
+  const int64_t* rk = s.rows[nb] + (s.info[MLP_INFO_PARITY0 + nb] ? s.rowcap : 0);
   for (int r = tid; r < R; r += kBlock) {
     const int64_t key = rk[r];
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
@@ -646,11 +658,11 @@ __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* 
 // ===========================================================================
 // k_sweep: full-neighborhood sweep of the ADS row map.
 //
-// kind 0 items own MLP_TILE_ROWS consecutive rows "by" of swap and of each
-// or-opt-k.  Every one of those moves reads distances d(tour[a], tour[c]) for
-// a in [by, by+k+1] only, so the workgroup computes the tour-ordered distance
-// vectors v_a[c] for a in [b0, b0+R+3] once per column chunk into LDS (uint16)
-// and all four neighborhoods' rows read them.  Per move:
+// kind 0 items own a tile of R consecutive rows "by" x C columns j of swap
+// and of each or-opt-k.  Every one of those moves reads distances
+// d(tour[a], tour[c]) for a in [by, by+k+1] only, so the workgroup computes
+// the tour-ordered distance vectors v_a[c], a in [b0, b0+R+3], once into LDS
+// (uint16) and all four neighborhoods' rows read them.  Per move:
 //   swap    delta = Cs(i) + Bs(j) + (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j]
 //                 + (N-j) v_i[j-1] + (N-j-1) v_i[j+1]
 //   or-opt  forward  RF(i) + CF(j) + (N-j-k) v_{i+k-1}[j+k] + (N-j) v_i[j+k-1] - j S(i)
@@ -659,35 +671,33 @@ __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* 
 // LDS.  These are the positional forms of the reference's ODC kernels
 // (mlkswap.cu:770-785, mlkoropt.cu:704-736) regrouped; each equals the ADS
 // concatenation value of mlkswap.cu:210-278 / mlkoropt.cu:187-300.
-// kind 1 items own MLP_2OPT_ROWS 2-opt rows (one per wave): each row is two
-// diagonals j - i = m, and delta = G[i] + H[j] + (N-i) e[i-1] + (N-j-1) e[i]
-// + m (T[i] + T[j]) with e[p] = d(tour[p], tour[p+m+1]) staged in LDS.
-// Tours whose |delta| bound does not fit int32 take the per-lane int64 path.
+// kind 1 items own four 2-opt rows (one per wave) x C row-local indices: a
+// row is two diagonals j - i = m, and delta = G[i] + H[j] + (N-i) e[i-1] +
+// (N-j-1) e[i] + m (T[i] + T[j]) with e[p] = d(tour[p], tour[p+m+1]) in LDS.
+// Partial per-row bests of a tile are folded into the row buffer with a
+// 64-bit atomicMin on the (delta, index) key.  Tours whose |delta| bound does
+// not fit int32 take the per-lane int64 path.
 // ===========================================================================
 namespace {
 
-constexpr int kR = MLP_TILE_ROWS;
-constexpr int kChunk = 1024;
-constexpr int kVecs = kR + 4;
-constexpr int kVecW = kChunk + 4;
-constexpr int kSegs = 4 * kR;
-constexpr int k2Chunk = 2048;
 
+template <int R, int C>
 struct TileLds {
-  uint16_t v[kVecs][kVecW];
-  int32_t cs[kChunk];
-  int32_t cf[3][kChunk];
-  int32_t cb[3][kChunk];
-  int64_t best[kSegs];
+  uint16_t v[R + 4][C + 4];
+  int32_t cs[C];
+  int32_t cf[3][C];
+  int32_t cb[3][C];
 };
 
+template <int C>
 struct DiagLds {
-  uint16_t e[MLP_2OPT_ROWS][k2Chunk + 2];
+  uint16_t e[MLP_2OPT_ROWS][2][C + 2];
 };
 
+template <int R, int C>
 union SweepLds {
-  TileLds t;
-  DiagLds d;
+  TileLds<R, C> t;
+  DiagLds<C> d;
 };
 
 __device__ __forceinline__ uint32_t umul(uint32_t a, uint32_t b) { return __umul24(a, b); }
@@ -696,186 +706,230 @@ __device__ __forceinline__ int64_t pair_key(int32_t d, int32_t c) {
   return c < 0 ? MLP_KEY_NONE : (int64_t)d * (int64_t)(1 << MLP_KEY_SHIFT) + c;
 }
 
-__device__ void sweep_fixed(const SlotDev& s, int b0, int mask, TileLds& L) {
-  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-  const float dr = round_dr(s.round);
-  for (int q = tid; q < kSegs; q += kBlock) L.best[q] = MLP_KEY_NONE;
-  for (int q0 = 1; q0 < N - 1; q0 += kChunk) {
-    const int q1 = min(q0 + kChunk, N - 1);  // move j in [q0, q1)
-    const int cbase = q0 - 1;                 // LDS column 0 <-> tour position q0 - 1
-    const int cend = min(q1 + 3, N);
-    for (int c = cbase + tid; c < cend; c += kBlock) {
-      const int xc = s.xy[c];
-#pragma unroll
-      for (int va = 0; va < kVecs; ++va) {
-        const int a = b0 + va;
-        if (a < N) L.v[va][c - cbase] = (uint16_t)dist_xy(s.xy[a], xc, dr);
-      }
+__device__ __forceinline__ int64_t* row_buf(const SlotDev& s, int nb, int par) {
+  return s.rows[nb] + (par ? s.rowcap : 0);
+}
+
+__device__ __forceinline__ void fold_key(int64_t* p, int64_t key) {
+  if (key != MLP_KEY_NONE) atomicMin((long long*)p, (long long)key);
+}
+
+// Reset of the other row buffer by the tiles of column 0; the first tile also
+// clears the tail beyond this row map and records the parity.
+__device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
+  int64_t* o = row_buf(s, nb, par ^ 1);
+  for (int r = r0 + (int)threadIdx.x; r < r1; r += kBlock) o[r] = MLP_KEY_NONE;
+  if (first) {
+    for (int r = rows_of(nb, 0, s.N) + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
+      o[r] = MLP_KEY_NONE;
+      s.rows[nb][(par ? s.rowcap : 0) + r] = MLP_KEY_NONE;
     }
-    for (int j = q0 + tid; j < q1; j += kBlock) {
-      const int x = j - q0;
-      const uint32_t wj = (uint32_t)s.w[j];
-      const uint32_t wj1 = j + 1 < N ? (uint32_t)s.w[j + 1] : 0u;
-      L.cs[x] = (int32_t)(0u - umul(N - j, wj) - umul(N - j - 1, wj1));
-#pragma unroll
-      for (int k = 1; k <= 3; ++k) {
-        uint32_t cf = 0u;
-        if (j + k <= N - 1)
-          cf = 0u - umul(N - j - k, (uint32_t)s.w[j + k]) + (uint32_t)k * (uint32_t)s.t32[j + k - 1];
-        L.cf[k - 1][x] = (int32_t)cf;
-        L.cb[k - 1][x] = (int32_t)(0u - umul(N - j, wj) + (uint32_t)k * (uint32_t)s.t32[j]);
-      }
-    }
-    __syncthreads();
-    for (int sg = wave; sg < kSegs; sg += kBlock / kWave) {
-      const int g = sg / kR, by = b0 + sg % kR;
-      const int nb = g == 0 ? 0 : g + 1;
-      if (!((mask >> nb) & 1)) continue;
-      int32_t bd = INT32_MAX, bc = -1;
-      if (g == 0) {
-        if (by > N - 5) continue;
-        const int i = by + 1;
-        const int jlo = max(q0, i + 2), jhi = min(q1, N - 1);
-        if (jlo >= jhi) continue;
-        const uint32_t Cs = 0u - umul(N - i, (uint32_t)s.w[i]) - umul(N - i - 1, (uint32_t)s.w[i + 1]);
-        const uint16_t* va = L.v[i - 1 - b0];
-        const uint16_t* vb = L.v[i + 1 - b0];
-        const uint16_t* vi = L.v[i - b0];
-        const uint32_t a1 = (uint32_t)(N - i), a2 = (uint32_t)(N - i - 1);
-        for (int j = jlo + lane; j < jhi; j += kWave) {
-          const int x = j - cbase;
-          const uint32_t nj = (uint32_t)(N - j);
-          uint32_t d = Cs + (uint32_t)L.cs[j - q0];
-          d += umul(a1, va[x]) + umul(a2, vb[x]) + umul(nj, vi[x - 1]) + umul(nj - 1u, vi[x + 1]);
-          if ((int32_t)d < bd) {
-            bd = (int32_t)d;
-            bc = j;
-          }
-        }
-      } else {
-        const int k = g;
-        if (by > N - k - 3) continue;
-        // backward part of the row: i = by + 2, j in [1, by + 1]
-        {
-          const int i = by + 2;
-          const int jlo = max(q0, 1), jhi = min(q1, by + 2);
-          if (jlo < jhi) {
-            const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
-            const uint32_t dB = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
-            const uint32_t R = (uint32_t)(N - i - k) * (dB - (uint32_t)s.w[i + k]) -
-                               (uint32_t)(N - i) * (uint32_t)s.w[i] + (uint32_t)i * S -
-                               (uint32_t)k * (uint32_t)s.t32[i - 1];
-            const uint16_t* vi = L.v[i - b0];
-            const uint16_t* vk = L.v[i + k - 1 - b0];
-            const int32_t* cb = L.cb[k - 1];
-            for (int j = jlo + lane; j < jhi; j += kWave) {
-              const int x = j - cbase;
-              const uint32_t nj = (uint32_t)(N - j);
-              uint32_t d = R + (uint32_t)cb[j - q0];
-              d += umul(nj, vi[x - 1]) + umul(nj - (uint32_t)k, vk[x]) - umul((uint32_t)j, S);
-              if ((int32_t)d < bd) {
-                bd = (int32_t)d;
-                bc = j - 1;
-              }
-            }
-          }
-        }
-        // forward part: i = by + 1, j in [by + 2, N - k - 1]
-        {
-          const int i = by + 1;
-          const int jlo = max(q0, by + 2), jhi = min(q1, N - k);
-          if (jlo < jhi) {
-            const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
-            const uint32_t dF = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
-            const uint32_t R = (uint32_t)(N - i) * (dF - (uint32_t)s.w[i]) -
-                               (uint32_t)(N - i - k) * (uint32_t)s.w[i + k] + (uint32_t)i * S -
-                               (uint32_t)k * (uint32_t)s.t32[i + k];
-            const uint16_t* vi = L.v[i - b0];
-            const uint16_t* vk = L.v[i + k - 1 - b0];
-            const int32_t* cf = L.cf[k - 1];
-            for (int j = jlo + lane; j < jhi; j += kWave) {
-              const int x = j - cbase;
-              const uint32_t nj = (uint32_t)(N - j);
-              uint32_t d = R + (uint32_t)cf[j - q0];
-              d += umul(nj - (uint32_t)k, vk[x + k]) + umul(nj, vi[x + k - 1]) - umul((uint32_t)j, S);
-              if ((int32_t)d < bd) {
-                bd = (int32_t)d;
-                bc = j - 1;
-              }
-            }
-          }
-        }
-      }
-      const int64_t key = wave_min64(pair_key(bd, bc));
-      if (lane == 0 && key < L.best[sg]) L.best[sg] = key;
-    }
-    __syncthreads();
-  }
-  for (int sg = tid; sg < kSegs; sg += kBlock) {
-    const int g = sg / kR, by = b0 + sg % kR;
-    const int nb = g == 0 ? 0 : g + 1;
-    if (((mask >> nb) & 1) && by < rows_of(nb, 0, N)) s.rows[nb][by] = L.best[sg];
+    if (threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
   }
 }
 
-// int64 fallback of a fixed-i block: one wave per (neighborhood, row).
-__device__ void sweep_fixed_wide(const SlotDev& s, int b0, int mask) {
+template <int R, int C>
+__device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
+                            TileLds<R, C>& L) {
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const float dr = round_dr(s.round);
+  const int b0 = rb * R;
+  const int q0 = 1 + ct * C;                 // move j in [q0, q1)
+  const int q1 = min(q0 + C, N - 1);
+  const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
+  const int cend = min(q1 + 3, N);
+  if (ct == 0) {
+#pragma unroll
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      if (nb != 1 && ((mask >> nb) & 1))
+        reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+  }
+  for (int c = cbase + tid; c < cend; c += kBlock) {
+    const int xc = s.xy[c];
+#pragma unroll
+    for (int va = 0; va < R + 4; ++va) {
+      const int a = b0 + va;
+      if (a < N) L.v[va][c - cbase] = (uint16_t)dist_xy(s.xy[a], xc, dr);
+    }
+  }
+  for (int j = q0 + tid; j < q1; j += kBlock) {
+    const int x = j - q0;
+    const uint32_t wj = (uint32_t)s.w[j];
+    const uint32_t wj1 = j + 1 < N ? (uint32_t)s.w[j + 1] : 0u;
+    L.cs[x] = (int32_t)(0u - umul(N - j, wj) - umul(N - j - 1, wj1));
+#pragma unroll
+    for (int k = 1; k <= 3; ++k) {
+      uint32_t cf = 0u;
+      if (j + k <= N - 1)
+        cf = 0u - umul(N - j - k, (uint32_t)s.w[j + k]) + (uint32_t)k * (uint32_t)s.t32[j + k - 1];
+      L.cf[k - 1][x] = (int32_t)cf;
+      L.cb[k - 1][x] = (int32_t)(0u - umul(N - j, wj) + (uint32_t)k * (uint32_t)s.t32[j]);
+    }
+  }
+  __syncthreads();
+  for (int sg = wave; sg < 4 * R; sg += kBlock / kWave) {
+    const int g = sg / R, by = b0 + sg % R;
+    const int nb = g == 0 ? 0 : g + 1;
+    if (!((mask >> nb) & 1)) continue;
+    int32_t bd = INT32_MAX, bc = -1;
+    if (g == 0) {
+      if (by > N - 5) continue;
+      const int i = by + 1;
+      const int jlo = max(q0, i + 2), jhi = min(q1, N - 1);
+      if (jlo >= jhi) continue;
+      const uint32_t Cs = 0u - umul(N - i, (uint32_t)s.w[i]) - umul(N - i - 1, (uint32_t)s.w[i + 1]);
+      const uint16_t* va = L.v[i - 1 - b0];
+      const uint16_t* vb = L.v[i + 1 - b0];
+      const uint16_t* vi = L.v[i - b0];
+      const uint32_t a1 = (uint32_t)(N - i), a2 = (uint32_t)(N - i - 1);
+      for (int j = jlo + lane; j < jhi; j += kWave) {
+        const int x = j - cbase;
+        const uint32_t nj = (uint32_t)(N - j);
+        uint32_t d = Cs + (uint32_t)L.cs[j - q0];
+        d += umul(a1, va[x]) + umul(a2, vb[x]) + umul(nj, vi[x - 1]) + umul(nj - 1u, vi[x + 1]);
+        if ((int32_t)d < bd) {
+          bd = (int32_t)d;
+          bc = j;
+        }
+      }
+    } else {
+      const int k = g;
+      if (by > N - k - 3) continue;
+      {  // backward part of the row: i = by + 2, j in [1, by + 1]
+        const int i = by + 2;
+        const int jlo = max(q0, 1), jhi = min(q1, by + 2);
+        if (jlo < jhi) {
+          const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
+          const uint32_t dB = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
+          const uint32_t Rc = (uint32_t)(N - i - k) * (dB - (uint32_t)s.w[i + k]) -
+                              (uint32_t)(N - i) * (uint32_t)s.w[i] + (uint32_t)i * S -
+                              (uint32_t)k * (uint32_t)s.t32[i - 1];
+          const uint16_t* vi = L.v[i - b0];
+          const uint16_t* vk = L.v[i + k - 1 - b0];
+          const int32_t* cb = L.cb[k - 1];
+          for (int j = jlo + lane; j < jhi; j += kWave) {
+            const int x = j - cbase;
+            const uint32_t nj = (uint32_t)(N - j);
+            uint32_t d = Rc + (uint32_t)cb[j - q0];
+            d += umul(nj, vi[x - 1]) + umul(nj - (uint32_t)k, vk[x]) - umul((uint32_t)j, S);
+            if ((int32_t)d < bd) {
+              bd = (int32_t)d;
+              bc = j - 1;
+            }
+          }
+        }
+      }
+      {  // forward part: i = by + 1, j in [by + 2, N - k - 1]
+        const int i = by + 1;
+        const int jlo = max(q0, by + 2), jhi = min(q1, N - k);
+        if (jlo < jhi) {
+          const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
+          const uint32_t dF = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
+          const uint32_t Rc = (uint32_t)(N - i) * (dF - (uint32_t)s.w[i]) -
+                              (uint32_t)(N - i - k) * (uint32_t)s.w[i + k] + (uint32_t)i * S -
+                              (uint32_t)k * (uint32_t)s.t32[i + k];
+          const uint16_t* vi = L.v[i - b0];
+          const uint16_t* vk = L.v[i + k - 1 - b0];
+          const int32_t* cf = L.cf[k - 1];
+          for (int j = jlo + lane; j < jhi; j += kWave) {
+            const int x = j - cbase;
+            const uint32_t nj = (uint32_t)(N - j);
+            uint32_t d = Rc + (uint32_t)cf[j - q0];
+            d += umul(nj - (uint32_t)k, vk[x + k]) + umul(nj, vi[x + k - 1]) - umul((uint32_t)j, S);
+            if ((int32_t)d < bd) {
+              bd = (int32_t)d;
+              bc = j - 1;
+            }
+          }
+        }
+      }
+    }
+    const int64_t key = wave_min64(pair_key(bd, bc));
+    if (lane == 0) fold_key(row_buf(s, nb, (parity >> nb) & 1) + by, key);
+  }
+}
+
+// int64 fallback of a fixed-i tile (column tile 0 only): one wave per
+// (neighborhood, row) over the whole row.
+template <int R>
+__device__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int parity) {
   const int N = s.N, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int sg = wave; sg < kSegs; sg += kBlock / kWave) {
-    const int g = sg / kR, by = b0 + sg % kR;
+  const int b0 = rb * R;
+  if (ct != 0) return;
+#pragma unroll
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    if (nb != 1 && ((mask >> nb) & 1))
+      reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+  for (int sg = wave; sg < 4 * R; sg += kBlock / kWave) {
+    const int g = sg / R, by = b0 + sg % R;
     const int nb = g == 0 ? 0 : g + 1;
     if (!((mask >> nb) & 1) || by >= rows_of(nb, 0, N)) continue;
-    if (nb == 0) eval_row<0, false>(s, nb, 0, by, lane);
-    else eval_row<2, false>(s, nb, 0, by, lane);
+    int64_t* out = row_buf(s, nb, (parity >> nb) & 1) + by;
+    if (nb == 0) eval_row<0, false>(s, nb, 0, by, lane, out);
+    else eval_row<2, false>(s, nb, 0, by, lane, out);
   }
 }
 
-__device__ void sweep_2opt(const SlotDev& s, int b0, DiagLds& L) {
+template <int C>
+__device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
-  const int by = b0 + wave;
-  const bool live = by < ((N + 1) >> 1);
-  uint16_t* e = L.e[wave];
-  int32_t bd = INT32_MAX, bc = -1;
+  const int rows = (N + 1) >> 1;
+  const int par = (parity >> 1) & 1;
+  if (ct == 0)
+    reset_other(s, 1, par, rb * MLP_2OPT_ROWS, min(rb * MLP_2OPT_ROWS + MLP_2OPT_ROWS, rows), rb == 0);
+  const int by = rb * MLP_2OPT_ROWS + wave;
+  const bool live = by < rows;
+  const int c0 = ct * C, c1 = min(c0 + C, N - 3);  // row-local indices [c0, c1)
+  // part 0: ctx < by  -> m = N-2-by, i = by - ctx  in [by - cb1 + 1, by - c0]
+  // part 1: ctx >= by -> m = by+1,   i = ctx-by+1  in [ca0 - by + 1, c1 - by]
+  int ilo[2], ihi[2], mm[2];
+  const int cb1 = min(c1, by), ca0 = max(c0, by);
+  mm[0] = N - 2 - by;
+  ilo[0] = by - cb1 + 1;
+  ihi[0] = (live && c0 < cb1) ? by - c0 : ilo[0] - 1;
+  mm[1] = by + 1;
+  ilo[1] = ca0 - by + 1;
+  ihi[1] = (live && ca0 < c1) ? c1 - by : ilo[1] - 1;
+#pragma unroll
   for (int part = 0; part < 2; ++part) {
-    // part 0: ctx < by, m = N-2-by, i = by - ctx in [1, by]
-    // part 1: ctx >= by, m = by+1,  i = ctx - by + 1 in [1, N-3-by]
-    const int m = part == 0 ? N - 2 - by : by + 1;
-    const int imax = live ? (part == 0 ? by : N - 3 - by) : 0;
-    for (int i0 = 1; i0 <= N - 3; i0 += k2Chunk) {
-      const int i1 = min(i0 + k2Chunk - 1, imax);
-      if (i0 <= i1) {
-        for (int x = lane; x <= i1 - i0 + 1; x += kWave) {
-          const int p = i0 - 1 + x;
-          e[x] = (uint16_t)dist_xy(s.xy[p], s.xy[p + m + 1], dr);
-        }
+    uint16_t* e = L.e[wave][part];
+    for (int x = lane; x <= ihi[part] - ilo[part] + 1; x += kWave) {
+      const int p = ilo[part] - 1 + x;
+      e[x] = (uint16_t)dist_xy(s.xy[p], s.xy[p + mm[part] + 1], dr);
+    }
+  }
+  __syncthreads();
+  int32_t bd = INT32_MAX, bc = -1;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const uint16_t* e = L.e[wave][part];
+    const int m = mm[part];
+    for (int i = ilo[part] + lane; i <= ihi[part]; i += kWave) {
+      const int j = i + m;
+      const int x = i - ilo[part];
+      uint32_t d = (uint32_t)s.g32[i] + (uint32_t)s.h32[j];
+      d += umul((uint32_t)(N - i), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
+      d += (uint32_t)m * ((uint32_t)s.t32[i] + (uint32_t)s.t32[j]);
+      const int ctx = part == 0 ? by - i : i + by - 1;
+      if ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc)) {
+        bd = (int32_t)d;
+        bc = ctx;
       }
-      __syncthreads();
-      if (i0 <= i1) {
-        for (int i = i0 + lane; i <= i1; i += kWave) {
-          const int j = i + m;
-          const int x = i - i0;
-          uint32_t d = (uint32_t)s.g32[i] + (uint32_t)s.h32[j];
-          d += umul((uint32_t)(N - i), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
-          d += (uint32_t)m * ((uint32_t)s.t32[i] + (uint32_t)s.t32[j]);
-          const int ctx = part == 0 ? by - i : i + by - 1;
-          if ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc)) {
-            bd = (int32_t)d;
-            bc = ctx;
-          }
-        }
-      }
-      __syncthreads();
     }
   }
   const int64_t key = wave_min64(pair_key(bd, bc));
-  if (live && lane == 0) s.rows[1][by] = key;
+  if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, key);
 }
 
-__device__ void sweep_2opt_wide(const SlotDev& s, int b0) {
+__device__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int by = b0 + wave;
-  if (by < ((s.N + 1) >> 1)) eval_row<1, false>(s, 1, 0, by, lane);
+  const int rows = (s.N + 1) >> 1;
+  const int par = (parity >> 1) & 1;
+  if (ct != 0) return;
+  reset_other(s, 1, par, rb * MLP_2OPT_ROWS, min(rb * MLP_2OPT_ROWS + MLP_2OPT_ROWS, rows), rb == 0);
+  const int by = rb * MLP_2OPT_ROWS + wave;
+  if (by < rows) eval_row<1, false>(s, 1, 0, by, lane, row_buf(s, 1, par) + by);
 }
 
 __device__ __forceinline__ SweepSeg find_sweep(const SweepSeg* segs, int nsegs, int item) {
@@ -897,30 +951,33 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
   return sg;
 }
 
+template <int R, int C>
 __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds& L) {
+                           SweepLds<R, C>& L) {
   const SlotDev& s = table[sg.slot];
   const int b = item - sg.blk0;
+  const int rb = b / sg.ncols, ct = b % sg.ncols;
   const bool p32 = s.info[MLP_INFO_PATH32] != 0;
   if (sg.kind == 0) {
-    if (p32) sweep_fixed(s, b * kR, sg.mask, L.t);
-    else sweep_fixed_wide(s, b * kR, sg.mask);
+    if (p32) sweep_fixed<R, C>(s, rb, ct, sg.mask, sg.parity, L.t);
+    else sweep_fixed_wide<R>(s, rb, ct, sg.mask, sg.parity);
   } else {
-    if (p32) sweep_2opt(s, b * MLP_2OPT_ROWS, L.d);
-    else sweep_2opt_wide(s, b * MLP_2OPT_ROWS);
+    if (p32) sweep_2opt<C>(s, rb, ct, sg.parity, L.d);
+    else sweep_2opt_wide(s, rb, ct, sg.parity);
   }
 }
 
 }  // namespace
 
+template <int R, int C>
 __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter) {
-  __shared__ SweepLds L;
+  __shared__ SweepLds<R, C> L;
   if (!counter) {
     const int item = blockIdx.x;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item(table, sg, item, L);
+    sweep_item<R, C>(table, sg, item, L);
     return;
   }
   __shared__ int s_item;
@@ -931,10 +988,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ ta
     __syncthreads();
     if (item >= total) break;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item(table, sg, item, L);
+    sweep_item<R, C>(table, sg, item, L);
     __syncthreads();
   }
 }
+
+template __global__ void k_sweep<4, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                         int, int*);
+template __global__ void k_sweep<8, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                          int, int*);
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -962,12 +1024,21 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
   return hipGetLastError();
 }
 
-hipError_t mlp_launch_sweep(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
-                            const SweepSeg* segs_dev, int nsegs, int total, int* counter,
-                            int grid) {
-  hipLaunchKernelGGL(k_sweep, dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs, total,
-                     counter);
+hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
+                            const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
+                            int total, int* counter, int grid) {
+  if (tile == MLP_TILE_SMALL)
+    hipLaunchKernelGGL((k_sweep<4, 256>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
+                       nsegs, total, counter);
+  else
+    hipLaunchKernelGGL((k_sweep<8, 1024>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
+                       nsegs, total, counter);
   return hipGetLastError();
+}
+
+void mlp_tile_shape(int tile, int* rows, int* cols) {
+  *rows = tile == MLP_TILE_SMALL ? 4 : 8;
+  *cols = tile == MLP_TILE_SMALL ? 256 : 1024;
 }
 
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {

@@ -15,9 +15,10 @@ hipError_t mlp_launch_round_reset(hipStream_t st, const SlotDev* table, const Sl
 hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsInline& inl,
                            const EvalSeg* segs_dev, int nsegs, int total, int rowmap,
                            int* counter, int grid);
-hipError_t mlp_launch_sweep(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
-                            const SweepSeg* segs_dev, int nsegs, int total, int* counter,
-                            int grid);
+hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
+                            const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
+                            int total, int* counter, int grid);
+void mlp_tile_shape(int tile, int* rows, int* cols);
 size_t mlp_merge_lds_bytes(int pcap, int ncap);
 hipError_t mlp_merge_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,

@@ -36,6 +36,7 @@ struct Slot {
   void* mem = nullptr;
   SlotDev dev{};
   int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
+  int parity[MLP_NB_COUNT] = {0, 0, 0, 0, 0};  // row buffer of the last eval
   bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
 };
 
@@ -67,6 +68,13 @@ void move_ij(int nb, int rowmap, int N, int row, int ctx, int& i, int& j) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Entries of one row buffer: the most rows any neighborhood has in any row map.
+int row_cap(int N) {
+  int r = 1;
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) r = std::max(r, std::max(rows_of(nb, 0, N), rows_of(nb, 1, N)));
+  return r;
+}
+
 // Byte layout of one slot's device block.
 struct SlotLayout {
   size_t tour, xy, w, t32, q32, g32, h32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
@@ -92,8 +100,8 @@ SlotLayout slot_layout(int N) {
   L.q64 = take(8 * (size_t)N);
   L.info = take(8 * MLP_INFO_WORDS);
   for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
-    const size_t r = (size_t)std::max(1, std::max(rows_of(nb, 0, N), rows_of(nb, 1, N)));
-    L.rows[nb] = take(8 * r);
+    const size_t r = (size_t)row_cap(N);
+    L.rows[nb] = take(16 * r);
     L.kc[nb] = take(8 * r);
     L.kij[nb] = take(12 * r);
   }
@@ -124,6 +132,7 @@ struct mlp_ctx {
   size_t i64_cap = 0;
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
+  int tile_override = -1;  // -1 auto, else MLP_TILE_SMALL / MLP_TILE_LARGE
   // per-launch timing of k_eval (HIP events on the context stream)
   bool timing = false;
   std::vector<hipEvent_t> tev;  // pairs
@@ -421,6 +430,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   d.n = in.n;
   d.round = in.round;
   d.maxd = in.maxd;
+  d.rowcap = row_cap(N);
   d.xs = in.xs;
   d.ys = in.ys;
   d.tour = (int32_t*)(b + L.tour);
@@ -437,6 +447,11 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
     d.rows[nb] = (int64_t*)(b + L.rows[nb]);
     d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
     d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
+  }
+  {
+    std::vector<int64_t> none(2 * (size_t)d.rowcap, MLP_KEY_NONE);
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      HCK(hipMemcpy(d.rows[nb], none.data(), 8 * none.size(), hipMemcpyHostToDevice));
   }
   HCK(hipMemcpy(ctx->d_table + id, &d, sizeof(SlotDev), hipMemcpyHostToDevice));
   s.alive = true;
@@ -578,8 +593,8 @@ hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
 
 hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
   EvalArgs& a = *(EvalArgs*)p;
-  return mlp_launch_sweep(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total, a.counter,
-                          a.grid);
+  return mlp_launch_sweep(ctx->stream, a.rowmap, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total,
+                          a.counter, a.grid);
 }
 
 }  // namespace
@@ -600,44 +615,58 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   }
   EvalArgs a{};
   if (rowmap == 0) {
-    // ADS row map: fixed-i row blocks (swap + or-opt) and 2-opt diagonal rows
+    // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
+    int tile = ctx->tile_override;
+    if (tile < 0) tile = count <= 8 ? MLP_TILE_SMALL : MLP_TILE_LARGE;
+    int TR, TC;
+    mlp_tile_shape(tile, &TR, &TC);
     std::vector<SweepSeg> segs;
     segs.reserve((size_t)count * 2);
     int total = 0;
     const uint32_t fixed = nb_mask & 0x1du;
     for (int q = 0; q < count; ++q) {
-      const Slot& s = ctx->slots[slots[q]];
+      Slot& s = ctx->slots[slots[q]];
+      int par = 0;
+      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+        if (nb_mask & (1u << nb)) {
+          s.parity[nb] ^= 1;
+          par |= s.parity[nb] << nb;
+        }
       if (fixed) {
         int rmax = 0;
         for (int nb : {0, 2, 3, 4})
           if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
         if (rmax > 0) {
-          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total});
-          total += (rmax + MLP_TILE_ROWS - 1) / MLP_TILE_ROWS;
+          const int ncols = (s.N - 2 + TC - 1) / TC;
+          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, par});
+          total += (rmax + TR - 1) / TR * ncols;
         }
       }
       if (nb_mask & 2u) {
         const int r = rows_of(1, 0, s.N);
         if (r > 0 && s.N >= 4) {
-          segs.push_back(SweepSeg{slots[q], 1, 2, total});
-          total += (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS;
+          const int ncols = (s.N - 3 + TC - 1) / TC;
+          segs.push_back(SweepSeg{slots[q], 1, 2, total, ncols, par});
+          total += (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
         }
       }
     }
     if (segs.empty()) return MLP_OK;
     a.nsegs = (int)segs.size();
     a.total = total;
+    a.rowmap = tile;
     if (a.nsegs <= MLP_SEGS_INLINE) {
       for (int q = 0; q < a.nsegs; ++q) a.sinl.s[q] = segs[(size_t)q];
     } else {
-      st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * 4, (const int32_t**)&a.sdev);
+      st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * MLP_SWEEPSEG_INTS,
+                       (const int32_t**)&a.sdev);
       if (st) return st;
     }
     a.grid = total;
     if (persistent) {
       a.counter = ctx->d_counter;
       HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
-      a.grid = std::min(total, ctx->cu_count * 3);
+      a.grid = std::min(total, ctx->cu_count * (tile == MLP_TILE_SMALL ? 8 : 3));
     }
     return timed_launch(ctx, launch_sweep_cb, &a);
   }
@@ -645,12 +674,13 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   segs.reserve((size_t)count * 5);
   int total = 0;
   for (int q = 0; q < count; ++q) {
-    const Slot& s = ctx->slots[slots[q]];
+    Slot& s = ctx->slots[slots[q]];
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
       if (!(nb_mask & (1u << nb))) continue;
       const int R = rows_of(nb, rowmap, s.N);
       if (R <= 0) continue;
-      segs.push_back(EvalSeg{slots[q], nb, R, total});
+      s.parity[nb] ^= 1;
+      segs.push_back(EvalSeg{slots[q], nb, R, total, s.parity[nb]});
       total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
     }
   }
@@ -661,7 +691,8 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   if (a.nsegs <= MLP_SEGS_INLINE) {
     for (int q = 0; q < a.nsegs; ++q) a.inl.s[q] = segs[(size_t)q];
   } else {
-    st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * 4, (const int32_t**)&a.dev);
+    st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * MLP_EVALSEG_INTS,
+                     (const int32_t**)&a.dev);
     if (st) return st;
   }
   a.grid = total;
@@ -671,6 +702,14 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
     a.grid = std::min(total, ctx->cu_count * 8);
   }
   return timed_launch(ctx, launch_eval_cb, &a);
+}
+
+int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
+  if (!ctx) return MLP_EINVAL;
+  if (tile != -1 && tile != MLP_TILE_SMALL && tile != MLP_TILE_LARGE)
+    return fail(ctx, MLP_EINVAL, "tile must be -1 (auto), 0 (small) or 1 (large)");
+  ctx->tile_override = tile;
+  return MLP_OK;
 }
 
 int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity) {
@@ -723,7 +762,8 @@ int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t*
   if (R == 0) return MLP_OK;
   std::vector<int64_t> keys((size_t)R);
   HCK(hipStreamSynchronize(ctx->stream));
-  HCK(hipMemcpy(keys.data(), s.dev.rows[nb], 8 * (size_t)R, hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(keys.data(), s.dev.rows[nb] + (s.parity[nb] ? s.dev.rowcap : 0), 8 * (size_t)R,
+                hipMemcpyDeviceToHost));
   for (int r = 0; r < R; ++r) {
     const int64_t k = keys[(size_t)r];
     int ii = 0, jj = 0;

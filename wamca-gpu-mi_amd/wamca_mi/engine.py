@@ -182,6 +182,10 @@ class Engine:
         self._ck(self._L.mlp_dist_list(self.h, s, len(s), rnd, out))
         return out
 
+    def set_tile(self, tile: int):
+        """-1 automatic, 0 small tiles (single tours), 1 large tiles (batches)."""
+        self._ck(self._L.mlp_set_tile(self.h, tile))
+
     def timing_begin(self, capacity=8192):
         self._ck(self._L.mlp_timing_begin(self.h, capacity))
 
