@@ -139,6 +139,9 @@ int mlp_ads_materialize(mlp_ctx* ctx, int32_t slot, uint32_t* host_out, int64_t 
 /* Device-side distance function over a range of squared lengths s (for
  * exhaustive checks against the host): out[q] = d(s0 + q). */
 int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32_t* out);
+/* Mismatch count of the kernels' square root against the library's correctly
+ * rounded sqrtf for every float(s), s in [0, 2^31) (0 expected). */
+int mlp_sqrt_selfcheck(mlp_ctx* ctx, int64_t* mismatches);
 /* Same for an explicit list of squared lengths. */
 int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out);
 

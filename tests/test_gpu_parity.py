@@ -322,3 +322,9 @@ def test_tile_shapes_and_row_buffers(eng, tile, n):
                 _check_rows(eng, s, W, t, nb, rowmap)
     finally:
         eng.set_tile(-1)
+
+
+def test_sqrt_exhaustive_vs_library(eng):
+    """The kernels' square root equals the library's correctly rounded sqrtf
+    for float(s) of every s in [0, 2^31)."""
+    assert eng.sqrt_selfcheck() == 0

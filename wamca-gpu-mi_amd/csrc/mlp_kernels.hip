@@ -29,10 +29,33 @@ constexpr int kBlock = 256;
 // ---------------------------------------------------------------------------
 // distance
 // ---------------------------------------------------------------------------
+// IEEE correctly rounded square root of f = float(s), s an integer in
+// [0, 2^31): the hardware estimate followed by the two-residual correction to
+// the nearest float (the same correction LLVM emits for sqrtf); the denormal
+// scaling and special-value handling of the general routine are not needed
+// for these inputs.  mlp_sqrt_selfcheck compares it with __builtin_sqrtf for
+// every s in [0, 2^31).
+__device__ __forceinline__ float sqrt_rn_int(float f) {
+  float r = __builtin_amdgcn_sqrtf(f);
+  const float rm = __int_as_float(__float_as_int(r) - 1);
+  const float rp = __int_as_float(__float_as_int(r) + 1);
+  const float em = __builtin_fmaf(-rm, r, f);
+  const float ep = __builtin_fmaf(-rp, r, f);
+  r = em <= 0.0F ? rm : r;
+  r = ep > 0.0F ? rp : r;
+  return r;
+}
+
 __device__ __forceinline__ int dist_s(int s, float dr) {
-  const float f = (float)s;           // v_cvt_f32_i32, round to nearest
-  const float r = __builtin_sqrtf(f);  // correctly rounded (checked exhaustively)
-  return (int)(r + dr);               // RN add, truncating convert
+  const float f = (float)(uint32_t)s;  // round to nearest
+  return (int)(sqrt_rn_int(f) + dr);  // RN add, truncating convert
+}
+
+// Global-address-space load of a slot array element (the slot table stores
+// generic pointers; this keeps the access a global_load).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
 }
 
 __device__ __forceinline__ int dist_xy(int a, int b, float dr) {
@@ -636,6 +659,19 @@ __global__ void k_dist_table(int s0, int count, int round, int32_t* out) {
   if (q < count) out[q] = dist_s(s0 + q, round_dr(round));
 }
 
+// Mismatches of sqrt_rn_int against the library's correctly rounded sqrtf
+// over s in [s0, s0 + count).
+__global__ void k_sqrt_selfcheck(uint32_t s0, uint32_t count, unsigned long long* bad) {
+  const uint32_t base = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  unsigned long long nb = 0;
+  for (uint32_t q = base; q < count; q += stride) {
+    const float f = (float)(s0 + q);
+    nb += (__float_as_uint(sqrt_rn_int(f)) != __float_as_uint(__builtin_sqrtf(f))) ? 1ull : 0ull;
+  }
+  if (nb) atomicAdd(bad, nb);
+}
+
 __global__ void k_dist_list(const int32_t* s, int count, int round, int32_t* out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < count) out[q] = dist_s(s[q], round_dr(round));
@@ -683,10 +719,10 @@ namespace {
 
 template <int R, int C>
 struct TileLds {
-  uint16_t v[R + 4][C + 4];
-  int32_t cs[C];
-  int32_t cf[3][C];
-  int32_t cb[3][C];
+  uint16_t v[R + 4][C + 4];  // v[a - b0][c - cbase] = d(tour[a], tour[c])
+  int32_t w[C + 4];          // w[c] staged at c - cbase
+  int32_t t[C + 4];          // T[c] (mod 2^32) staged at c - cbase
+  int64_t wb[4][4][R];       // per-wave partial keys [wave][group][row]
 };
 
 template <int C>
@@ -701,6 +737,8 @@ union SweepLds {
 };
 
 __device__ __forceinline__ uint32_t umul(uint32_t a, uint32_t b) { return __umul24(a, b); }
+// signed 24-bit multiply, wrapped to 32 bits (|a|, |b| < 2^23)
+__device__ __forceinline__ uint32_t smul(int a, int b) { return (uint32_t)__mul24(a, b); }
 
 __device__ __forceinline__ int64_t pair_key(int32_t d, int32_t c) {
   return c < 0 ? MLP_KEY_NONE : (int64_t)d * (int64_t)(1 << MLP_KEY_SHIFT) + c;
@@ -714,6 +752,10 @@ __device__ __forceinline__ void fold_key(int64_t* p, int64_t key) {
   if (key != MLP_KEY_NONE) atomicMin((long long*)p, (long long)key);
 }
 
+__device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
 // Reset of the other row buffer by the tiles of column 0; the first tile also
 // clears the tail beyond this row map and records the parity.
 __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
@@ -725,6 +767,174 @@ __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, b
       s.rows[nb][(par ? s.rowcap : 0) + r] = MLP_KEY_NONE;
     }
     if (threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
+  }
+}
+
+// Swap rows of a tile: lane = column j, rows r unrolled.  Row r: i = b0+r+1,
+// j in [i+2, N-2].  delta = Cs_r + Bs(j) + (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j]
+// + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
+template <int R, int C>
+__device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
+                          int wave, int lane) {
+  const int N = s.N;
+  const int rv = min(R, N - 4 - b0);  // rows with i <= N-4
+  int32_t bd[R], bc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    bd[r] = INT32_MAX;
+    bc[r] = -1;
+  }
+  if (rv > 0) {
+    uint32_t myCs = 0u;
+    if (lane < rv) {
+      const int i = b0 + lane + 1;
+      myCs = 0u - umul(N - i, (uint32_t)ldg(s.w + i)) - umul(N - i - 1, (uint32_t)ldg(s.w + i + 1));
+    }
+    uint32_t Cs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) Cs[r] = lane_bcast(myCs, r);
+    const int jhi = min(q1, N - 1);
+    const int jlo = max(q0, b0 + 3);
+    for (int jb = q0 + wave * kWave; jb < jhi; jb += kBlock) {
+      if (jb + kWave <= jlo) continue;
+      const int j = jb + lane;
+      const int jj = min(j, jhi - 1);
+      const int x = jj - cbase;
+      const uint32_t nj = (uint32_t)(N - jj);
+      const uint32_t cs = 0u - umul(nj, (uint32_t)L.w[x]) - umul(nj - 1u, (uint32_t)L.w[x + 1]);
+      uint32_t c[R + 2], lf[R], rt[R];
+#pragma unroll
+      for (int a = 0; a < R + 2; ++a) c[a] = L.v[a][x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        lf[r] = L.v[r + 1][x - 1];
+        rt[r] = L.v[r + 1][x + 1];
+      }
+      const bool full = jb >= b0 + R + 2 && jb + kWave <= jhi;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r < rv) {
+          const uint32_t a1 = (uint32_t)(N - b0 - r - 1);
+          const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) + umul(nj, lf[r]) +
+                             umul(nj - 1u, rt[r]);
+          const bool ok = full || (j < jhi && j >= b0 + r + 3);
+          if (ok && (int32_t)d < bd[r]) {
+            bd[r] = (int32_t)d;
+            bc[r] = j;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t key = wave_min64(pair_key(bd[r], bc[r]));
+    if (lane == 0) L.wb[wave][0][r] = key;
+  }
+}
+
+// Or-opt-K rows of a tile.  Row r (by = b0+r): forward i = by+1 for j >= by+2,
+// backward i = by+2 for j <= by+1, j in [1, N-K-1]; row-local index j - 1.
+template <int R, int C, int K>
+__device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
+                           int wave, int lane) {
+  const int N = s.N;
+  const float dr = round_dr(s.round);
+  const int rv = min(R, N - K - 2 - b0);  // rows with by <= N-K-3
+  int32_t bd[R], bc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    bd[r] = INT32_MAX;
+    bc[r] = -1;
+  }
+  if (rv > 0) {
+    uint32_t mRF = 0u, mRB = 0u;
+    int mSF = 0, mSB = 0;
+    if (lane < rv) {
+      const int iF = b0 + lane + 1, iB = b0 + lane + 2;
+      const uint32_t SF = (uint32_t)ldg(s.t32 + iF + K - 1) - (uint32_t)ldg(s.t32 + iF);
+      const uint32_t SB = (uint32_t)ldg(s.t32 + iB + K - 1) - (uint32_t)ldg(s.t32 + iB);
+      const uint32_t dF = (uint32_t)dist_xy(ldg(s.xy + iF - 1), ldg(s.xy + iF + K), dr);
+      const uint32_t dB = (uint32_t)dist_xy(ldg(s.xy + iB - 1), ldg(s.xy + iB + K), dr);
+      mRF = (uint32_t)(N - iF) * (dF - (uint32_t)ldg(s.w + iF)) -
+            (uint32_t)(N - iF - K) * (uint32_t)ldg(s.w + iF + K) + (uint32_t)iF * SF -
+            (uint32_t)K * (uint32_t)ldg(s.t32 + iF + K);
+      mRB = (uint32_t)(N - iB - K) * (dB - (uint32_t)ldg(s.w + iB + K)) -
+            (uint32_t)(N - iB) * (uint32_t)ldg(s.w + iB) + (uint32_t)iB * SB -
+            (uint32_t)K * (uint32_t)ldg(s.t32 + iB - 1);
+      mSF = -(int)SF;
+      mSB = -(int)SB;
+    }
+    uint32_t RF[R], RB[R];
+    int nSF[R], nSB[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      RF[r] = lane_bcast(mRF, r);
+      RB[r] = lane_bcast(mRB, r);
+      nSF[r] = (int)lane_bcast((uint32_t)mSF, r);
+      nSB[r] = (int)lane_bcast((uint32_t)mSB, r);
+    }
+    const int jhi = min(q1, N - K);
+    for (int jb = q0 + wave * kWave; jb < jhi; jb += kBlock) {
+      const int j = jb + lane;
+      const int jj = min(j, jhi - 1);
+      const int x = jj - cbase;
+      const uint32_t nj = (uint32_t)(N - jj);
+      const bool tail = jb + kWave > jhi;
+      const bool allF = !tail && jb >= b0 + R + 1;
+      const bool allB = !tail && jb + kWave - 1 <= b0 + 1;
+      if (allF) {
+        const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r < rv) {
+            uint32_t d = RF[r] + CF + umul(nj - K, L.v[r + K][x + K]) + umul(nj, L.v[r + 1][x + K - 1]);
+            if (K > 1) d += smul(jj, nSF[r]);
+            if ((int32_t)d < bd[r]) {
+              bd[r] = (int32_t)d;
+              bc[r] = j - 1;
+            }
+          }
+        }
+      } else if (allB) {
+        const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r < rv) {
+            uint32_t d = RB[r] + CB + umul(nj, L.v[r + 2][x - 1]) + umul(nj - K, L.v[r + K + 1][x]);
+            if (K > 1) d += smul(jj, nSB[r]);
+            if ((int32_t)d < bd[r]) {
+              bd[r] = (int32_t)d;
+              bc[r] = j - 1;
+            }
+          }
+        }
+      } else {
+        const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
+        const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r < rv) {
+            uint32_t dF = RF[r] + CF + umul(nj - K, L.v[r + K][x + K]) + umul(nj, L.v[r + 1][x + K - 1]);
+            uint32_t dB = RB[r] + CB + umul(nj, L.v[r + 2][x - 1]) + umul(nj - K, L.v[r + K + 1][x]);
+            if (K > 1) {
+              dF += smul(jj, nSF[r]);
+              dB += smul(jj, nSB[r]);
+            }
+            const uint32_t d = jj >= b0 + r + 2 ? dF : dB;
+            if (j < jhi && (int32_t)d < bd[r]) {
+              bd[r] = (int32_t)d;
+              bc[r] = j - 1;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t key = wave_min64(pair_key(bd[r], bc[r]));
+    if (lane == 0) L.wb[wave][K][r] = key;
   }
 }
 
@@ -744,108 +954,30 @@ __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int pari
       if (nb != 1 && ((mask >> nb) & 1))
         reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
   }
+  int xa[R + 4];
+#pragma unroll
+  for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
   for (int c = cbase + tid; c < cend; c += kBlock) {
-    const int xc = s.xy[c];
+    const int xc = ldg(s.xy + c);
 #pragma unroll
-    for (int va = 0; va < R + 4; ++va) {
-      const int a = b0 + va;
-      if (a < N) L.v[va][c - cbase] = (uint16_t)dist_xy(s.xy[a], xc, dr);
-    }
-  }
-  for (int j = q0 + tid; j < q1; j += kBlock) {
-    const int x = j - q0;
-    const uint32_t wj = (uint32_t)s.w[j];
-    const uint32_t wj1 = j + 1 < N ? (uint32_t)s.w[j + 1] : 0u;
-    L.cs[x] = (int32_t)(0u - umul(N - j, wj) - umul(N - j - 1, wj1));
-#pragma unroll
-    for (int k = 1; k <= 3; ++k) {
-      uint32_t cf = 0u;
-      if (j + k <= N - 1)
-        cf = 0u - umul(N - j - k, (uint32_t)s.w[j + k]) + (uint32_t)k * (uint32_t)s.t32[j + k - 1];
-      L.cf[k - 1][x] = (int32_t)cf;
-      L.cb[k - 1][x] = (int32_t)(0u - umul(N - j, wj) + (uint32_t)k * (uint32_t)s.t32[j]);
-    }
+    for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc, dr);
+    L.w[c - cbase] = ldg(s.w + c);
+    L.t[c - cbase] = ldg(s.t32 + c);
   }
   __syncthreads();
-  for (int sg = wave; sg < 4 * R; sg += kBlock / kWave) {
-    const int g = sg / R, by = b0 + sg % R;
+  if (mask & 1) tile_swap<R, C>(s, b0, q0, q1, cbase, L, wave, lane);
+  if (mask & 4) tile_oropt<R, C, 1>(s, b0, q0, q1, cbase, L, wave, lane);
+  if (mask & 8) tile_oropt<R, C, 2>(s, b0, q0, q1, cbase, L, wave, lane);
+  if (mask & 16) tile_oropt<R, C, 3>(s, b0, q0, q1, cbase, L, wave, lane);
+  __syncthreads();
+  for (int q = tid; q < 4 * R; q += kBlock) {
+    const int g = q / R, r = q % R, by = b0 + r;
     const int nb = g == 0 ? 0 : g + 1;
-    if (!((mask >> nb) & 1)) continue;
-    int32_t bd = INT32_MAX, bc = -1;
-    if (g == 0) {
-      if (by > N - 5) continue;
-      const int i = by + 1;
-      const int jlo = max(q0, i + 2), jhi = min(q1, N - 1);
-      if (jlo >= jhi) continue;
-      const uint32_t Cs = 0u - umul(N - i, (uint32_t)s.w[i]) - umul(N - i - 1, (uint32_t)s.w[i + 1]);
-      const uint16_t* va = L.v[i - 1 - b0];
-      const uint16_t* vb = L.v[i + 1 - b0];
-      const uint16_t* vi = L.v[i - b0];
-      const uint32_t a1 = (uint32_t)(N - i), a2 = (uint32_t)(N - i - 1);
-      for (int j = jlo + lane; j < jhi; j += kWave) {
-        const int x = j - cbase;
-        const uint32_t nj = (uint32_t)(N - j);
-        uint32_t d = Cs + (uint32_t)L.cs[j - q0];
-        d += umul(a1, va[x]) + umul(a2, vb[x]) + umul(nj, vi[x - 1]) + umul(nj - 1u, vi[x + 1]);
-        if ((int32_t)d < bd) {
-          bd = (int32_t)d;
-          bc = j;
-        }
-      }
-    } else {
-      const int k = g;
-      if (by > N - k - 3) continue;
-      {  // backward part of the row: i = by + 2, j in [1, by + 1]
-        const int i = by + 2;
-        const int jlo = max(q0, 1), jhi = min(q1, by + 2);
-        if (jlo < jhi) {
-          const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
-          const uint32_t dB = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
-          const uint32_t Rc = (uint32_t)(N - i - k) * (dB - (uint32_t)s.w[i + k]) -
-                              (uint32_t)(N - i) * (uint32_t)s.w[i] + (uint32_t)i * S -
-                              (uint32_t)k * (uint32_t)s.t32[i - 1];
-          const uint16_t* vi = L.v[i - b0];
-          const uint16_t* vk = L.v[i + k - 1 - b0];
-          const int32_t* cb = L.cb[k - 1];
-          for (int j = jlo + lane; j < jhi; j += kWave) {
-            const int x = j - cbase;
-            const uint32_t nj = (uint32_t)(N - j);
-            uint32_t d = Rc + (uint32_t)cb[j - q0];
-            d += umul(nj, vi[x - 1]) + umul(nj - (uint32_t)k, vk[x]) - umul((uint32_t)j, S);
-            if ((int32_t)d < bd) {
-              bd = (int32_t)d;
-              bc = j - 1;
-            }
-          }
-        }
-      }
-      {  // forward part: i = by + 1, j in [by + 2, N - k - 1]
-        const int i = by + 1;
-        const int jlo = max(q0, by + 2), jhi = min(q1, N - k);
-        if (jlo < jhi) {
-          const uint32_t S = (uint32_t)s.t32[i + k - 1] - (uint32_t)s.t32[i];
-          const uint32_t dF = (uint32_t)dist_xy(s.xy[i - 1], s.xy[i + k], dr);
-          const uint32_t Rc = (uint32_t)(N - i) * (dF - (uint32_t)s.w[i]) -
-                              (uint32_t)(N - i - k) * (uint32_t)s.w[i + k] + (uint32_t)i * S -
-                              (uint32_t)k * (uint32_t)s.t32[i + k];
-          const uint16_t* vi = L.v[i - b0];
-          const uint16_t* vk = L.v[i + k - 1 - b0];
-          const int32_t* cf = L.cf[k - 1];
-          for (int j = jlo + lane; j < jhi; j += kWave) {
-            const int x = j - cbase;
-            const uint32_t nj = (uint32_t)(N - j);
-            uint32_t d = Rc + (uint32_t)cf[j - q0];
-            d += umul(nj - (uint32_t)k, vk[x + k]) + umul(nj, vi[x + k - 1]) - umul((uint32_t)j, S);
-            if ((int32_t)d < bd) {
-              bd = (int32_t)d;
-              bc = j - 1;
-            }
-          }
-        }
-      }
-    }
-    const int64_t key = wave_min64(pair_key(bd, bc));
-    if (lane == 0) fold_key(row_buf(s, nb, (parity >> nb) & 1) + by, key);
+    if (!((mask >> nb) & 1) || by >= rows_of(nb, 0, N)) continue;
+    int64_t key = L.wb[0][g][r];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; ++w) key = min(key, L.wb[w][g][r]);
+    fold_key(row_buf(s, nb, (parity >> nb) & 1) + by, key);
   }
 }
 
@@ -896,7 +1028,7 @@ __device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds
     uint16_t* e = L.e[wave][part];
     for (int x = lane; x <= ihi[part] - ilo[part] + 1; x += kWave) {
       const int p = ilo[part] - 1 + x;
-      e[x] = (uint16_t)dist_xy(s.xy[p], s.xy[p + mm[part] + 1], dr);
+      e[x] = (uint16_t)dist_xy(ldg(s.xy + p), ldg(s.xy + p + mm[part] + 1), dr);
     }
   }
   __syncthreads();
@@ -908,9 +1040,9 @@ __device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds
     for (int i = ilo[part] + lane; i <= ihi[part]; i += kWave) {
       const int j = i + m;
       const int x = i - ilo[part];
-      uint32_t d = (uint32_t)s.g32[i] + (uint32_t)s.h32[j];
+      uint32_t d = (uint32_t)ldg(s.g32 + i) + (uint32_t)ldg(s.h32 + j);
       d += umul((uint32_t)(N - i), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
-      d += (uint32_t)m * ((uint32_t)s.t32[i] + (uint32_t)s.t32[j]);
+      d += (uint32_t)m * ((uint32_t)ldg(s.t32 + i) + (uint32_t)ldg(s.t32 + j));
       const int ctx = part == 0 ? by - i : i + by - 1;
       if ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc)) {
         bd = (int32_t)d;
@@ -1079,6 +1211,12 @@ hipError_t mlp_launch_dist_list(hipStream_t st, const int32_t* s, int count, int
                                 int32_t* out) {
   hipLaunchKernelGGL(k_dist_list, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s,
                      count, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_sqrt_selfcheck(hipStream_t st, uint32_t s0, uint32_t count,
+                                     unsigned long long* bad) {
+  hipLaunchKernelGGL(k_sqrt_selfcheck, dim3(4096), dim3(kBlock), 0, st, s0, count, bad);
   return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, u
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);
 hipError_t mlp_launch_dist_list(hipStream_t st, const int32_t* s, int count, int round,
                                 int32_t* out);
+hipError_t mlp_launch_sqrt_selfcheck(hipStream_t st, uint32_t s0, uint32_t count,
+                                     unsigned long long* bad);
 hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* ys, int n,
                               int round, int32_t* out);
 hipError_t mlp_launch_gather_info(hipStream_t st, const SlotDev* table, const int32_t* ids,

@@ -947,6 +947,21 @@ int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32
   return MLP_OK;
 }
 
+int mlp_sqrt_selfcheck(mlp_ctx* ctx, int64_t* mismatches) {
+  if (!ctx || !mismatches) return MLP_EINVAL;
+  unsigned long long* d = nullptr;
+  HCK(hipMalloc(&d, 8));
+  hipError_t e = hipMemsetAsync(d, 0, 8, ctx->stream);
+  if (e == hipSuccess) e = mlp_launch_sqrt_selfcheck(ctx->stream, 0u, 0x80000000u, d);
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "sqrt selfcheck: %s", hipGetErrorString(e));
+  *mismatches = (int64_t)h;
+  return MLP_OK;
+}
+
 int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out) {
   if (!ctx || !out || !s || count < 0) return MLP_EINVAL;
   for (int q = 0; q < count; ++q)

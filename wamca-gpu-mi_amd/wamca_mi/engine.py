@@ -182,6 +182,11 @@ class Engine:
         self._ck(self._L.mlp_dist_list(self.h, s, len(s), rnd, out))
         return out
 
+    def sqrt_selfcheck(self) -> int:
+        out = C.c_int64()
+        self._ck(self._L.mlp_sqrt_selfcheck(self.h, C.byref(out)))
+        return out.value
+
     def set_tile(self, tile: int):
         """-1 automatic, 0 small tiles (single tours), 1 large tiles (batches)."""
         self._ck(self._L.mlp_set_tile(self.h, tile))
