@@ -185,6 +185,30 @@ __device__ __forceinline__ int64_t delta_oropt(const SlotDev& s, int k, int i, i
   }
 }
 
+// Wave-wide minimum of an int32 with DPP (row_shr 1/2/4/8 inside each
+// 16-lane row, then row_bcast 15 and 31 across rows); the result is read from
+// lane 63 and is uniform.  Lanes whose DPP source lies outside the row see the
+// identity INT32_MAX.
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x111, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x112, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x114, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x118, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x142, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT32_MAX, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Lexicographic (delta, index) minimum over the wave from per-lane bests
+// (lanes without a move hold index -1): the minimum delta first, then the
+// smallest index among the lanes holding it.  Returns the packed row key.
+__device__ __forceinline__ int64_t wave_argmin(int32_t bd, int32_t bc) {
+  const int32_t m = wave_min_i32(bc >= 0 ? bd : INT32_MAX);
+  const int32_t c = wave_min_i32((bc >= 0 && bd == m) ? bc : INT32_MAX);
+  return c == INT32_MAX ? MLP_KEY_NONE
+                        : (int64_t)m * (int64_t)(1 << MLP_KEY_SHIFT) + (int64_t)c;
+}
+
 __device__ __forceinline__ int64_t wave_min64(int64_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -828,7 +852,7 @@ __device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, T
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t key = wave_min64(pair_key(bd[r], bc[r]));
+    const int64_t key = wave_argmin(bd[r], bc[r]);
     if (lane == 0) L.wb[wave][0][r] = key;
   }
 }
@@ -933,7 +957,7 @@ __device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, 
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t key = wave_min64(pair_key(bd[r], bc[r]));
+    const int64_t key = wave_argmin(bd[r], bc[r]);
     if (lane == 0) L.wb[wave][K][r] = key;
   }
 }
@@ -1050,7 +1074,7 @@ __device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds
       }
     }
   }
-  const int64_t key = wave_min64(pair_key(bd, bc));
+  const int64_t key = wave_argmin(bd, bc);
   if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, key);
 }
 
