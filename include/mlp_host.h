@@ -36,6 +36,12 @@ int32_t mlp_gen_loguniform_n(uint32_t seed, int32_t index, int32_t lo, int32_t h
 int mlp_tsplib_load(const char* path, int32_t* xs, int32_t* ys, int32_t cap, int32_t* n_out);
 int mlp_shift_nonnegative(int32_t n, int32_t* xs, int32_t* ys);
 
+/* int(sqrtf(float(dx^2 + dy^2)) + dr), dr = 0.5 * round (mlproblem.cpp:159-162). */
+int mlp_host_dist(int32_t x1, int32_t y1, int32_t x2, int32_t y2, int32_t round);
+/* Latency sum_{p=1}^{N-1} (N - p) d(tour[p-1], tour[p]) as int64. */
+int64_t mlp_host_latency(const int32_t* xs, const int32_t* ys, int32_t round, const int32_t* tour,
+                         int32_t N);
+
 /* [0, shuffle(1..n-1), 0] -- N = n + 1 entries. */
 int mlp_tour_perm(uint32_t seed, int32_t n, int32_t* tour);
 /* Randomized greedy start tour (alpha RCL), the (skip+1)-th one drawn from

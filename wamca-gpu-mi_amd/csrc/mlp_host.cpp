@@ -177,6 +177,22 @@ int mlp_shift_nonnegative(int32_t n, int32_t* xs, int32_t* ys) {
   return 0;
 }
 
+int mlp_host_dist(int32_t x1, int32_t y1, int32_t x2, int32_t y2, int32_t round) {
+  return host_dist(x1, y1, x2, y2, round);
+}
+
+// Latency of a tour of N positions over client coordinates, int64
+// (mlsolution.cpp:215-228 without the uint32 wrap).
+int64_t mlp_host_latency(const int32_t* xs, const int32_t* ys, int32_t round, const int32_t* tour,
+                         int32_t N) {
+  int64_t L = 0;
+  for (int p = 1; p < N; ++p) {
+    const int a = tour[p - 1], b = tour[p];
+    L += (int64_t)(N - p) * host_dist(xs[a], ys[a], xs[b], ys[b], round);
+  }
+  return L;
+}
+
 int mlp_tour_perm(uint32_t seed, int32_t n, int32_t* tour) {
   if (n < 2 || !tour) return -1;
   MT m(seed);
