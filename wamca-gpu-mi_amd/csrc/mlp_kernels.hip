@@ -746,7 +746,6 @@ struct TileLds {
   uint16_t v[R + 4][C + 4];  // v[a - b0][c - cbase] = d(tour[a], tour[c])
   int32_t w[C + 4];          // w[c] staged at c - cbase
   int32_t t[C + 4];          // T[c] (mod 2^32) staged at c - cbase
-  int64_t wb[4][4][R];       // per-wave partial keys [wave][group][row]
 };
 
 template <int C>
@@ -794,177 +793,181 @@ __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, b
   }
 }
 
-// Swap rows of a tile: lane = column j, rows r unrolled.  Row r: i = b0+r+1,
-// j in [i+2, N-2].  delta = Cs_r + Bs(j) + (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j]
-// + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
-template <int R, int C>
+// Swap rows of a tile: each wave owns RW consecutive rows (r0 = wave * RW),
+// lanes stride over the tile's columns j and evaluate all RW rows for their
+// column.  Row r: i = b0+r+1, j in [i+2, N-2].  delta = Cs_r + Bs(j) +
+// (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j] + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
+template <int R, int C, int RW>
 __device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                          int wave, int lane) {
+                          int parity, int wave, int lane) {
   const int N = s.N;
-  const int rv = min(R, N - 4 - b0);  // rows with i <= N-4
-  int32_t bd[R], bc[R];
+  const int r0 = wave * RW;
+  const int rv = min(RW, N - 4 - b0 - r0);  // rows with i <= N-4
+  if (rv <= 0) return;
+  int32_t bd[RW], bc[RW];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RW; ++r) {
     bd[r] = INT32_MAX;
     bc[r] = -1;
   }
-  if (rv > 0) {
-    uint32_t myCs = 0u;
-    if (lane < rv) {
-      const int i = b0 + lane + 1;
-      myCs = 0u - umul(N - i, (uint32_t)ldg(s.w + i)) - umul(N - i - 1, (uint32_t)ldg(s.w + i + 1));
-    }
-    uint32_t Cs[R];
+  uint32_t myCs = 0u;
+  if (lane < rv) {
+    const int i = b0 + r0 + lane + 1;
+    myCs = 0u - umul(N - i, (uint32_t)ldg(s.w + i)) - umul(N - i - 1, (uint32_t)ldg(s.w + i + 1));
+  }
+  uint32_t Cs[RW];
 #pragma unroll
-    for (int r = 0; r < R; ++r) Cs[r] = lane_bcast(myCs, r);
-    const int jhi = min(q1, N - 1);
-    const int jlo = max(q0, b0 + 3);
-    for (int jb = q0 + wave * kWave; jb < jhi; jb += kBlock) {
-      if (jb + kWave <= jlo) continue;
-      const int j = jb + lane;
-      const int jj = min(j, jhi - 1);
-      const int x = jj - cbase;
-      const uint32_t nj = (uint32_t)(N - jj);
-      const uint32_t cs = 0u - umul(nj, (uint32_t)L.w[x]) - umul(nj - 1u, (uint32_t)L.w[x + 1]);
-      uint32_t c[R + 2], lf[R], rt[R];
+  for (int r = 0; r < RW; ++r) Cs[r] = lane_bcast(myCs, r);
+  const uint16_t(*V)[C + 4] = L.v + r0;
+  const int jhi = min(q1, N - 1);
+  const int jlo = max(q0, b0 + r0 + 3);
+  for (int jb = jlo - ((jlo - q0) & (kWave - 1)); jb < jhi; jb += kWave) {
+    const int j = jb + lane;
+    const int jj = min(max(j, jlo), jhi - 1);
+    const int x = jj - cbase;
+    const uint32_t nj = (uint32_t)(N - jj);
+    const uint32_t cs = 0u - umul(nj, (uint32_t)L.w[x]) - umul(nj - 1u, (uint32_t)L.w[x + 1]);
+    uint32_t c[RW + 2];
 #pragma unroll
-      for (int a = 0; a < R + 2; ++a) c[a] = L.v[a][x];
+    for (int a = 0; a < RW + 2; ++a) c[a] = V[a][x];
+    const bool full = jb >= b0 + r0 + RW + 2 && jb + kWave <= jhi;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        lf[r] = L.v[r + 1][x - 1];
-        rt[r] = L.v[r + 1][x + 1];
-      }
-      const bool full = jb >= b0 + R + 2 && jb + kWave <= jhi;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (r < rv) {
-          const uint32_t a1 = (uint32_t)(N - b0 - r - 1);
-          const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) + umul(nj, lf[r]) +
-                             umul(nj - 1u, rt[r]);
-          const bool ok = full || (j < jhi && j >= b0 + r + 3);
-          if (ok && (int32_t)d < bd[r]) {
-            bd[r] = (int32_t)d;
-            bc[r] = j;
-          }
+    for (int r = 0; r < RW; ++r) {
+      if (r < rv) {
+        const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
+        const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) +
+                           umul(nj, V[r + 1][x - 1]) + umul(nj - 1u, V[r + 1][x + 1]);
+        const bool ok = full || (j < jhi && j >= b0 + r0 + r + 3);
+        if (ok && (int32_t)d < bd[r]) {
+          bd[r] = (int32_t)d;
+          bc[r] = j;
         }
       }
     }
   }
+  int64_t* out = row_buf(s, 0, parity & 1) + b0 + r0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RW; ++r) {
     const int64_t key = wave_argmin(bd[r], bc[r]);
-    if (lane == 0) L.wb[wave][0][r] = key;
+    if (lane == 0 && r < rv) fold_key(out + r, key);
   }
 }
 
-// Or-opt-K rows of a tile.  Row r (by = b0+r): forward i = by+1 for j >= by+2,
-// backward i = by+2 for j <= by+1, j in [1, N-K-1]; row-local index j - 1.
-template <int R, int C, int K>
+// Or-opt-K rows of a tile, each wave owning RW rows.  Row r (by = b0+r):
+// forward i = by+1 for j >= by+2, backward i = by+2 for j <= by+1, j in
+// [1, N-K-1]; row-local index j - 1.
+template <int R, int C, int RW, int K>
 __device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                           int wave, int lane) {
+                           int parity, int wave, int lane) {
   const int N = s.N;
   const float dr = round_dr(s.round);
-  const int rv = min(R, N - K - 2 - b0);  // rows with by <= N-K-3
-  int32_t bd[R], bc[R];
+  const int r0 = wave * RW;
+  const int rv = min(RW, N - K - 2 - b0 - r0);  // rows with by <= N-K-3
+  if (rv <= 0) return;
+  const int bb = b0 + r0;  // first row of this wave
+  int32_t bd[RW], bc[RW];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RW; ++r) {
     bd[r] = INT32_MAX;
     bc[r] = -1;
   }
-  if (rv > 0) {
-    uint32_t mRF = 0u, mRB = 0u;
-    int mSF = 0, mSB = 0;
-    if (lane < rv) {
-      const int iF = b0 + lane + 1, iB = b0 + lane + 2;
-      const uint32_t SF = (uint32_t)ldg(s.t32 + iF + K - 1) - (uint32_t)ldg(s.t32 + iF);
-      const uint32_t SB = (uint32_t)ldg(s.t32 + iB + K - 1) - (uint32_t)ldg(s.t32 + iB);
-      const uint32_t dF = (uint32_t)dist_xy(ldg(s.xy + iF - 1), ldg(s.xy + iF + K), dr);
-      const uint32_t dB = (uint32_t)dist_xy(ldg(s.xy + iB - 1), ldg(s.xy + iB + K), dr);
-      mRF = (uint32_t)(N - iF) * (dF - (uint32_t)ldg(s.w + iF)) -
-            (uint32_t)(N - iF - K) * (uint32_t)ldg(s.w + iF + K) + (uint32_t)iF * SF -
-            (uint32_t)K * (uint32_t)ldg(s.t32 + iF + K);
-      mRB = (uint32_t)(N - iB - K) * (dB - (uint32_t)ldg(s.w + iB + K)) -
-            (uint32_t)(N - iB) * (uint32_t)ldg(s.w + iB) + (uint32_t)iB * SB -
-            (uint32_t)K * (uint32_t)ldg(s.t32 + iB - 1);
-      mSF = -(int)SF;
-      mSB = -(int)SB;
-    }
-    uint32_t RF[R], RB[R];
-    int nSF[R], nSB[R];
+  uint32_t mRF = 0u, mRB = 0u;
+  int mSF = 0, mSB = 0;
+  if (lane < rv) {
+    const int iF = bb + lane + 1, iB = bb + lane + 2;
+    const uint32_t SF = (uint32_t)ldg(s.t32 + iF + K - 1) - (uint32_t)ldg(s.t32 + iF);
+    const uint32_t SB = (uint32_t)ldg(s.t32 + iB + K - 1) - (uint32_t)ldg(s.t32 + iB);
+    const uint32_t dF = (uint32_t)dist_xy(ldg(s.xy + iF - 1), ldg(s.xy + iF + K), dr);
+    const uint32_t dB = (uint32_t)dist_xy(ldg(s.xy + iB - 1), ldg(s.xy + iB + K), dr);
+    mRF = (uint32_t)(N - iF) * (dF - (uint32_t)ldg(s.w + iF)) -
+          (uint32_t)(N - iF - K) * (uint32_t)ldg(s.w + iF + K) + (uint32_t)iF * SF -
+          (uint32_t)K * (uint32_t)ldg(s.t32 + iF + K);
+    mRB = (uint32_t)(N - iB - K) * (dB - (uint32_t)ldg(s.w + iB + K)) -
+          (uint32_t)(N - iB) * (uint32_t)ldg(s.w + iB) + (uint32_t)iB * SB -
+          (uint32_t)K * (uint32_t)ldg(s.t32 + iB - 1);
+    mSF = -(int)SF;
+    mSB = -(int)SB;
+  }
+  uint32_t RF[RW], RB[RW];
+  int nSF[RW], nSB[RW];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      RF[r] = lane_bcast(mRF, r);
-      RB[r] = lane_bcast(mRB, r);
-      nSF[r] = (int)lane_bcast((uint32_t)mSF, r);
-      nSB[r] = (int)lane_bcast((uint32_t)mSB, r);
-    }
-    const int jhi = min(q1, N - K);
-    for (int jb = q0 + wave * kWave; jb < jhi; jb += kBlock) {
-      const int j = jb + lane;
-      const int jj = min(j, jhi - 1);
-      const int x = jj - cbase;
-      const uint32_t nj = (uint32_t)(N - jj);
-      const bool tail = jb + kWave > jhi;
-      const bool allF = !tail && jb >= b0 + R + 1;
-      const bool allB = !tail && jb + kWave - 1 <= b0 + 1;
-      if (allF) {
-        const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
+  for (int r = 0; r < RW; ++r) {
+    RF[r] = lane_bcast(mRF, r);
+    RB[r] = lane_bcast(mRB, r);
+    nSF[r] = (int)lane_bcast((uint32_t)mSF, r);
+    nSB[r] = (int)lane_bcast((uint32_t)mSB, r);
+  }
+  const uint16_t(*V)[C + 4] = L.v + r0;
+  const int jhi = min(q1, N - K);
+  for (int jb = q0; jb < jhi; jb += kWave) {
+    const int j = jb + lane;
+    const int jj = min(j, jhi - 1);
+    const int x = jj - cbase;
+    const uint32_t nj = (uint32_t)(N - jj);
+    const bool tail = jb + kWave > jhi;
+    const bool allF = !tail && jb >= bb + RW + 1;
+    const bool allB = !tail && jb + kWave - 1 <= bb + 1;
+    if (allF) {
+      const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (r < rv) {
-            uint32_t d = RF[r] + CF + umul(nj - K, L.v[r + K][x + K]) + umul(nj, L.v[r + 1][x + K - 1]);
+      for (int r = 0; r < RW; ++r) {
+        if (r < rv) {
+          uint32_t d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
+          if (K > 1) d += smul(jj, nSF[r]);
+          if ((int32_t)d < bd[r]) {
+            bd[r] = (int32_t)d;
+            bc[r] = j - 1;
+          }
+        }
+      }
+    } else if (allB) {
+      const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (r < rv) {
+          uint32_t d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
+          if (K > 1) d += smul(jj, nSB[r]);
+          if ((int32_t)d < bd[r]) {
+            bd[r] = (int32_t)d;
+            bc[r] = j - 1;
+          }
+        }
+      }
+    } else {
+      const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
+      const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (r < rv) {
+          const bool fwd = jj >= bb + r + 2;
+          uint32_t d;
+          if (fwd) {
+            d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
             if (K > 1) d += smul(jj, nSF[r]);
-            if ((int32_t)d < bd[r]) {
-              bd[r] = (int32_t)d;
-              bc[r] = j - 1;
-            }
-          }
-        }
-      } else if (allB) {
-        const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (r < rv) {
-            uint32_t d = RB[r] + CB + umul(nj, L.v[r + 2][x - 1]) + umul(nj - K, L.v[r + K + 1][x]);
+          } else {
+            d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
             if (K > 1) d += smul(jj, nSB[r]);
-            if ((int32_t)d < bd[r]) {
-              bd[r] = (int32_t)d;
-              bc[r] = j - 1;
-            }
           }
-        }
-      } else {
-        const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
-        const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (r < rv) {
-            uint32_t dF = RF[r] + CF + umul(nj - K, L.v[r + K][x + K]) + umul(nj, L.v[r + 1][x + K - 1]);
-            uint32_t dB = RB[r] + CB + umul(nj, L.v[r + 2][x - 1]) + umul(nj - K, L.v[r + K + 1][x]);
-            if (K > 1) {
-              dF += smul(jj, nSF[r]);
-              dB += smul(jj, nSB[r]);
-            }
-            const uint32_t d = jj >= b0 + r + 2 ? dF : dB;
-            if (j < jhi && (int32_t)d < bd[r]) {
-              bd[r] = (int32_t)d;
-              bc[r] = j - 1;
-            }
+          if (j < jhi && (int32_t)d < bd[r]) {
+            bd[r] = (int32_t)d;
+            bc[r] = j - 1;
           }
         }
       }
     }
   }
+  int64_t* out = row_buf(s, K + 1, (parity >> (K + 1)) & 1) + bb;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RW; ++r) {
     const int64_t key = wave_argmin(bd[r], bc[r]);
-    if (lane == 0) L.wb[wave][K][r] = key;
+    if (lane == 0 && r < rv) fold_key(out + r, key);
   }
 }
 
 template <int R, int C>
 __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
                             TileLds<R, C>& L) {
+  constexpr int RW = R / (kBlock / kWave);
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int b0 = rb * R;
@@ -989,20 +992,10 @@ __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int pari
     L.t[c - cbase] = ldg(s.t32 + c);
   }
   __syncthreads();
-  if (mask & 1) tile_swap<R, C>(s, b0, q0, q1, cbase, L, wave, lane);
-  if (mask & 4) tile_oropt<R, C, 1>(s, b0, q0, q1, cbase, L, wave, lane);
-  if (mask & 8) tile_oropt<R, C, 2>(s, b0, q0, q1, cbase, L, wave, lane);
-  if (mask & 16) tile_oropt<R, C, 3>(s, b0, q0, q1, cbase, L, wave, lane);
-  __syncthreads();
-  for (int q = tid; q < 4 * R; q += kBlock) {
-    const int g = q / R, r = q % R, by = b0 + r;
-    const int nb = g == 0 ? 0 : g + 1;
-    if (!((mask >> nb) & 1) || by >= rows_of(nb, 0, N)) continue;
-    int64_t key = L.wb[0][g][r];
-#pragma unroll
-    for (int w = 1; w < kBlock / kWave; ++w) key = min(key, L.wb[w][g][r]);
-    fold_key(row_buf(s, nb, (parity >> nb) & 1) + by, key);
-  }
+  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane);
 }
 
 // int64 fallback of a fixed-i tile (column tile 0 only): one wave per
@@ -1149,10 +1142,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ ta
   }
 }
 
-template __global__ void k_sweep<4, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+template __global__ void k_sweep<8, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
                                          int, int*);
-template __global__ void k_sweep<8, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                          int, int*);
+template __global__ void k_sweep<16, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                           int, int*);
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -1184,17 +1177,17 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid) {
   if (tile == MLP_TILE_SMALL)
-    hipLaunchKernelGGL((k_sweep<4, 256>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
+    hipLaunchKernelGGL((k_sweep<8, 512>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
                        nsegs, total, counter);
   else
-    hipLaunchKernelGGL((k_sweep<8, 1024>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
+    hipLaunchKernelGGL((k_sweep<16, 1024>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
                        nsegs, total, counter);
   return hipGetLastError();
 }
 
 void mlp_tile_shape(int tile, int* rows, int* cols) {
-  *rows = tile == MLP_TILE_SMALL ? 4 : 8;
-  *cols = tile == MLP_TILE_SMALL ? 256 : 1024;
+  *rows = tile == MLP_TILE_SMALL ? 8 : 16;
+  *cols = tile == MLP_TILE_SMALL ? 512 : 1024;
 }
 
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {
