@@ -1,0 +1,242 @@
+// mlkernel.hpp -- the reference's C++ surface for the MLP local search
+// (MLProblem / MLSolution / MTRandom / MLKernel{Swap,2Opt,OrOpt} /
+// WAMCAExperiment), implemented over the MI355X C ABI (mlp_gpu.h) in
+// libwamca_mi.so.  Plain C++17, no HIP headers: link with -lwamca_mi.
+//
+// Reference interfaces mirrored (LabIC-UFF/wamca-gpu-mi):
+//   MTRandom                     mtrand.h:168-312, mtrand.cpp:51-121
+//   MLProblem(costTour, distRound, coordShift), load()   mlproblem.h:61-151, mlproblem.cpp:58-207
+//   MLSolution: clients/weights/cost, update, costCalc, random, sequence, save/load
+//                                mlsolution.h, mlsolution.cpp:215-302, 684-906
+//   MLMove64 / MLMovePack / MLMove / MLMoveId            mlads.h:144-216
+//   MLKernel: init, term, setSolution, getSolution, sendSolution, defineKernelGrid,
+//             launchKernel, sync, recvResult, mergeGPU, bestMove, mergeGreedy, applyMove
+//                                mlkernel.h:104-318, mlkernel.cu:254-864
+//   MLKernelSwap / MLKernel2Opt / MLKernelOrOpt           mlkswap.h:29-48, mlk2opt.h:29-48, mlkoropt.h:29-48
+//   WAMCAExperiment::runKernel / runWAMCA2016             WamcaExperiment.hpp:292-581
+//
+// Differences: device work goes through the C ABI (the ADS is built on the
+// device from the tour; sendSolution uploads the tour only); latencies are
+// kept as uint32 like the reference (cost64 holds the exact value); errors
+// throw wamca::Error instead of exiting; isTotal kernels are not provided
+// (constructing one throws).
+#ifndef WAMCA_MLKERNEL_HPP_
+#define WAMCA_MLKERNEL_HPP_
+
+#include <cstdint>
+#include <iosfwd>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mlp_gpu.h"
+
+namespace wamca {
+
+using uint = unsigned int;
+using ushort = unsigned short;
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+// mlads.h:144-176
+enum MLMoveId {
+  MLMI_SWAP,
+  MLMI_2OPT,
+  MLMI_OROPT1,
+  MLMI_OROPT2,
+  MLMI_OROPT3,
+};
+
+// mlads.h:190-216: 64-bit packed move (id:4 | i:14 | j:14, int32 cost)
+struct MLMove64 {
+  unsigned int id : 4;
+  unsigned int i : 14;
+  unsigned int j : 14;
+  int cost;
+};
+union MLMovePack {
+  MLMove64 s;
+  uint64_t w;
+};
+struct MLMove {
+  MLMoveId id;
+  int i;
+  int j;
+  int cost;
+};
+static_assert(sizeof(MLMove64) == 8, "MLMove64 is 8 bytes");
+
+extern const char* nameMove[5];
+
+class MTRandom {
+ public:
+  MTRandom() { seed(5489U); }
+  explicit MTRandom(uint s) { seed(s); }
+  void seed(uint s);
+  uint getSeed() const { return rseed_; }
+  uint rand();
+  uint rand(uint max) { return (uint)(((max + 1.0) * rand()) / 4294967296.0); }
+  uint rand(uint min, uint max) { return min + rand(max - min); }
+  double realCO() { return ((double)rand()) * (1.0 / 4294967296.0); }
+  double realOO() { return (((double)rand()) + 0.5) * (1.0 / 4294967296.0); }
+  template <typename T>
+  void shuffle(T* data, uint n) {
+    for (uint i = 0; i + 1 < n; ++i) {
+      const uint j = rand(n - i - 1);
+      T t = data[i];
+      data[i] = data[j];
+      data[j] = t;
+    }
+  }
+
+ private:
+  void regen();
+  uint state_[624];
+  int p_ = 624;
+  uint rseed_ = 0;
+};
+
+struct MLClientData {
+  ushort id;
+  int x, y;
+};
+
+class MLProblem {
+ public:
+  MLProblem(bool costTour = true, bool distRound = false, bool coordShift = false);
+  ~MLProblem();
+  MLProblem(const MLProblem&) = delete;
+  MLProblem& operator=(const MLProblem&) = delete;
+  void load(const char* fname);
+  void create(const int32_t* xs, const int32_t* ys, int n, const char* name = "synthetic");
+  int weight(int a, int b) const { return w_[(size_t)a * n_ + (size_t)b]; }
+
+  bool costTour, distRound, coordShift;
+  uint size = 0;  // positions of a solution (clients + costTour)
+  char name[64] = {0};
+  char filename[256] = {0};
+  std::vector<MLClientData> clients;  // size entries; the depot copy repeats client 0
+  int maxWeight = 0;
+
+  // device side, created lazily by the first kernel init on a context
+  mlp_ctx* ctx();
+  int32_t instance();
+  int clientCount() const { return n_; }
+  const int32_t* xs() const { return xs_.data(); }
+  const int32_t* ys() const { return ys_.data(); }
+
+ private:
+  void build_weights();
+  int n_ = 0;
+  std::vector<int32_t> xs_, ys_, w_;
+  mlp_ctx* ctx_ = nullptr;
+  int32_t inst_ = -1;
+};
+
+class MLSolution {
+ public:
+  explicit MLSolution(MLProblem& prob, int flags = 0);
+  void clear() {
+    clientCount = 0;
+    cost = 0x7fffffff;
+  }
+  void add(ushort c) { clients[clientCount++] = c; }
+  void assign(const MLSolution* sol, bool ads = true);
+  void update();
+  uint costCalc() const;
+  void random(MTRandom& rng, float alpha);
+  void sequence();
+  int dist(uint i, uint j) const { return problem.weight(clients[i], clients[j]); }
+  void save(std::ostream& os) const;
+  void load(std::istream& is);
+  std::string str(bool showCost = false) const;
+
+  MLProblem& problem;
+  std::vector<ushort> clients;
+  std::vector<ushort> weights;
+  uint clientCount = 0;
+  uint cost = 0x7fffffff;
+  int64_t cost64 = 0;  // exact latency from the last update()
+  MLMove move{MLMI_SWAP, 0, 0, 0};
+};
+
+class MLKernel {
+ public:
+  MLKernel(MLProblem& problem, bool isTotal, int kid, uint ktag = 0);
+  virtual ~MLKernel();
+  MLKernel(const MLKernel&) = delete;
+  MLKernel& operator=(const MLKernel&) = delete;
+
+  void init(bool solCreate);
+  void term();
+  void resetStats() { callCount = mergeCount = imprvCount = 0; }
+  void setSolution(MLSolution* sol, bool ads = true) { solution->assign(sol, ads); }
+  void getSolution(MLSolution* sol, bool ads = true) { sol->assign(solution, ads); }
+  void sendSolution();
+  void defineKernelGrid();
+  void launchKernel();
+  void sync();
+  void recvResult();
+  void mergeGPU();
+  int bestMove(MLMove& move);
+  int mergeGreedy(MLMove64* merge, int& count);
+  virtual void applyMove(MLMove& move) = 0;
+
+  MLProblem& problem;
+  int id;
+  uint tag;
+  const char* name;
+  bool isTotal;
+  int rowmap = MLP_ROWMAP_ODC;  // the shipped build's kernels; MLP_ROWMAP_ADS for the logged ones
+  uint moveElems = 0;
+  std::vector<MLMove64> moves;      // per-row records (recvResult) or merged list (mergeGPU)
+  MLMove64* transBuffer = nullptr;  // = moves.data()
+  MLSolution* solution = nullptr;
+  bool solDestroy = false;
+  uint callCount = 0, mergeCount = 0, imprvCount = 0;
+
+ protected:
+  int32_t slot_ = -1;
+  bool merged_ = false;
+};
+
+class MLKernelSwap : public MLKernel {
+ public:
+  explicit MLKernelSwap(MLProblem& p, bool isTotal = false) : MLKernel(p, isTotal, MLMI_SWAP) {}
+  void applyMove(MLMove& move) override;
+};
+
+class MLKernel2Opt : public MLKernel {
+ public:
+  explicit MLKernel2Opt(MLProblem& p, bool isTotal = false) : MLKernel(p, isTotal, MLMI_2OPT) {}
+  void applyMove(MLMove& move) override;
+};
+
+class MLKernelOrOpt : public MLKernel {
+ public:
+  MLKernelOrOpt(MLProblem& p, uint tag, bool isTotal = false)
+      : MLKernel(p, isTotal, (int)MLMI_OROPT1 + (int)tag - 1, tag) {}
+  void applyMove(MLMove& move) override;
+};
+
+// In-repo driver: one pass over neighborhoods [kMin, kMax) chained on the
+// solution (WamcaExperiment.hpp:292-581).
+class WAMCAExperiment {
+ public:
+  WAMCAExperiment(MLProblem& problem, int seed);
+  ~WAMCAExperiment();
+  void runKernel(int k, std::vector<MLMove>* moves = nullptr);
+  unsigned int runWAMCA2016(int mMax = 3, int kMin = 0, int kMax = 3, int* solution = nullptr,
+                            unsigned int solutionSize = 0, std::vector<MLMove>* moves = nullptr,
+                            int* solutionResp = nullptr);
+  MLProblem& problem;
+  MTRandom rng;
+  MLKernel* kernels[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  MLSolution* solDevice;
+};
+
+}  // namespace wamca
+
+#endif  // WAMCA_MLKERNEL_HPP_
