@@ -145,9 +145,10 @@ int mlp_sqrt_selfcheck(mlp_ctx* ctx, int64_t* mismatches);
 /* Same for an explicit list of squared lengths. */
 int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out);
 
-/* Sweep tile shape for mlp_eval with the ADS row map: -1 automatic (small
- * tiles for launches of at most 8 tours, large tiles otherwise), 0 small
- * (4 rows x 256 columns), 1 large (8 rows x 1024 columns). */
+/* Sweep tile shape for mlp_eval with the ADS row map: -1 automatic (chosen
+ * from the launch's total move count, or by the table mlp_autotune fills),
+ * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024.  Results do not
+ * depend on the shape. */
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
 
 /* Timing of the last eval launch on the context stream (ms, HIP events). */

@@ -298,13 +298,15 @@ def test_rows_int64_path(eng, n, span):
             _check_rows(eng, s, W, t, nb, rowmap)
 
 
-@pytest.mark.parametrize("tile", [0, 1])
-@pytest.mark.parametrize("n", [9, 300, 1000, 2000])
-def test_tile_shapes_and_row_buffers(eng, tile, n):
-    """Both sweep tile shapes (column-split rows folded with atomicMin into
-    double-buffered row arrays) give the oracle's rows, across repeated
-    evaluations and alternating row maps."""
-    xs, ys = O.instance_clustered(n, 300 + n)
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("n,span", [(5, 2000), (9, 2000), (300, 2000), (1000, 2000),
+                                    (2000, 2000), (700, 30000)])
+def test_tile_shapes_and_row_buffers(eng, tile, n, span):
+    """Every sweep tile shape (column-split rows folded with atomicMin into
+    double-buffered row arrays) gives the oracle's rows, across repeated
+    evaluations and alternating row maps; span 30000 takes the int64 path."""
+    xs, ys = O.instance_clustered(n, 300 + n) if span == 2000 else \
+        O.instance_uniform(n, 300 + n, span)
     W = O.weights(xs, ys, 1)
     inst = eng.instance(xs, ys, 1)
     s = eng.slot(inst)

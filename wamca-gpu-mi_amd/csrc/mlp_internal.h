@@ -75,7 +75,6 @@ struct EvalSeg {
 
 // Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
 // the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
-#define MLP_TILE_ROWS 8
 #define MLP_2OPT_ROWS 4
 struct SweepSeg {
   int32_t slot;
@@ -86,9 +85,13 @@ struct SweepSeg {
   int32_t parity;  // bit nb: row buffer written by this eval
 };
 #define MLP_SWEEPSEG_INTS 6
-// Tile shapes (rows x columns) of the sweep; 0 = small (single tours), 1 = large (batches).
-#define MLP_TILE_SMALL 0
-#define MLP_TILE_LARGE 1
+// Tile shapes (rows x columns) of the sweep: 4 x 256 (most workgroups, for
+// launches of one or a few tours), 8 x 512, 16 x 1024 (fewest distance
+// evaluations per move, for large batches).
+#define MLP_TILE_4x256 0
+#define MLP_TILE_8x512 1
+#define MLP_TILE_16x1024 2
+#define MLP_TILE_COUNT 3
 
 #define MLP_SEGS_INLINE 32
 struct EvalSegsInline {

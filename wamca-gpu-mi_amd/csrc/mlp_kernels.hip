@@ -746,6 +746,9 @@ struct TileLds {
   uint16_t v[R + 4][C + 4];  // v[a - b0][c - cbase] = d(tour[a], tour[c])
   int32_t w[C + 4];          // w[c] staged at c - cbase
   int32_t t[C + 4];          // T[c] (mod 2^32) staged at c - cbase
+  int32_t rw[R + 8];         // row side: w, T and xy at positions b0 .. b0 + R + 7
+  int32_t rt[R + 8];
+  int32_t rxy[R + 8];
 };
 
 template <int C>
@@ -812,8 +815,9 @@ __device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, T
   }
   uint32_t myCs = 0u;
   if (lane < rv) {
-    const int i = b0 + r0 + lane + 1;
-    myCs = 0u - umul(N - i, (uint32_t)ldg(s.w + i)) - umul(N - i - 1, (uint32_t)ldg(s.w + i + 1));
+    const int o = r0 + lane + 1;  // i - b0
+    const int i = b0 + o;
+    myCs = 0u - umul(N - i, (uint32_t)L.rw[o]) - umul(N - i - 1, (uint32_t)L.rw[o + 1]);
   }
   uint32_t Cs[RW];
 #pragma unroll
@@ -874,17 +878,18 @@ __device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, 
   uint32_t mRF = 0u, mRB = 0u;
   int mSF = 0, mSB = 0;
   if (lane < rv) {
-    const int iF = bb + lane + 1, iB = bb + lane + 2;
-    const uint32_t SF = (uint32_t)ldg(s.t32 + iF + K - 1) - (uint32_t)ldg(s.t32 + iF);
-    const uint32_t SB = (uint32_t)ldg(s.t32 + iB + K - 1) - (uint32_t)ldg(s.t32 + iB);
-    const uint32_t dF = (uint32_t)dist_xy(ldg(s.xy + iF - 1), ldg(s.xy + iF + K), dr);
-    const uint32_t dB = (uint32_t)dist_xy(ldg(s.xy + iB - 1), ldg(s.xy + iB + K), dr);
-    mRF = (uint32_t)(N - iF) * (dF - (uint32_t)ldg(s.w + iF)) -
-          (uint32_t)(N - iF - K) * (uint32_t)ldg(s.w + iF + K) + (uint32_t)iF * SF -
-          (uint32_t)K * (uint32_t)ldg(s.t32 + iF + K);
-    mRB = (uint32_t)(N - iB - K) * (dB - (uint32_t)ldg(s.w + iB + K)) -
-          (uint32_t)(N - iB) * (uint32_t)ldg(s.w + iB) + (uint32_t)iB * SB -
-          (uint32_t)K * (uint32_t)ldg(s.t32 + iB - 1);
+    const int o = r0 + lane + 1;  // iF - b0; iB = iF + 1
+    const int iF = b0 + o, iB = iF + 1;
+    const uint32_t SF = (uint32_t)L.rt[o + K - 1] - (uint32_t)L.rt[o];
+    const uint32_t SB = (uint32_t)L.rt[o + K] - (uint32_t)L.rt[o + 1];
+    const uint32_t dF = (uint32_t)dist_xy(L.rxy[o - 1], L.rxy[o + K], dr);
+    const uint32_t dB = (uint32_t)dist_xy(L.rxy[o], L.rxy[o + 1 + K], dr);
+    mRF = (uint32_t)(N - iF) * (dF - (uint32_t)L.rw[o]) -
+          (uint32_t)(N - iF - K) * (uint32_t)L.rw[o + K] + (uint32_t)iF * SF -
+          (uint32_t)K * (uint32_t)L.rt[o + K];
+    mRB = (uint32_t)(N - iB - K) * (dB - (uint32_t)L.rw[o + 1 + K]) -
+          (uint32_t)(N - iB) * (uint32_t)L.rw[o + 1] + (uint32_t)iB * SB -
+          (uint32_t)K * (uint32_t)L.rt[o];
     mSF = -(int)SF;
     mSB = -(int)SB;
   }
@@ -964,40 +969,6 @@ __device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, 
   }
 }
 
-template <int R, int C>
-__device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
-                            TileLds<R, C>& L) {
-  constexpr int RW = R / (kBlock / kWave);
-  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-  const float dr = round_dr(s.round);
-  const int b0 = rb * R;
-  const int q0 = 1 + ct * C;                 // move j in [q0, q1)
-  const int q1 = min(q0 + C, N - 1);
-  const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
-  const int cend = min(q1 + 3, N);
-  if (ct == 0) {
-#pragma unroll
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
-      if (nb != 1 && ((mask >> nb) & 1))
-        reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
-  }
-  int xa[R + 4];
-#pragma unroll
-  for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
-  for (int c = cbase + tid; c < cend; c += kBlock) {
-    const int xc = ldg(s.xy + c);
-#pragma unroll
-    for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc, dr);
-    L.w[c - cbase] = ldg(s.w + c);
-    L.t[c - cbase] = ldg(s.t32 + c);
-  }
-  __syncthreads();
-  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-}
-
 // int64 fallback of a fixed-i tile (column tile 0 only): one wave per
 // (neighborhood, row) over the whole row.
 template <int R>
@@ -1019,56 +990,68 @@ __device__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int
   }
 }
 
-template <int C>
-__device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
+template <int R, int C>
+__device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
+                            TileLds<R, C>& L) {
+  constexpr int RW = R / (kBlock / kWave);
+  constexpr int FI = (C + 4 + kBlock - 1) / kBlock;  // fill iterations per thread
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
-  const int rows = (N + 1) >> 1;
-  const int par = (parity >> 1) & 1;
-  if (ct == 0)
-    reset_other(s, 1, par, rb * MLP_2OPT_ROWS, min(rb * MLP_2OPT_ROWS + MLP_2OPT_ROWS, rows), rb == 0);
-  const int by = rb * MLP_2OPT_ROWS + wave;
-  const bool live = by < rows;
-  const int c0 = ct * C, c1 = min(c0 + C, N - 3);  // row-local indices [c0, c1)
-  // part 0: ctx < by  -> m = N-2-by, i = by - ctx  in [by - cb1 + 1, by - c0]
-  // part 1: ctx >= by -> m = by+1,   i = ctx-by+1  in [ca0 - by + 1, c1 - by]
-  int ilo[2], ihi[2], mm[2];
-  const int cb1 = min(c1, by), ca0 = max(c0, by);
-  mm[0] = N - 2 - by;
-  ilo[0] = by - cb1 + 1;
-  ihi[0] = (live && c0 < cb1) ? by - c0 : ilo[0] - 1;
-  mm[1] = by + 1;
-  ilo[1] = ca0 - by + 1;
-  ihi[1] = (live && ca0 < c1) ? c1 - by : ilo[1] - 1;
+  const int b0 = rb * R;
+  const int q0 = 1 + ct * C;                 // move j in [q0, q1)
+  const int q1 = min(q0 + C, N - 1);
+  const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
+  const int cend = min(q1 + 3, N);
+  // every global load of the tile is issued before the first wait
+  const bool p32 = ldg(s.info + MLP_INFO_PATH32) != 0;
+  int xa[R + 4];
 #pragma unroll
-  for (int part = 0; part < 2; ++part) {
-    uint16_t* e = L.e[wave][part];
-    for (int x = lane; x <= ihi[part] - ilo[part] + 1; x += kWave) {
-      const int p = ilo[part] - 1 + x;
-      e[x] = (uint16_t)dist_xy(ldg(s.xy + p), ldg(s.xy + p + mm[part] + 1), dr);
+  for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
+  int xc[FI], wv[FI], tv[FI];
+#pragma unroll
+  for (int it = 0; it < FI; ++it) {
+    const int c = min(cbase + tid + it * kBlock, cend - 1);
+    xc[it] = ldg(s.xy + c);
+    wv[it] = ldg(s.w + c);
+    tv[it] = ldg(s.t32 + c);
+  }
+  int rwv = 0, rtv = 0, rxv = 0;
+  if (tid < R + 8) {
+    const int p = min(b0 + tid, N - 1);
+    rwv = ldg(s.w + p);
+    rtv = ldg(s.t32 + p);
+    rxv = ldg(s.xy + p);
+  }
+  if (!p32) {
+    sweep_fixed_wide<R>(s, rb, ct, mask, parity);
+    return;
+  }
+  if (ct == 0) {
+#pragma unroll
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      if (nb != 1 && ((mask >> nb) & 1))
+        reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+  }
+#pragma unroll
+  for (int it = 0; it < FI; ++it) {
+    const int c = cbase + tid + it * kBlock;
+    if (c < cend) {
+#pragma unroll
+      for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc[it], dr);
+      L.w[c - cbase] = wv[it];
+      L.t[c - cbase] = tv[it];
     }
+  }
+  if (tid < R + 8) {
+    L.rw[tid] = rwv;
+    L.rt[tid] = rtv;
+    L.rxy[tid] = rxv;
   }
   __syncthreads();
-  int32_t bd = INT32_MAX, bc = -1;
-#pragma unroll
-  for (int part = 0; part < 2; ++part) {
-    const uint16_t* e = L.e[wave][part];
-    const int m = mm[part];
-    for (int i = ilo[part] + lane; i <= ihi[part]; i += kWave) {
-      const int j = i + m;
-      const int x = i - ilo[part];
-      uint32_t d = (uint32_t)ldg(s.g32 + i) + (uint32_t)ldg(s.h32 + j);
-      d += umul((uint32_t)(N - i), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
-      d += (uint32_t)m * ((uint32_t)ldg(s.t32 + i) + (uint32_t)ldg(s.t32 + j));
-      const int ctx = part == 0 ? by - i : i + by - 1;
-      if ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc)) {
-        bd = (int32_t)d;
-        bc = ctx;
-      }
-    }
-  }
-  const int64_t key = wave_argmin(bd, bc);
-  if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, key);
+  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane);
 }
 
 __device__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
@@ -1079,6 +1062,98 @@ __device__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
   reset_other(s, 1, par, rb * MLP_2OPT_ROWS, min(rb * MLP_2OPT_ROWS + MLP_2OPT_ROWS, rows), rb == 0);
   const int by = rb * MLP_2OPT_ROWS + wave;
   if (by < rows) eval_row<1, false>(s, 1, 0, by, lane, row_buf(s, 1, par) + by);
+}
+
+template <int C>
+__device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
+  constexpr int FE = (C + 1 + kWave - 1) / kWave;  // e entries per lane and part
+  constexpr int CU = 4;                            // compute iterations per load batch
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const float dr = round_dr(s.round);
+  const int rows = (N + 1) >> 1;
+  const int par = (parity >> 1) & 1;
+  const bool p32 = ldg(s.info + MLP_INFO_PATH32) != 0;
+  const int by = rb * MLP_2OPT_ROWS + wave;
+  const bool live = by < rows;
+  const int c0 = ct * C, c1 = min(c0 + C, N - 3);  // row-local indices [c0, c1)
+  // part 0: ctx < by  -> m = N-2-by, i = by - ctx  in [by - cb1 + 1, by - c0]
+  // part 1: ctx >= by -> m = by+1,   i = ctx-by+1  in [ca0 - by + 1, c1 - by]
+  // ilo / ihi / m are uniform over the wave (one row per wave).
+  int ilo[2], ihi[2], mm[2];
+  const int cb1 = min(c1, by), ca0 = max(c0, by);
+  mm[0] = N - 2 - by;
+  ilo[0] = by - cb1 + 1;
+  ihi[0] = (live && c0 < cb1) ? by - c0 : ilo[0] - 1;
+  mm[1] = by + 1;
+  ilo[1] = ca0 - by + 1;
+  ihi[1] = (live && ca0 < c1) ? c1 - by : ilo[1] - 1;
+  // e[x] = d(tour[ilo-1+x], tour[ilo+x+m]) for x in [0, ihi-ilo+1]
+  int pa[2][FE], pb[2][FE];
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const int ne = ihi[part] - ilo[part] + 2;
+    if (ne >= 2) {
+#pragma unroll
+      for (int it = 0; it < FE; ++it) {
+        const int p = ilo[part] - 1 + min(lane + it * kWave, ne - 1);
+        pa[part][it] = ldg(s.xy + p);
+        pb[part][it] = ldg(s.xy + p + mm[part] + 1);
+      }
+    }
+  }
+  if (!p32) {
+    sweep_2opt_wide(s, rb, ct, parity);
+    return;
+  }
+  if (ct == 0)
+    reset_other(s, 1, par, rb * MLP_2OPT_ROWS, min(rb * MLP_2OPT_ROWS + MLP_2OPT_ROWS, rows), rb == 0);
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const int ne = ihi[part] - ilo[part] + 2;
+    uint16_t* e = L.e[wave][part];
+    if (ne >= 2) {
+#pragma unroll
+      for (int it = 0; it < FE; ++it) {
+        const int x = lane + it * kWave;
+        if (x < ne) e[x] = (uint16_t)dist_xy(pa[part][it], pb[part][it], dr);
+      }
+    }
+  }
+  __syncthreads();
+  int32_t bd = INT32_MAX, bc = -1;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const uint16_t* e = L.e[wave][part];
+    const int m = mm[part], lo = ilo[part], hi = ihi[part];
+    for (int i0 = lo; i0 <= hi; i0 += CU * kWave) {
+      uint32_t gi[CU], hj[CU], ti[CU], tj[CU];
+#pragma unroll
+      for (int u = 0; u < CU; ++u) {
+        const int i = min(i0 + lane + u * kWave, hi);
+        gi[u] = (uint32_t)ldg(s.g32 + i);
+        hj[u] = (uint32_t)ldg(s.h32 + i + m);
+        ti[u] = (uint32_t)ldg(s.t32 + i);
+        tj[u] = (uint32_t)ldg(s.t32 + i + m);
+      }
+#pragma unroll
+      for (int u = 0; u < CU; ++u) {
+        const int i = i0 + lane + u * kWave;
+        const int ic = min(i, hi);
+        const int j = ic + m;
+        const int x = ic - lo;
+        uint32_t d = gi[u] + hj[u];
+        d += umul((uint32_t)(N - ic), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
+        d += (uint32_t)m * (ti[u] + tj[u]);
+        const int ctx = part == 0 ? by - ic : ic + by - 1;
+        if (i <= hi && ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc))) {
+          bd = (int32_t)d;
+          bc = ctx;
+        }
+      }
+    }
+  }
+  const int64_t key = wave_argmin(bd, bc);
+  if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, key);
 }
 
 __device__ __forceinline__ SweepSeg find_sweep(const SweepSeg* segs, int nsegs, int item) {
@@ -1106,14 +1181,8 @@ __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg
   const SlotDev& s = table[sg.slot];
   const int b = item - sg.blk0;
   const int rb = b / sg.ncols, ct = b % sg.ncols;
-  const bool p32 = s.info[MLP_INFO_PATH32] != 0;
-  if (sg.kind == 0) {
-    if (p32) sweep_fixed<R, C>(s, rb, ct, sg.mask, sg.parity, L.t);
-    else sweep_fixed_wide<R>(s, rb, ct, sg.mask, sg.parity);
-  } else {
-    if (p32) sweep_2opt<C>(s, rb, ct, sg.parity, L.d);
-    else sweep_2opt_wide(s, rb, ct, sg.parity);
-  }
+  if (sg.kind == 0) sweep_fixed<R, C>(s, rb, ct, sg.mask, sg.parity, L.t);
+  else sweep_2opt<C>(s, rb, ct, sg.parity, L.d);
 }
 
 }  // namespace
@@ -1142,6 +1211,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ ta
   }
 }
 
+template __global__ void k_sweep<4, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                         int, int*);
 template __global__ void k_sweep<8, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
                                          int, int*);
 template __global__ void k_sweep<16, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
@@ -1176,7 +1247,10 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid) {
-  if (tile == MLP_TILE_SMALL)
+  if (tile == MLP_TILE_4x256)
+    hipLaunchKernelGGL((k_sweep<4, 256>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
+                       nsegs, total, counter);
+  else if (tile == MLP_TILE_8x512)
     hipLaunchKernelGGL((k_sweep<8, 512>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
                        nsegs, total, counter);
   else
@@ -1186,8 +1260,8 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
 }
 
 void mlp_tile_shape(int tile, int* rows, int* cols) {
-  *rows = tile == MLP_TILE_SMALL ? 8 : 16;
-  *cols = tile == MLP_TILE_SMALL ? 512 : 1024;
+  *rows = tile == MLP_TILE_4x256 ? 4 : tile == MLP_TILE_8x512 ? 8 : 16;
+  *cols = tile == MLP_TILE_4x256 ? 256 : tile == MLP_TILE_8x512 ? 512 : 1024;
 }
 
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {

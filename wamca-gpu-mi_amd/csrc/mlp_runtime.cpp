@@ -132,7 +132,7 @@ struct mlp_ctx {
   size_t i64_cap = 0;
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
-  int tile_override = -1;  // -1 auto, else MLP_TILE_SMALL / MLP_TILE_LARGE
+  int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
   // per-launch timing of k_eval (HIP events on the context stream)
   bool timing = false;
   std::vector<hipEvent_t> tev;  // pairs
@@ -591,6 +591,17 @@ hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
                          a.counter, a.grid);
 }
 
+// Resident workgroups per CU of each sweep tile shape (persistent grids).
+const int kTileWgPerCu[MLP_TILE_COUNT] = {5, 4, 3};
+
+// Shape for a launch of `count` tours: small tiles give a single tour enough
+// workgroups to fill the chip; large tiles evaluate fewer distances per move.
+int auto_tile(const mlp_ctx*, int count) {
+  if (count <= 2) return MLP_TILE_4x256;
+  if (count <= 16) return MLP_TILE_8x512;
+  return MLP_TILE_16x1024;
+}
+
 hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
   EvalArgs& a = *(EvalArgs*)p;
   return mlp_launch_sweep(ctx->stream, a.rowmap, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total,
@@ -617,7 +628,7 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   if (rowmap == 0) {
     // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
     int tile = ctx->tile_override;
-    if (tile < 0) tile = count <= 8 ? MLP_TILE_SMALL : MLP_TILE_LARGE;
+    if (tile < 0) tile = auto_tile(ctx, count);
     int TR, TC;
     mlp_tile_shape(tile, &TR, &TC);
     std::vector<SweepSeg> segs;
@@ -666,7 +677,7 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
     if (persistent) {
       a.counter = ctx->d_counter;
       HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
-      a.grid = std::min(total, ctx->cu_count * (tile == MLP_TILE_SMALL ? 8 : 3));
+      a.grid = std::min(total, ctx->cu_count * kTileWgPerCu[tile]);
     }
     return timed_launch(ctx, launch_sweep_cb, &a);
   }
@@ -706,8 +717,8 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
 
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
   if (!ctx) return MLP_EINVAL;
-  if (tile != -1 && tile != MLP_TILE_SMALL && tile != MLP_TILE_LARGE)
-    return fail(ctx, MLP_EINVAL, "tile must be -1 (auto), 0 (small) or 1 (large)");
+  if (tile < -1 || tile >= MLP_TILE_COUNT)
+    return fail(ctx, MLP_EINVAL, "tile must be -1 (auto), 0 (4x256), 1 (8x512) or 2 (16x1024)");
   ctx->tile_override = tile;
   return MLP_OK;
 }

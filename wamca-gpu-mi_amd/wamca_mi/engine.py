@@ -188,7 +188,7 @@ class Engine:
         return out.value
 
     def set_tile(self, tile: int):
-        """-1 automatic, 0 small tiles (single tours), 1 large tiles (batches)."""
+        """Sweep tile shape: -1 automatic, 0 = 4x256, 1 = 8x512, 2 = 16x1024."""
         self._ck(self._L.mlp_set_tile(self.h, tile))
 
     def timing_begin(self, capacity=8192):
