@@ -131,6 +131,23 @@ int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int
 int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
              int32_t max_rounds, int32_t* rounds_out);
 
+/* DVND in steps, for callers that interleave other work between rounds (a
+ * serving loop admitting new jobs).  mlp_dvnd_start marks the slots active
+ * with no rounds run; mlp_dvnd_rounds enqueues `rounds` rounds over the
+ * slots (asynchronous; stopped slots are skipped on the device: a slot stops
+ * after a round that applied no move, or after max_rounds > 0 rounds);
+ * mlp_dvnd_status synchronises and reports, per slot, whether it is still
+ * active and how many rounds it ran.  mlp_dvnd runs these to completion. */
+int mlp_dvnd_start(mlp_ctx* ctx, const int32_t* slots, int32_t count);
+int mlp_dvnd_rounds(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
+                    int32_t rounds, int32_t max_rounds);
+int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* active,
+                    int32_t* rounds);
+/* Rounds per convergence check (1..64, default 4), capture of a chunk of
+ * rounds into a HIP graph replayed while the active set is unchanged
+ * (default on), persistent evaluation grids (default off). */
+int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t persistent);
+
 /* The reference's ADS buffer (mlads.h:38-74) for the slot's current tour, built
  * on the device: 128-byte header, coords, [client|weight], T, C[N][N] (uint32
  * with the reference's wrap-around).  bytes = 128 + 4*N*(N+3). */

@@ -1,0 +1,140 @@
+"""Batched DVND on the GPU against the oracle's DVND (oracle/mlp_oracle.cpp
+or_dvnd: rounds of swap, 2-opt, or-opt-1/2/3, each evaluated, merged and
+applied in turn, until a round applies nothing).
+
+Covers every execution form of the rounds (chunk size, HIP-graph replay,
+persistent grids), both row maps, max_rounds, the stepwise API a serving loop
+uses (slots joining a running batch between chunks), and instances of uneven
+n in one batch.  Final tours, latencies and round counts must be identical to
+the oracle's for every slot, alone or in a batch.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+wm = pytest.importorskip("wamca_mi")
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = wm.Engine(0)
+    yield e
+    e.dvnd_config(4, True, False)
+    e.close()
+
+
+def _jobs(ns, seed0, rnd=1):
+    jobs = []
+    for q, n in enumerate(ns):
+        if q % 2:
+            xs, ys = O.instance_clustered(n, seed0 + q)
+        else:
+            xs, ys = O.instance_uniform(n, seed0 + q)
+        W = O.weights(xs, ys, rnd)
+        t0 = O.random_tour(O.MT(seed0 + q), W, 0.5)
+        jobs.append((xs, ys, W, t0))
+    return jobs
+
+
+def _run(eng, jobs, rowmap, max_rounds=0, rnd=1):
+    insts = [eng.instance(xs, ys, rnd) for xs, ys, _, _ in jobs]
+    S = [eng.slot(i) for i in insts]
+    eng.set_tours(S, [j[3] for j in jobs])
+    rounds = eng.dvnd(S, rowmap, max_rounds)
+    out = [(eng.get_tour(s), int(eng.latency([s])[0]), int(rounds[q])) for q, s in enumerate(S)]
+    for s in S:
+        eng.destroy_slot(s)
+    for i in insts:
+        eng.destroy_instance(i)
+    return out
+
+
+@pytest.mark.parametrize("chunk,graph,persistent", [(1, False, False), (4, True, False),
+                                                    (3, True, True), (8, False, True)])
+@pytest.mark.parametrize("rowmap", [0, 1])
+def test_dvnd_forms_match_oracle(eng, chunk, graph, persistent, rowmap):
+    jobs = _jobs([40, 97, 180, 333, 611], 900 + rowmap)
+    eng.dvnd_config(chunk, graph, persistent)
+    try:
+        got = _run(eng, jobs, rowmap)
+    finally:
+        eng.dvnd_config(4, True, False)
+    for q, (xs, ys, W, t0) in enumerate(jobs):
+        tref, Lref, rref, _ = O.dvnd(W, t0, rowmap)
+        assert got[q][1] == Lref, (q, chunk, graph, persistent)
+        assert np.array_equal(got[q][0], tref)
+        assert got[q][2] == rref
+
+
+@pytest.mark.parametrize("max_rounds", [1, 2, 5])
+def test_dvnd_max_rounds(eng, max_rounds):
+    jobs = _jobs([120, 260], 77)
+    got = _run(eng, jobs, 0, max_rounds)
+    for q, (xs, ys, W, t0) in enumerate(jobs):
+        tref, Lref, rref, _ = O.dvnd(W, t0, 0, max_rounds)
+        assert got[q][1] == Lref and np.array_equal(got[q][0], tref) and got[q][2] == rref
+
+
+def test_dvnd_stepwise_join_equals_alone(eng):
+    """Slots that join a running batch between chunks (as a serving loop
+    admits jobs) end exactly where they end alone."""
+    jobs = _jobs([150, 64, 300, 90, 500, 210], 4242)
+    insts = [eng.instance(xs, ys, 1) for xs, ys, _, _ in jobs]
+    S = [eng.slot(i) for i in insts]
+    try:
+        running = []
+        pending = list(range(len(jobs)))
+        finished = {}
+        step = 0
+        while pending or running:
+            # admit up to two new jobs every other step
+            if pending and step % 2 == 0:
+                new = pending[:2]
+                pending = pending[2:]
+                eng.set_tours([S[q] for q in new], [jobs[q][3] for q in new])
+                eng.dvnd_start([S[q] for q in new])
+                running += new
+            eng.dvnd_rounds([S[q] for q in running], 3, 0)
+            act, rnd = eng.dvnd_status([S[q] for q in running])
+            still = []
+            for k, q in enumerate(running):
+                if act[k]:
+                    still.append(q)
+                else:
+                    finished[q] = (eng.get_tour(S[q]), int(eng.latency([S[q]])[0]), int(rnd[k]))
+            running = still
+            step += 1
+        for q, (xs, ys, W, t0) in enumerate(jobs):
+            tref, Lref, rref, _ = O.dvnd(W, t0, 0)
+            assert finished[q][1] == Lref and np.array_equal(finished[q][0], tref)
+            assert finished[q][2] == rref
+    finally:
+        for s in S:
+            eng.destroy_slot(s)
+        for i in insts:
+            eng.destroy_instance(i)
+
+
+def test_dvnd_many_slots_list_on_device(eng):
+    """More slots than the inline kernel-argument lists hold (device lists,
+    captured in the graph after their first use)."""
+    xs, ys = O.instance_uniform(60, 5)
+    W = O.weights(xs, ys, 1)
+    inst = eng.instance(xs, ys, 1)
+    tours = [O.random_tour(O.MT(1000 + q), W, 0.5) for q in range(150)]
+    S = [eng.slot(inst) for _ in tours]
+    try:
+        eng.set_tours(S, tours)
+        rounds = eng.dvnd(S, 0)
+        for q in range(0, 150, 7):
+            tref, Lref, rref, _ = O.dvnd(W, tours[q], 0)
+            assert int(eng.latency([S[q]])[0]) == Lref
+            assert np.array_equal(eng.get_tour(S[q]), tref)
+            assert int(rounds[q]) == rref
+    finally:
+        for s in S:
+            eng.destroy_slot(s)
+        eng.destroy_instance(inst)

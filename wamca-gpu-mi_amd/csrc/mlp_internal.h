@@ -27,8 +27,11 @@ enum {
   MLP_INFO_PATH32 = 2,    // 1 if the int32 path is exact for this tour
   MLP_INFO_ROUND_MOVES = 3,  // moves applied since the last round reset
   MLP_INFO_TOTAL_MOVES = 4,  // moves applied since the tour was set
+  MLP_INFO_ACTIVE = 5,    // 1 while a DVND run on this slot continues
+  MLP_INFO_ROUNDS = 6,    // DVND rounds run since the tour was set
   MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
   MLP_INFO_PARITY0 = 16,  // row buffer (0/1) holding the last eval of nb (16 + nb)
+  MLP_INFO_DONE0 = 21,    // finished tiles of the running sweep, per kind (21 + kind)
   MLP_INFO_WORDS = 24
 };
 
@@ -50,9 +53,11 @@ struct SlotDev {
   int64_t* t64;   // [N]
   int64_t* q64;   // [N]
   int64_t* info;  // [MLP_INFO_WORDS]
-  // per-row best keys (delta << 20 | ctx), two buffers of rowcap entries: an
-  // eval of nb folds its partial results into buffer p with atomicMin and
-  // resets buffer p ^ 1 for the next eval (p recorded in info[PARITY0 + nb]).
+  // per-row best keys (delta << 20 | ctx), two buffers of rowcap entries.
+  // info[PARITY0 + nb] = p names the buffer holding the last result; the
+  // other one is all KEY_NONE.  A sweep folds its tiles into buffer p ^ 1
+  // with atomicMin, resets buffer p, and its last tile sets the parity to
+  // p ^ 1; a one-wave-per-row eval overwrites buffer p in place.
   int64_t* rows[MLP_NB_COUNT];
   int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
   int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
@@ -69,9 +74,8 @@ struct EvalSeg {
   int32_t nb;
   int32_t rows;
   int32_t blk0;
-  int32_t parity;
 };
-#define MLP_EVALSEG_INTS 5
+#define MLP_EVALSEG_INTS 4
 
 // Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
 // the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
@@ -82,7 +86,7 @@ struct SweepSeg {
   int32_t mask;    // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
   int32_t blk0;
   int32_t ncols;   // column tiles per row block
-  int32_t parity;  // bit nb: row buffer written by this eval
+  int32_t ntiles;  // tiles of this segment (completion count)
 };
 #define MLP_SWEEPSEG_INTS 6
 // Tile shapes (rows x columns) of the sweep: 4 x 256 (most workgroups, for

@@ -273,22 +273,18 @@ __device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, in
 }
 
 __device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
-                          int rowmap) {
+                          int rowmap, bool skip_inactive) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int row = (item - sg.blk0) * MLP_EVAL_ROWS_PER_WG + wave;
   const SlotDev& s = table[sg.slot];
-  const int par = sg.parity;
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
+  // every row of the map is written by one wave, so the buffer holding the
+  // last result is overwritten in place; the other one stays all KEY_NONE
+  const int par = (int)s.info[MLP_INFO_PARITY0 + sg.nb];
   int64_t* cur = s.rows[sg.nb] + (par ? s.rowcap : 0);
-  int64_t* oth = s.rows[sg.nb] + (par ? 0 : s.rowcap);
-  if (item == sg.blk0) {  // first block: clear the tails of both buffers, record parity
-    for (int r = sg.rows + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
-      cur[r] = MLP_KEY_NONE;
-      oth[r] = MLP_KEY_NONE;
-    }
-    if (threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + sg.nb] = par;
-  }
+  if (item == sg.blk0)  // first block: clear the tail beyond this row map
+    for (int r = sg.rows + (int)threadIdx.x; r < s.rowcap; r += kBlock) cur[r] = MLP_KEY_NONE;
   if (row >= sg.rows) return;
-  if (lane == 0) oth[row] = MLP_KEY_NONE;
   if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
   else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane, cur + row);
 }
@@ -398,16 +394,53 @@ __global__ __launch_bounds__(kBlock) void k_prep(const SlotDev* __restrict__ tab
   if (threadIdx.x == 0) {
     s.info[MLP_INFO_ROUND_MOVES] = 0;
     s.info[MLP_INFO_TOTAL_MOVES] = 0;
+    s.info[MLP_INFO_ACTIVE] = 1;
+    s.info[MLP_INFO_ROUNDS] = 0;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_round_reset(const SlotDev* __restrict__ table,
-                                                        SlotListInline list,
-                                                        const int32_t* list_dev, int count) {
+// Start of a DVND run: every listed slot becomes active with no rounds.
+__global__ __launch_bounds__(kBlock) void k_dvnd_start(const SlotDev* __restrict__ table,
+                                                       SlotListInline list,
+                                                       const int32_t* list_dev, int count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < count) {
-    const int id = list_at(list, list_dev, t);
-    table[id].info[MLP_INFO_ROUND_MOVES] = 0;
+    const SlotDev& s = table[list_at(list, list_dev, t)];
+    s.info[MLP_INFO_ACTIVE] = 1;
+    s.info[MLP_INFO_ROUNDS] = 0;
+  }
+}
+
+// Start of a DVND round: active slots count their moves from zero; the
+// persistent-grid work counters of the round's launches are cleared (a
+// kernel, not a memset, so the round can be captured in a graph).
+__global__ __launch_bounds__(kBlock) void k_round_begin(const SlotDev* __restrict__ table,
+                                                        SlotListInline list,
+                                                        const int32_t* list_dev, int count,
+                                                        int* counters, int ncounters) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) {
+    const SlotDev& s = table[list_at(list, list_dev, t)];
+    if (s.info[MLP_INFO_ACTIVE]) s.info[MLP_INFO_ROUND_MOVES] = 0;
+  }
+  if (t < ncounters) counters[t] = 0;
+}
+
+// End of a DVND round: an active slot that applied no move (or reached
+// max_rounds > 0) stops.
+__global__ __launch_bounds__(kBlock) void k_round_end(const SlotDev* __restrict__ table,
+                                                      SlotListInline list,
+                                                      const int32_t* list_dev, int count,
+                                                      int max_rounds) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) {
+    const SlotDev& s = table[list_at(list, list_dev, t)];
+    if (s.info[MLP_INFO_ACTIVE]) {
+      const int64_t r = s.info[MLP_INFO_ROUNDS] + 1;
+      s.info[MLP_INFO_ROUNDS] = r;
+      if (s.info[MLP_INFO_ROUND_MOVES] == 0 || (max_rounds > 0 && r >= max_rounds))
+        s.info[MLP_INFO_ACTIVE] = 0;
+    }
   }
 }
 
@@ -416,11 +449,11 @@ __global__ __launch_bounds__(kBlock) void k_round_reset(const SlotDev* __restric
 __global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ table,
                                                  EvalSegsInline inl, const EvalSeg* segs_dev,
                                                  int nsegs, int total, int rowmap,
-                                                 int* counter) {
+                                                 int* counter, int skip_inactive) {
   if (!counter) {
     const int item = blockIdx.x;
     const EvalSeg sg = segs_dev ? find_seg(segs_dev, nsegs, item) : find_seg_inline(inl, nsegs, item);
-    eval_item(table, sg, item, rowmap);
+    eval_item(table, sg, item, rowmap, skip_inactive != 0);
     return;
   }
   __shared__ int s_item;
@@ -431,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ tab
     __syncthreads();
     if (item >= total) break;
     const EvalSeg sg = segs_dev ? find_seg(segs_dev, nsegs, item) : find_seg_inline(inl, nsegs, item);
-    eval_item(table, sg, item, rowmap);
+    eval_item(table, sg, item, rowmap, skip_inactive != 0);
   }
 }
 
@@ -479,12 +512,14 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
-                                                        int rowmap, int pcap, int ncap) {
+                                                        int rowmap, int pcap, int ncap,
+                                                        int skip_inactive) {
   extern __shared__ int64_t lds[];
   __shared__ int s_m, s_kept;
   __shared__ int64_t sh[2 * kBlock + 4];
   const int id = list_at(list, list_dev, blockIdx.x);
   const SlotDev& s = table[id];
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
   const int N = s.N, tid = threadIdx.x;
   const int R = rows_of(nb, rowmap, N);
   const int k = nb_k(nb);
@@ -782,8 +817,8 @@ __device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
 
-// Reset of the other row buffer by the tiles of column 0; the first tile also
-// clears the tail beyond this row map and records the parity.
+// Reset of the buffer holding the previous result (par ^ 1) by the tiles of
+// column 0; the first tile also clears both tails beyond this row map.
 __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
   int64_t* o = row_buf(s, nb, par ^ 1);
   for (int r = r0 + (int)threadIdx.x; r < r1; r += kBlock) o[r] = MLP_KEY_NONE;
@@ -792,7 +827,6 @@ __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, b
       o[r] = MLP_KEY_NONE;
       s.rows[nb][(par ? s.rowcap : 0) + r] = MLP_KEY_NONE;
     }
-    if (threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
   }
 }
 
@@ -1177,12 +1211,31 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
 
 template <int R, int C>
 __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds<R, C>& L) {
+                           SweepLds<R, C>& L, bool skip_inactive) {
   const SlotDev& s = table[sg.slot];
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
+  // bit nb of par: the buffer this sweep writes (the one not holding the last result)
+  int par = 0;
+#pragma unroll
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    if ((sg.mask >> nb) & 1) par |= (1 - (int)(s.info[MLP_INFO_PARITY0 + nb] & 1)) << nb;
   const int b = item - sg.blk0;
   const int rb = b / sg.ncols, ct = b % sg.ncols;
-  if (sg.kind == 0) sweep_fixed<R, C>(s, rb, ct, sg.mask, sg.parity, L.t);
-  else sweep_2opt<C>(s, rb, ct, sg.parity, L.d);
+  if (sg.kind == 0) sweep_fixed<R, C>(s, rb, ct, sg.mask, par, L.t);
+  else sweep_2opt<C>(s, rb, ct, par, L.d);
+  // the last tile of the segment to finish publishes the new parity
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned long long* done = (unsigned long long*)(s.info + MLP_INFO_DONE0 + sg.kind);
+    if (atomicAdd(done, 1ull) == (unsigned long long)(sg.ntiles - 1)) {
+      __threadfence();
+#pragma unroll
+      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+        if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
+      atomicExch(done, 0ull);
+    }
+  }
 }
 
 }  // namespace
@@ -1190,12 +1243,13 @@ __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg
 template <int R, int C>
 __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
-                                                  int nsegs, int total, int* counter) {
+                                                  int nsegs, int total, int* counter,
+                                                  int skip_inactive) {
   __shared__ SweepLds<R, C> L;
   if (!counter) {
     const int item = blockIdx.x;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C>(table, sg, item, L);
+    sweep_item<R, C>(table, sg, item, L, skip_inactive != 0);
     return;
   }
   __shared__ int s_item;
@@ -1206,17 +1260,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ ta
     __syncthreads();
     if (item >= total) break;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C>(table, sg, item, L);
+    sweep_item<R, C>(table, sg, item, L, skip_inactive != 0);
     __syncthreads();
   }
 }
 
 template __global__ void k_sweep<4, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*);
+                                         int, int*, int);
 template __global__ void k_sweep<8, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*);
+                                         int, int*, int);
 template __global__ void k_sweep<16, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                           int, int*);
+                                           int, int*, int);
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -1229,33 +1283,49 @@ hipError_t mlp_launch_prep(hipStream_t st, const SlotDev* table, const SlotListI
   return hipGetLastError();
 }
 
-hipError_t mlp_launch_round_reset(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
-                                  const int32_t* list_dev, int count) {
-  hipLaunchKernelGGL(k_round_reset, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+hipError_t mlp_launch_dvnd_start(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                 const int32_t* list_dev, int count) {
+  hipLaunchKernelGGL(k_dvnd_start, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                      table, inl, list_dev, count);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_round_begin(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                  const int32_t* list_dev, int count, int* counters,
+                                  int ncounters) {
+  const int n = std::max(count, ncounters);
+  hipLaunchKernelGGL(k_round_begin, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, table,
+                     inl, list_dev, count, counters, ncounters);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_round_end(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                const int32_t* list_dev, int count, int max_rounds) {
+  hipLaunchKernelGGL(k_round_end, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     table, inl, list_dev, count, max_rounds);
   return hipGetLastError();
 }
 
 hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsInline& inl,
                            const EvalSeg* segs_dev, int nsegs, int total, int rowmap,
-                           int* counter, int grid) {
+                           int* counter, int grid, int skip_inactive) {
   hipLaunchKernelGGL(k_eval, dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs, total,
-                     rowmap, counter);
+                     rowmap, counter, skip_inactive);
   return hipGetLastError();
 }
 
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
-                            int total, int* counter, int grid) {
+                            int total, int* counter, int grid, int skip_inactive) {
   if (tile == MLP_TILE_4x256)
     hipLaunchKernelGGL((k_sweep<4, 256>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter);
+                       nsegs, total, counter, skip_inactive);
   else if (tile == MLP_TILE_8x512)
     hipLaunchKernelGGL((k_sweep<8, 512>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter);
+                       nsegs, total, counter, skip_inactive);
   else
     hipLaunchKernelGGL((k_sweep<16, 1024>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter);
+                       nsegs, total, counter, skip_inactive);
   return hipGetLastError();
 }
 
@@ -1276,9 +1346,9 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds) {
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap) {
+                                  int pcap, int ncap, int skip_inactive) {
   hipLaunchKernelGGL(k_merge_apply, dim3(count), dim3(kBlock), mlp_merge_lds_bytes(pcap, ncap),
-                     st, table, inl, list_dev, nb, rowmap, pcap, ncap);
+                     st, table, inl, list_dev, nb, rowmap, pcap, ncap, skip_inactive);
   return hipGetLastError();
 }
 

@@ -10,20 +10,25 @@
 
 hipError_t mlp_launch_prep(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                            const int32_t* list_dev, int count);
-hipError_t mlp_launch_round_reset(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
-                                  const int32_t* list_dev, int count);
+hipError_t mlp_launch_dvnd_start(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                 const int32_t* list_dev, int count);
+hipError_t mlp_launch_round_begin(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                  const int32_t* list_dev, int count, int* counters,
+                                  int ncounters);
+hipError_t mlp_launch_round_end(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                const int32_t* list_dev, int count, int max_rounds);
 hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsInline& inl,
                            const EvalSeg* segs_dev, int nsegs, int total, int rowmap,
-                           int* counter, int grid);
+                           int* counter, int grid, int skip_inactive);
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
-                            int total, int* counter, int grid);
+                            int total, int* counter, int grid, int skip_inactive);
 void mlp_tile_shape(int tile, int* rows, int* cols);
 size_t mlp_merge_lds_bytes(int pcap, int ncap);
 hipError_t mlp_merge_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap);
+                                  int pcap, int ncap, int skip_inactive);
 hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, uint32_t* S,
                           uint32_t* out);
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);

@@ -36,7 +36,6 @@ struct Slot {
   void* mem = nullptr;
   SlotDev dev{};
   int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
-  int parity[MLP_NB_COUNT] = {0, 0, 0, 0, 0};  // row buffer of the last eval
   bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
 };
 
@@ -133,6 +132,16 @@ struct mlp_ctx {
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
+  // DVND rounds: launches are enqueued `dvnd_chunk` rounds at a time between
+  // convergence checks; with dvnd_graph the chunk is captured once per active
+  // set and replayed as a HIP graph; dvnd_persistent drains work counters.
+  int dvnd_chunk = 4;
+  int dvnd_graph = 1;
+  int dvnd_persistent = 0;
+  bool capturing = false;
+  hipGraphExec_t graph_exec = nullptr;
+  std::string graph_key;
+  int graph_uses = 0;
   // per-launch timing of k_eval (HIP events on the context stream)
   bool timing = false;
   std::vector<hipEvent_t> tev;  // pairs
@@ -172,8 +181,14 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     *out = (const int32_t*)it->second.p;
     return MLP_OK;
   }
+  if (ctx->capturing) return fail(ctx, MLP_ESTATE, "device list missing during graph capture");
   if (ctx->lists.size() >= 8192) {
     HCK(hipStreamSynchronize(ctx->stream));
+    if (ctx->graph_exec) {  // its launches point into the lists
+      HCK(hipGraphExecDestroy(ctx->graph_exec));
+      ctx->graph_exec = nullptr;
+      ctx->graph_key.clear();
+    }
     for (auto& kv : ctx->lists) hipFree(kv.second.p);
     ctx->lists.clear();
   }
@@ -299,6 +314,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   }
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->d_counter) hipFree(ctx->d_counter);
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
@@ -562,6 +578,10 @@ namespace {
 // Records one eval launch between a pair of HIP events (timing ring or the
 // context's last-launch pair).
 int timed_launch(mlp_ctx* ctx, hipError_t (*launch)(mlp_ctx*, void*), void* arg) {
+  if (ctx->capturing) {
+    HCK(launch(ctx, arg));
+    return MLP_OK;
+  }
   hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
   if (ctx->timing) {
     if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
@@ -581,14 +601,14 @@ struct EvalArgs {
   const EvalSeg* dev;
   SweepSegsInline sinl;
   const SweepSeg* sdev;
-  int nsegs, total, rowmap, grid;
+  int nsegs, total, rowmap, grid, skip;
   int* counter;
 };
 
 hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
   EvalArgs& a = *(EvalArgs*)p;
   return mlp_launch_eval(ctx->stream, ctx->d_table, a.inl, a.dev, a.nsegs, a.total, a.rowmap,
-                         a.counter, a.grid);
+                         a.counter, a.grid, a.skip);
 }
 
 // Resident workgroups per CU of each sweep tile shape (persistent grids).
@@ -605,17 +625,16 @@ int auto_tile(const mlp_ctx*, int count) {
 hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
   EvalArgs& a = *(EvalArgs*)p;
   return mlp_launch_sweep(ctx->stream, a.rowmap, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total,
-                          a.counter, a.grid);
+                          a.counter, a.grid, a.skip);
 }
 
 }  // namespace
 
-int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
-             int32_t rowmap, int32_t persistent) {
-  int st = check_slots(ctx, slots, count);
-  if (st) return st;
-  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
-  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+// One evaluation launch.  counter == nullptr: one workgroup per work item
+// (persistent == 0) or the context's standalone counter cleared by a memset;
+// a DVND round passes its own counter, cleared by k_round_begin.
+static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+                     int32_t rowmap, int32_t persistent, int skip, int* round_counter) {
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
@@ -625,6 +644,8 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
       }
   }
   EvalArgs a{};
+  a.skip = skip;
+  int st = MLP_OK;
   if (rowmap == 0) {
     // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
     int tile = ctx->tile_override;
@@ -636,29 +657,25 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
     int total = 0;
     const uint32_t fixed = nb_mask & 0x1du;
     for (int q = 0; q < count; ++q) {
-      Slot& s = ctx->slots[slots[q]];
-      int par = 0;
-      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
-        if (nb_mask & (1u << nb)) {
-          s.parity[nb] ^= 1;
-          par |= s.parity[nb] << nb;
-        }
+      const Slot& s = ctx->slots[slots[q]];
       if (fixed) {
         int rmax = 0;
         for (int nb : {0, 2, 3, 4})
           if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
         if (rmax > 0) {
           const int ncols = (s.N - 2 + TC - 1) / TC;
-          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, par});
-          total += (rmax + TR - 1) / TR * ncols;
+          const int nt = (rmax + TR - 1) / TR * ncols;
+          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, nt});
+          total += nt;
         }
       }
       if (nb_mask & 2u) {
         const int r = rows_of(1, 0, s.N);
         if (r > 0 && s.N >= 4) {
           const int ncols = (s.N - 3 + TC - 1) / TC;
-          segs.push_back(SweepSeg{slots[q], 1, 2, total, ncols, par});
-          total += (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
+          const int nt = (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
+          segs.push_back(SweepSeg{slots[q], 1, 2, total, ncols, nt});
+          total += nt;
         }
       }
     }
@@ -675,9 +692,13 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
     }
     a.grid = total;
     if (persistent) {
-      a.counter = ctx->d_counter;
-      HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
       a.grid = std::min(total, ctx->cu_count * kTileWgPerCu[tile]);
+      if (round_counter) {
+        a.counter = round_counter;
+      } else {
+        a.counter = ctx->d_counter;
+        HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+      }
     }
     return timed_launch(ctx, launch_sweep_cb, &a);
   }
@@ -685,13 +706,12 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   segs.reserve((size_t)count * 5);
   int total = 0;
   for (int q = 0; q < count; ++q) {
-    Slot& s = ctx->slots[slots[q]];
+    const Slot& s = ctx->slots[slots[q]];
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
       if (!(nb_mask & (1u << nb))) continue;
       const int R = rows_of(nb, rowmap, s.N);
       if (R <= 0) continue;
-      s.parity[nb] ^= 1;
-      segs.push_back(EvalSeg{slots[q], nb, R, total, s.parity[nb]});
+      segs.push_back(EvalSeg{slots[q], nb, R, total});
       total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
     }
   }
@@ -708,11 +728,24 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   }
   a.grid = total;
   if (persistent) {
-    a.counter = ctx->d_counter;
-    HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
     a.grid = std::min(total, ctx->cu_count * 8);
+    if (round_counter) {
+      a.counter = round_counter;
+    } else {
+      a.counter = ctx->d_counter;
+      HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+    }
   }
   return timed_launch(ctx, launch_eval_cb, &a);
+}
+
+int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+             int32_t rowmap, int32_t persistent) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  return eval_impl(ctx, slots, count, nb_mask, rowmap, persistent, 0, nullptr);
 }
 
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
@@ -773,7 +806,9 @@ int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t*
   if (R == 0) return MLP_OK;
   std::vector<int64_t> keys((size_t)R);
   HCK(hipStreamSynchronize(ctx->stream));
-  HCK(hipMemcpy(keys.data(), s.dev.rows[nb] + (s.parity[nb] ? s.dev.rowcap : 0), 8 * (size_t)R,
+  int64_t par = 0;
+  HCK(hipMemcpy(&par, s.dev.info + MLP_INFO_PARITY0 + nb, 8, hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(keys.data(), s.dev.rows[nb] + (par ? s.dev.rowcap : 0), 8 * (size_t)R,
                 hipMemcpyDeviceToHost));
   for (int r = 0; r < R; ++r) {
     const int64_t k = keys[(size_t)r];
@@ -818,10 +853,7 @@ int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int
   return MLP_OK;
 }
 
-int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb) {
-  int st = check_slots(ctx, slots, count);
-  if (st) return st;
-  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb, int skip) {
   if (count == 0) return MLP_OK;
   int rowmap = -1, pcap = 1, ncap = 1;
   for (int q = 0; q < count; ++q) {
@@ -847,11 +879,19 @@ int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t n
     const int c = std::min(65535, count - q0);
     SlotListInline inl;
     const int32_t* dev = nullptr;
-    st = slot_list(ctx, slots + q0, c, inl, &dev);
+    int st = slot_list(ctx, slots + q0, c, inl, &dev);
     if (st) return st;
-    HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap));
+    HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
+                               skip));
   }
   return MLP_OK;
+}
+
+int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  return merge_impl(ctx, slots, count, nb, 0);
 }
 
 int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int32_t cap,
@@ -878,47 +918,180 @@ int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int
   return MLP_OK;
 }
 
+namespace {
+
+// One DVND round over `slots`: round begin, {eval, merge + apply} for each
+// neighborhood in order, round end.  Slots that stopped are skipped by every
+// kernel, so the same launches serve all rounds of a chunk.
+int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds) {
+  SlotListInline inl;
+  const int32_t* dev = nullptr;
+  int st = slot_list(ctx, slots, count, inl, &dev);
+  if (st) return st;
+  HCK(mlp_launch_round_begin(ctx->stream, ctx->d_table, inl, dev, count, ctx->d_counter + 1,
+                             MLP_NB_COUNT));
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    bool any = false;
+    for (int q = 0; q < count && !any; ++q)
+      if (rows_of(nb, rowmap, ctx->slots[slots[q]].N) > 0) any = true;
+    if (!any) continue;
+    st = eval_impl(ctx, slots, count, 1u << nb, rowmap, ctx->dvnd_persistent, 1,
+                   ctx->d_counter + 1 + nb);
+    if (st) return st;
+    st = merge_impl(ctx, slots, count, nb, 1);
+    if (st) return st;
+  }
+  HCK(mlp_launch_round_end(ctx->stream, ctx->d_table, inl, dev, count, max_rounds));
+  return MLP_OK;
+}
+
+// `rounds` DVND rounds over `slots`, as one captured graph when enabled.  A
+// chunk is captured the second time the same (slots, settings) is seen, so
+// every device list it needs already exists (no copies inside a capture).
+int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
+              int rounds) {
+  if (!ctx->dvnd_graph) {
+    for (int r = 0; r < rounds; ++r) {
+      int st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
+      if (st) return st;
+    }
+    return MLP_OK;
+  }
+  int tile = ctx->tile_override;
+  if (tile < 0) tile = auto_tile(ctx, count);
+  // the captured launches embed the slot table address, every slot's id and
+  // size (segment tables, merge capacities) and the settings
+  const int32_t hdr[6] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count};
+  std::string key((const char*)hdr, sizeof hdr);
+  key.append((const char*)&ctx->d_table, sizeof ctx->d_table);
+  key.append((const char*)slots, 4 * (size_t)count);
+  for (int q = 0; q < count; ++q) {
+    const int32_t N = ctx->slots[slots[q]].N;
+    key.append((const char*)&N, 4);
+  }
+  if (key == ctx->graph_key && ctx->graph_exec) {
+    HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    // the launches' host bookkeeping (evaluated / merged flags)
+    for (int q = 0; q < count; ++q)
+      for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+        ctx->slots[slots[q]].eval_rowmap[nb] = rowmap;
+        ctx->slots[slots[q]].merged[nb] = true;
+      }
+    return MLP_OK;
+  }
+  if (key != ctx->graph_key) {  // first sight: run eagerly, remember the key
+    if (ctx->graph_exec) {
+      HCK(hipGraphExecDestroy(ctx->graph_exec));
+      ctx->graph_exec = nullptr;
+    }
+    ctx->graph_key = key;
+    for (int r = 0; r < rounds; ++r) {
+      int st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
+      if (st) return st;
+    }
+    return MLP_OK;
+  }
+  hipGraph_t g = nullptr;
+  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  int st = MLP_OK;
+  for (int r = 0; r < rounds && st == MLP_OK; ++r) st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
+  ctx->capturing = false;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  if (st || ec != hipSuccess) {  // nothing captured ran: run the chunk eagerly instead
+    if (g) hipGraphDestroy(g);
+    ctx->graph_key.clear();
+    for (int r = 0; r < rounds; ++r) {
+      st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
+      if (st) return st;
+    }
+    return MLP_OK;
+  }
+  const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    ctx->graph_exec = nullptr;
+    return fail(ctx, MLP_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+  }
+  HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return MLP_OK;
+}
+
+}  // namespace
+
+int mlp_dvnd_start(mlp_ctx* ctx, const int32_t* slots, int32_t count) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  for (int q0 = 0; q0 < count; q0 += 65535) {
+    const int c = std::min(65535, count - q0);
+    SlotListInline inl;
+    const int32_t* dev = nullptr;
+    st = slot_list(ctx, slots + q0, c, inl, &dev);
+    if (st) return st;
+    HCK(mlp_launch_dvnd_start(ctx->stream, ctx->d_table, inl, dev, c));
+  }
+  return MLP_OK;
+}
+
+int mlp_dvnd_rounds(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
+                    int32_t rounds, int32_t max_rounds) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  if (rounds < 0) return fail(ctx, MLP_EINVAL, "rounds must be >= 0");
+  if (count == 0 || rounds == 0) return MLP_OK;
+  return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
+}
+
+int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* active,
+                    int32_t* rounds) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (count == 0) return MLP_OK;
+  std::vector<int64_t> v((size_t)count);
+  if (active) {
+    st = gather_info(ctx, slots, count, MLP_INFO_ACTIVE, v.data());
+    if (st) return st;
+    for (int q = 0; q < count; ++q) active[q] = (int32_t)v[(size_t)q];
+  }
+  if (rounds) {
+    st = gather_info(ctx, slots, count, MLP_INFO_ROUNDS, v.data());
+    if (st) return st;
+    for (int q = 0; q < count; ++q) rounds[q] = (int32_t)v[(size_t)q];
+  }
+  return MLP_OK;
+}
+
+int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t persistent) {
+  if (!ctx) return MLP_EINVAL;
+  if (chunk < 1 || chunk > 64) return fail(ctx, MLP_EINVAL, "chunk must be in [1, 64]");
+  ctx->dvnd_chunk = chunk;
+  ctx->dvnd_graph = use_graph ? 1 : 0;
+  ctx->dvnd_persistent = persistent ? 1 : 0;
+  return MLP_OK;
+}
+
 int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
              int32_t max_rounds, int32_t* rounds_out) {
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
-  std::vector<int32_t> active(slots, slots + count);
-  std::vector<int> pos(ctx->slots.size(), -1);
-  for (int q = 0; q < count; ++q) {
-    pos[(size_t)slots[q]] = q;
-    if (rounds_out) rounds_out[q] = 0;
-  }
-  std::vector<int64_t> moved;
-  int round = 0;
-  while (!active.empty() && (max_rounds <= 0 || round < max_rounds)) {
+  if (count == 0) return MLP_OK;
+  st = mlp_dvnd_start(ctx, slots, count);
+  if (st) return st;
+  std::vector<int32_t> active(slots, slots + count), flag((size_t)count);
+  while (!active.empty()) {
     const int c = (int)active.size();
-    SlotListInline inl;
-    const int32_t* dev = nullptr;
-    st = slot_list(ctx, active.data(), c, inl, &dev);
+    st = run_chunk(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);
     if (st) return st;
-    HCK(mlp_launch_round_reset(ctx->stream, ctx->d_table, inl, dev, c));
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
-      bool any = false;
-      for (int32_t s : active)
-        if (rows_of(nb, rowmap, ctx->slots[s].N) > 0) any = true;
-      if (!any) continue;
-      st = mlp_eval(ctx, active.data(), c, 1u << nb, rowmap, 0);
-      if (st) return st;
-      st = mlp_merge_apply(ctx, active.data(), c, nb);
-      if (st) return st;
-    }
-    ++round;
-    moved.assign((size_t)c, 0);
-    st = gather_info(ctx, active.data(), c, MLP_INFO_ROUND_MOVES, moved.data());
+    st = mlp_dvnd_status(ctx, active.data(), c, flag.data(), nullptr);
     if (st) return st;
     std::vector<int32_t> next;
-    for (int q = 0; q < c; ++q) {
-      if (rounds_out) rounds_out[pos[(size_t)active[(size_t)q]]] = round;
-      if (moved[(size_t)q] > 0) next.push_back(active[(size_t)q]);
-    }
+    for (int q = 0; q < c; ++q)
+      if (flag[(size_t)q]) next.push_back(active[(size_t)q]);
     active.swap(next);
   }
+  if (rounds_out) return mlp_dvnd_status(ctx, slots, count, nullptr, rounds_out);
   return MLP_OK;
 }
 

@@ -60,6 +60,10 @@ _SIGS = {
     "mlp_get_applied": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(MoveRec),
                                   C.c_int32, C.POINTER(C.c_int32)]),
     "mlp_dvnd": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, C.c_int32, I32P]),
+    "mlp_dvnd_start": (C.c_int, [C.c_void_p, I32P, C.c_int32]),
+    "mlp_dvnd_rounds": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "mlp_dvnd_status": (C.c_int, [C.c_void_p, I32P, C.c_int32, I32P, I32P]),
+    "mlp_dvnd_config": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "mlp_ads_materialize": (C.c_int, [C.c_void_p, C.c_int32, U32P, C.c_int64]),
     "mlp_dist_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, I32P]),
     "mlp_dist_list": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, I32P]),

@@ -164,6 +164,27 @@ class Engine:
         self._ck(self._L.mlp_dvnd(self.h, slots, len(slots), rowmap, max_rounds, rounds))
         return rounds
 
+    def dvnd_config(self, chunk=4, use_graph=True, persistent=False):
+        """Rounds per convergence check, HIP-graph replay of a chunk, persistent grids."""
+        self._ck(self._L.mlp_dvnd_config(self.h, chunk, int(use_graph), int(persistent)))
+
+    def dvnd_start(self, slots):
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_dvnd_start(self.h, slots, len(slots)))
+
+    def dvnd_rounds(self, slots, rounds, rowmap=ROWMAP_ADS, max_rounds=0):
+        """Enqueue `rounds` DVND rounds over the slots (asynchronous)."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_dvnd_rounds(self.h, slots, len(slots), rowmap, rounds, max_rounds))
+
+    def dvnd_status(self, slots):
+        """(active flags, rounds run) per slot; synchronises the stream."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        act = np.zeros(len(slots), np.int32)
+        rnd = np.zeros(len(slots), np.int32)
+        self._ck(self._L.mlp_dvnd_status(self.h, slots, len(slots), act, rnd))
+        return act, rnd
+
     def ads(self, s):
         N = self.N[s]
         nbytes = 128 + 4 * N * (N + 3)
