@@ -138,3 +138,31 @@ def test_dvnd_many_slots_list_on_device(eng):
         for s in S:
             eng.destroy_slot(s)
         eng.destroy_instance(inst)
+
+
+def test_standalone_eval_after_dvnd(eng):
+    """Row buffers written inside DVND rounds (parity committed on the
+    device) are followed correctly by standalone evals and merges."""
+    xs, ys = O.instance_uniform(150, 31)
+    W = O.weights(xs, ys, 1)
+    inst = eng.instance(xs, ys, 1)
+    s = eng.slot(inst)
+    try:
+        for rowmap_dvnd, rowmap_eval in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            t0 = O.random_tour(O.MT(31 + rowmap_dvnd), W, 0.5)
+            eng.set_tour(s, t0)
+            eng.dvnd([s], rowmap_dvnd, 3)
+            t1 = O.perm_tour(5 + rowmap_eval, 150)
+            eng.set_tour(s, t1)
+            for rep in range(2):
+                eng.eval([s], 0x1F, rowmap_eval)
+                for nb in range(5):
+                    got = eng.rows(s, nb)
+                    ref = O.eval_rows(nb, rowmap_eval, W, t1)
+                    assert np.array_equal(got["cost"], ref["cost"]), (rowmap_dvnd, rowmap_eval, nb)
+            eng.merge_apply([s], 2)
+            t2, _, _ = O.step(2, rowmap_eval, W, t1)
+            assert np.array_equal(eng.get_tour(s), t2)
+    finally:
+        eng.destroy_slot(s)
+        eng.destroy_instance(inst)

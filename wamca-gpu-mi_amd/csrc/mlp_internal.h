@@ -31,7 +31,6 @@ enum {
   MLP_INFO_ROUNDS = 6,    // DVND rounds run since the tour was set
   MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
   MLP_INFO_PARITY0 = 16,  // row buffer (0/1) holding the last eval of nb (16 + nb)
-  MLP_INFO_DONE0 = 21,    // finished tiles of the running sweep, per kind (21 + kind)
   MLP_INFO_WORDS = 24
 };
 
@@ -55,9 +54,10 @@ struct SlotDev {
   int64_t* info;  // [MLP_INFO_WORDS]
   // per-row best keys (delta << 20 | ctx), two buffers of rowcap entries.
   // info[PARITY0 + nb] = p names the buffer holding the last result; the
-  // other one is all KEY_NONE.  A sweep folds its tiles into buffer p ^ 1
-  // with atomicMin, resets buffer p, and its last tile sets the parity to
-  // p ^ 1; a one-wave-per-row eval overwrites buffer p in place.
+  // other one is all KEY_NONE.  An eval writes buffer p ^ 1 (column tiles
+  // fold with atomicMin) and resets buffer p.  A standalone eval gets p ^ 1
+  // from the host and its first tile records it; inside a DVND round the
+  // eval reads p from info and the merge that follows flips it.
   int64_t* rows[MLP_NB_COUNT];
   int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
   int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
@@ -74,8 +74,9 @@ struct EvalSeg {
   int32_t nb;
   int32_t rows;
   int32_t blk0;
+  int32_t parity;  // buffer written by a standalone eval
 };
-#define MLP_EVALSEG_INTS 4
+#define MLP_EVALSEG_INTS 5
 
 // Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
 // the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
@@ -86,7 +87,7 @@ struct SweepSeg {
   int32_t mask;    // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
   int32_t blk0;
   int32_t ncols;   // column tiles per row block
-  int32_t ntiles;  // tiles of this segment (completion count)
+  int32_t parity;  // bit nb: buffer written by a standalone eval
 };
 #define MLP_SWEEPSEG_INTS 6
 // Tile shapes (rows x columns) of the sweep: 4 x 256 (most workgroups, for

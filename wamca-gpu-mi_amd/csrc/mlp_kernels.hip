@@ -273,18 +273,23 @@ __device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, in
 }
 
 __device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
-                          int rowmap, bool skip_inactive) {
+                          int rowmap, bool dvnd) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int row = (item - sg.blk0) * MLP_EVAL_ROWS_PER_WG + wave;
   const SlotDev& s = table[sg.slot];
-  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
-  // every row of the map is written by one wave, so the buffer holding the
-  // last result is overwritten in place; the other one stays all KEY_NONE
-  const int par = (int)s.info[MLP_INFO_PARITY0 + sg.nb];
+  if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
+  const int par = dvnd ? 1 - (int)(s.info[MLP_INFO_PARITY0 + sg.nb] & 1) : sg.parity;
   int64_t* cur = s.rows[sg.nb] + (par ? s.rowcap : 0);
-  if (item == sg.blk0)  // first block: clear the tail beyond this row map
-    for (int r = sg.rows + (int)threadIdx.x; r < s.rowcap; r += kBlock) cur[r] = MLP_KEY_NONE;
+  int64_t* oth = s.rows[sg.nb] + (par ? 0 : s.rowcap);
+  if (item == sg.blk0) {  // first block: clear the tails of both buffers, record parity
+    for (int r = sg.rows + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
+      cur[r] = MLP_KEY_NONE;
+      oth[r] = MLP_KEY_NONE;
+    }
+    if (!dvnd && threadIdx.x == 0) s.info[MLP_INFO_PARITY0 + sg.nb] = par;
+  }
   if (row >= sg.rows) return;
+  if (lane == 0) oth[row] = MLP_KEY_NONE;
   if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
   else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane, cur + row);
 }
@@ -532,8 +537,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   if (tid == 0) s_m = 0;
   for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
   __syncthreads();
-  // 1. improving rows -> (cost, row) keys
-  const int64_t* rk = s.rows[nb] + (s.info[MLP_INFO_PARITY0 + nb] ? s.rowcap : 0);
+  // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
+  // wrote the buffer opposite info[PARITY0 + nb]; this merge commits it.
+  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
+  const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
+  if (skip_inactive) {
+    __syncthreads();  // every thread has read the old parity
+    if (tid == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
+  }
   for (int r = tid; r < R; r += kBlock) {
     const int64_t key = rk[r];
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
@@ -1211,31 +1222,26 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
 
 template <int R, int C>
 __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds<R, C>& L, bool skip_inactive) {
+                           SweepLds<R, C>& L, bool dvnd) {
   const SlotDev& s = table[sg.slot];
-  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
+  if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
   // bit nb of par: the buffer this sweep writes (the one not holding the last result)
-  int par = 0;
+  int par = sg.parity;
+  if (dvnd) {
+    par = 0;
 #pragma unroll
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
-    if ((sg.mask >> nb) & 1) par |= (1 - (int)(s.info[MLP_INFO_PARITY0 + nb] & 1)) << nb;
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      if ((sg.mask >> nb) & 1) par |= (1 - (int)(s.info[MLP_INFO_PARITY0 + nb] & 1)) << nb;
+  }
   const int b = item - sg.blk0;
   const int rb = b / sg.ncols, ct = b % sg.ncols;
+  if (!dvnd && b == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
+  }
   if (sg.kind == 0) sweep_fixed<R, C>(s, rb, ct, sg.mask, par, L.t);
   else sweep_2opt<C>(s, rb, ct, par, L.d);
-  // the last tile of the segment to finish publishes the new parity
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned long long* done = (unsigned long long*)(s.info + MLP_INFO_DONE0 + sg.kind);
-    if (atomicAdd(done, 1ull) == (unsigned long long)(sg.ntiles - 1)) {
-      __threadfence();
-#pragma unroll
-      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
-        if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
-      atomicExch(done, 0ull);
-    }
-  }
 }
 
 }  // namespace

@@ -36,6 +36,8 @@ struct Slot {
   void* mem = nullptr;
   SlotDev dev{};
   int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
+  int parity[MLP_NB_COUNT] = {0, 0, 0, 0, 0};  // row buffer of the last eval
+  bool parity_stale = false;  // DVND rounds flipped parities on the device
   bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
 };
 
@@ -637,6 +639,15 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
                      int32_t rowmap, int32_t persistent, int skip, int* round_counter) {
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
+    if (skip) {
+      s.parity_stale = true;  // the DVND merges commit parities on the device
+    } else if (s.parity_stale) {
+      int64_t par[MLP_NB_COUNT];
+      HCK(hipStreamSynchronize(ctx->stream));
+      HCK(hipMemcpy(par, s.dev.info + MLP_INFO_PARITY0, sizeof par, hipMemcpyDeviceToHost));
+      for (int nb = 0; nb < MLP_NB_COUNT; ++nb) s.parity[nb] = (int)(par[nb] & 1);
+      s.parity_stale = false;
+    }
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if (nb_mask & (1u << nb)) {
         s.eval_rowmap[nb] = rowmap;
@@ -657,25 +668,30 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     int total = 0;
     const uint32_t fixed = nb_mask & 0x1du;
     for (int q = 0; q < count; ++q) {
-      const Slot& s = ctx->slots[slots[q]];
+      Slot& s = ctx->slots[slots[q]];
+      int par = 0;
+      if (!skip)
+        for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+          if (nb_mask & (1u << nb)) {
+            s.parity[nb] ^= 1;
+            par |= s.parity[nb] << nb;
+          }
       if (fixed) {
         int rmax = 0;
         for (int nb : {0, 2, 3, 4})
           if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
         if (rmax > 0) {
           const int ncols = (s.N - 2 + TC - 1) / TC;
-          const int nt = (rmax + TR - 1) / TR * ncols;
-          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, nt});
-          total += nt;
+          segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, par});
+          total += (rmax + TR - 1) / TR * ncols;
         }
       }
       if (nb_mask & 2u) {
         const int r = rows_of(1, 0, s.N);
         if (r > 0 && s.N >= 4) {
           const int ncols = (s.N - 3 + TC - 1) / TC;
-          const int nt = (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
-          segs.push_back(SweepSeg{slots[q], 1, 2, total, ncols, nt});
-          total += nt;
+          segs.push_back(SweepSeg{slots[q], 1, 2, total, ncols, par});
+          total += (r + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
         }
       }
     }
@@ -706,12 +722,13 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
   segs.reserve((size_t)count * 5);
   int total = 0;
   for (int q = 0; q < count; ++q) {
-    const Slot& s = ctx->slots[slots[q]];
+    Slot& s = ctx->slots[slots[q]];
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
       if (!(nb_mask & (1u << nb))) continue;
       const int R = rows_of(nb, rowmap, s.N);
       if (R <= 0) continue;
-      segs.push_back(EvalSeg{slots[q], nb, R, total});
+      if (!skip) s.parity[nb] ^= 1;
+      segs.push_back(EvalSeg{slots[q], nb, R, total, s.parity[nb]});
       total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
     }
   }
@@ -972,11 +989,14 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
   if (key == ctx->graph_key && ctx->graph_exec) {
     HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
     // the launches' host bookkeeping (evaluated / merged flags)
-    for (int q = 0; q < count; ++q)
+    for (int q = 0; q < count; ++q) {
+      Slot& s = ctx->slots[slots[q]];
+      s.parity_stale = true;
       for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
-        ctx->slots[slots[q]].eval_rowmap[nb] = rowmap;
-        ctx->slots[slots[q]].merged[nb] = true;
+        s.eval_rowmap[nb] = rowmap;
+        s.merged[nb] = true;
       }
+    }
     return MLP_OK;
   }
   if (key != ctx->graph_key) {  // first sight: run eagerly, remember the key
