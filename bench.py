@@ -7,9 +7,10 @@ each evaluated over swap, 2-opt and or-opt-1/2/3 (3,982,521 moves per tour)
 with one launch per tour.  A step is one pass over the 1000 tours; inputs are
 resident in HBM before the timed region.
 
-Prints one JSON line: moves/s (value), the k_eval roofline against the LDS
-operand bytes of the reference formulation, the batched (all tours in one
-persistent launch) rate, and the CPU oracle timed on a bounded sample.
+Prints one JSON line: moves/s (value), the k_sweep roofline against the LDS
+operand bytes of the reference formulation (average launch time from HIP
+events around the timed region on the engine's stream), the batched rate (all
+tours in one launch), and the CPU oracle timed on a bounded sample.
 """
 import argparse
 import json
@@ -21,8 +22,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "wamca-gpu-mi_amd"))
 
-import numpy as np  # noqa: E402
-
 # MI355X published figures used as roofline denominators
 PEAK_HBM_GBS = 8000.0
 CUS, LDS_BYTES_PER_CLK, CLOCK_GHZ = 256, 128, 2.4
@@ -31,6 +30,8 @@ PEAK_LDS_GBS = CUS * LDS_BYTES_PER_CLK * CLOCK_GHZ  # 78,643 GB/s
 # lane-varying operand bytes per move of the reference's ADS formulation
 # (swap 7 words, 2-opt 5, or-opt 5 forward / 6 backward -> 22 B average)
 LDS_BYTES = {0: 28, 1: 20, 2: 22, 3: 22, 4: 22}
+TILES = {0: "k_sweep<4,256>", 1: "k_sweep<8,512>", 2: "k_sweep<16,1024>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_n1000_traffic.json")
 
 
 def moves_per_nb(N):
@@ -55,6 +56,19 @@ def cpu_baseline(xs, ys, tours, budget_s=12.0):
                       f"({moves} moves, {dt:.1f} s, oracle eval_rows, 1 thread)"}
 
 
+def traffic_per_launch(tile):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (calibrated against a copy kernel), if present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("kernel") == TILES[tile]:
+            return t.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,7 +76,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--tours", type=int, default=1000)
+    ap.add_argument("--tile", type=int, default=-1, help="-1: autotune over the three shapes")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run device copies of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -71,11 +89,17 @@ def main():
     import wamca_mi
     from wamca_mi import inputs as I
 
+    eng = wamca_mi.Engine(0)
+    if args.copy_bytes:
+        eng.copy_probe(args.copy_bytes, 4)
+        print(json.dumps({"copy_bytes": args.copy_bytes, "reps": 4}))
+        eng.close()
+        return
+
     n, T = args.n, args.tours
     N = n + 1
     xs, ys = I.uniform(n, 1, 1000)
     tours = [I.perm_tour(seed, n) for seed in range(1, T + 1)]
-    eng = wamca_mi.Engine(0)
     info = eng.device_info()
     inst = eng.instance(xs, ys, 1)
     slots = eng.slots(inst, T)
@@ -84,43 +108,40 @@ def main():
     mp = moves_per_nb(N)
     per_tour = sum(mp.values())
 
-    def step():
-        for s in slots:
-            eng.eval([s], 0x1F, 0)
+    # autotune (untimed): tile shape for one-tour launches
+    tune = {}
+    tile = args.tile
+    if tile < 0:
+        probe = slots[:min(T, 128)]
+        for t in (0, 1, 2):
+            eng.set_tile(t)
+            eng.eval_each(probe, 0x1F, 0)
+            eng.region_begin()
+            eng.eval_each(probe, 0x1F, 0)
+            tune[TILES[t]] = eng.region_end() * 1e3 / len(probe)
+        tile = min(tune, key=tune.get)
+        tile = [k for k, v in TILES.items() if v == tile][0]
+    eng.set_tile(tile)
 
     for _ in range(args.warmup):
-        step()
+        eng.eval_each(slots, 0x1F, 0)
     eng.sync()
-    eng.timing_begin(args.steps * T + 8)
+    eng.region_begin()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
-    eng.sync()
+        eng.eval_each(slots, 0x1F, 0)
+    region_ms = eng.region_end()
     dt = time.perf_counter() - t0
-    kern_ms, launches = eng.timing_end()
-    value = args.steps * T * per_tour / dt
-    avg_launch_s = kern_ms / 1e3 / launches
+    launches = args.steps * T
+    value = launches * per_tour / dt
+    avg_launch_s = region_ms / 1e3 / launches
 
     lds_bytes = sum(mp[nb] * LDS_BYTES[nb] for nb in range(5))
     ach = lds_bytes / avg_launch_s / 1e9
-    # HBM algorithmic bytes per launch: the O(N) device ADS read once
-    # (xy, w int32; T, Q int32 path) plus the per-row records written
+    # compulsory HBM bytes per launch: the O(N) device ADS read (xy, w, T as
+    # int32 and the 2-opt G, H) plus the per-row records written
     rows = sum(wamca_mi.rows_of(nb, 0, N) for nb in range(5))
-    hbm_bytes = 4 * 4 * N + 8 * rows
-
-    # batched form: every tour in one persistent launch
-    for _ in range(2):
-        eng.eval(slots, 0x1F, 0, persistent=True)
-    eng.sync()
-    eng.timing_begin(16)
-    t1 = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        eng.eval(slots, 0x1F, 0, persistent=True)
-    eng.sync()
-    dtb = time.perf_counter() - t1
-    bk_ms, bl = eng.timing_end()
-    batched = reps * T * per_tour / dtb
+    hbm_alg = 4 * 5 * N + 8 * rows
 
     out = {
         "metric": "moves evaluated/s, n=1000 full-neighborhood sweep (one solution per launch)",
@@ -140,15 +161,34 @@ def main():
                    "n": n, "tours": T, "moves_per_tour": per_tour, "rowmap": "ADS",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "lds", "achieved": ach, "peak": PEAK_LDS_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_LDS_GBS, "traffic": None,
-                     "kernel": "k_eval", "avg_launch_us": avg_launch_s * 1e6,
+                     "frac": ach / PEAK_LDS_GBS, "traffic": traffic_per_launch(tile),
+                     "kernel": TILES[tile], "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": lds_bytes,
-                     "hbm_frac": hbm_bytes / avg_launch_s / 1e9 / PEAK_HBM_GBS},
-        "batched": {"moves_per_s": batched, "tours_per_launch": T,
-                    "avg_launch_ms": bk_ms / max(bl, 1),
-                    "lds_frac": lds_bytes * T / (bk_ms / max(bl, 1) / 1e3) / 1e9 / PEAK_LDS_GBS},
+                     "hbm_algorithmic_bytes_per_launch": hbm_alg,
+                     "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS},
+        "autotune_us_per_launch": tune,
         "device": info,
     }
+
+    if not args.no_batched:
+        # batched form: every tour in one launch, largest tiles
+        eng.set_tile(2)
+        best = None
+        for persistent in (False, True):
+            eng.eval(slots, 0x1F, 0, persistent=persistent)
+            eng.region_begin()
+            t1 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                eng.eval(slots, 0x1F, 0, persistent=persistent)
+            bms = eng.region_end() / reps
+            dtb = (time.perf_counter() - t1) / reps
+            rec = {"moves_per_s": T * per_tour / dtb, "tours_per_launch": T,
+                   "kernel": TILES[2], "persistent": persistent, "avg_launch_ms": bms,
+                   "lds_frac": lds_bytes * T / (bms / 1e3) / 1e9 / PEAK_LDS_GBS}
+            if best is None or rec["moves_per_s"] > best["moves_per_s"]:
+                best = rec
+        out["batched"] = best
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(xs, ys, tours)
     eng.close()

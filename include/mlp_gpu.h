@@ -168,8 +168,17 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
  * depend on the shape. */
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
 
-/* Timing of the last eval launch on the context stream (ms, HIP events). */
-int mlp_last_eval_ms(mlp_ctx* ctx, float* ms);
+/* A timed region on the context stream: a HIP event at begin and one at
+ * end; end synchronises and returns the elapsed milliseconds. */
+int mlp_region_begin(mlp_ctx* ctx);
+int mlp_region_end(mlp_ctx* ctx, float* ms);
+/* One eval launch per slot (the one-solution-per-launch form), enqueued from
+ * a host loop with nothing between the launches. */
+int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+                  int32_t rowmap);
+/* Device-to-device copies of `bytes` (reps times): a kernel of known traffic
+ * for calibrating memory counters. */
+int mlp_copy_probe(mlp_ctx* ctx, int64_t bytes, int32_t reps);
 /* Per-launch eval timing: between begin and end every mlp_eval launch is
  * bracketed by its own pair of HIP events (up to capacity launches); end
  * synchronises and returns the summed kernel time and the launch count. */

@@ -124,8 +124,8 @@ struct mlp_ctx {
   SlotDev* d_table = nullptr;
   int table_cap = 0;
   int* d_counter = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool eval_timed = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timed region
+  bool region_open = false;
   int32_t* h_stage = nullptr;  // pinned staging for tour uploads
   size_t h_stage_cap = 0;
   int64_t* d_i64 = nullptr;  // small device scratch for gathers
@@ -577,24 +577,19 @@ int mlp_get_latency(mlp_ctx* ctx, const int32_t* slots, int32_t count, int64_t* 
 
 namespace {
 
-// Records one eval launch between a pair of HIP events (timing ring or the
-// context's last-launch pair).
+// Launches one eval; between mlp_timing_begin and mlp_timing_end the launch
+// is bracketed by its own pair of HIP events.
 int timed_launch(mlp_ctx* ctx, hipError_t (*launch)(mlp_ctx*, void*), void* arg) {
-  if (ctx->capturing) {
+  if (ctx->capturing || !ctx->timing) {
     HCK(launch(ctx, arg));
     return MLP_OK;
   }
-  hipEvent_t e0 = ctx->ev0, e1 = ctx->ev1;
-  if (ctx->timing) {
-    if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
-    e0 = ctx->tev[2 * ctx->tcount];
-    e1 = ctx->tev[2 * ctx->tcount + 1];
-    ctx->tcount++;
-  }
+  if (ctx->tcount >= ctx->tev.size() / 2) return fail(ctx, MLP_ERANGE, "timing ring full");
+  hipEvent_t e0 = ctx->tev[2 * ctx->tcount], e1 = ctx->tev[2 * ctx->tcount + 1];
+  ctx->tcount++;
   HCK(hipEventRecord(e0, ctx->stream));
   HCK(launch(ctx, arg));
   HCK(hipEventRecord(e1, ctx->stream));
-  ctx->eval_timed = true;
   return MLP_OK;
 }
 
@@ -802,11 +797,51 @@ int mlp_timing_end(mlp_ctx* ctx, double* total_ms, int32_t* launches) {
   return MLP_OK;
 }
 
-int mlp_last_eval_ms(mlp_ctx* ctx, float* ms) {
+int mlp_region_begin(mlp_ctx* ctx) {
+  if (!ctx) return MLP_EINVAL;
+  HCK(hipEventRecord(ctx->ev0, ctx->stream));
+  ctx->region_open = true;
+  return MLP_OK;
+}
+
+int mlp_region_end(mlp_ctx* ctx, float* ms) {
   if (!ctx || !ms) return MLP_EINVAL;
-  if (!ctx->eval_timed) return fail(ctx, MLP_ESTATE, "no eval launched");
+  if (!ctx->region_open) return fail(ctx, MLP_ESTATE, "no timed region open");
+  HCK(hipEventRecord(ctx->ev1, ctx->stream));
   HCK(hipEventSynchronize(ctx->ev1));
   HCK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  ctx->region_open = false;
+  return MLP_OK;
+}
+
+int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+                  int32_t rowmap) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  for (int q = 0; q < count; ++q) {
+    st = eval_impl(ctx, slots + q, 1, nb_mask, rowmap, 0, 0, nullptr);
+    if (st) return st;
+  }
+  return MLP_OK;
+}
+
+int mlp_copy_probe(mlp_ctx* ctx, int64_t bytes, int32_t reps) {
+  if (!ctx || bytes <= 0 || reps <= 0) return MLP_EINVAL;
+  void *a = nullptr, *b = nullptr;
+  HCK(hipMalloc(&a, (size_t)bytes));
+  if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+    hipFree(a);
+    return fail(ctx, MLP_ENOMEM, "copy probe buffers");
+  }
+  hipError_t e = hipMemsetAsync(a, 1, (size_t)bytes, ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(ctx, MLP_EDEVICE, "copy probe: %s", hipGetErrorString(e));
   return MLP_OK;
 }
 

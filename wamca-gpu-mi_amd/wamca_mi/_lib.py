@@ -67,7 +67,10 @@ _SIGS = {
     "mlp_ads_materialize": (C.c_int, [C.c_void_p, C.c_int32, U32P, C.c_int64]),
     "mlp_dist_table": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, I32P]),
     "mlp_dist_list": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, I32P]),
-    "mlp_last_eval_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "mlp_region_begin": (C.c_int, [C.c_void_p]),
+    "mlp_region_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "mlp_eval_each": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_uint32, C.c_int32]),
+    "mlp_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
     "mlp_sqrt_selfcheck": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mlp_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_timing_begin": (C.c_int, [C.c_void_p, C.c_int32]),

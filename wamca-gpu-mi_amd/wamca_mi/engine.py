@@ -222,7 +222,21 @@ class Engine:
         self._ck(self._L.mlp_timing_end(self.h, C.byref(tot), C.byref(cnt)))
         return tot.value, cnt.value
 
-    def last_eval_ms(self) -> float:
+    def region_begin(self):
+        """Open a timed region on the context stream (HIP event)."""
+        self._ck(self._L.mlp_region_begin(self.h))
+
+    def region_end(self) -> float:
+        """Close the timed region; synchronises; elapsed milliseconds."""
         ms = C.c_float()
-        self._ck(self._L.mlp_last_eval_ms(self.h, C.byref(ms)))
+        self._ck(self._L.mlp_region_end(self.h, C.byref(ms)))
         return ms.value
+
+    def eval_each(self, slots, nb_mask=0x1F, rowmap=ROWMAP_ADS):
+        """One eval launch per slot, enqueued from a C loop."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_eval_each(self.h, slots, len(slots), nb_mask, rowmap))
+
+    def copy_probe(self, nbytes, reps=1):
+        """Device-to-device copies of nbytes (counter calibration)."""
+        self._ck(self._L.mlp_copy_probe(self.h, int(nbytes), int(reps)))
