@@ -62,11 +62,9 @@ def traffic_per_launch(tile):
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("kernel") == TILES[tile]:
-            return t.get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
-        pass
-    return None
+        return t["kernels"][TILES[tile]]["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def main():
