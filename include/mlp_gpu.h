@@ -75,11 +75,19 @@ int mlp_ctx_destroy(mlp_ctx* ctx);
 const char* mlp_ctx_error(const mlp_ctx* ctx);
 const char* mlp_strerror(int status);
 int mlp_sync(mlp_ctx* ctx);
+/* Device memory held by instances and slots (live) and reserved by the
+ * context's pool (released blocks are reused, not returned to HIP). */
+int mlp_pool_stats(mlp_ctx* ctx, int64_t* live_bytes, int64_t* reserved_bytes);
 int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* name, int32_t name_len);
 
 /* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5). */
 int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                         int32_t round, int32_t* inst);
+/* As mlp_instance_create, but the coordinate copies are only enqueued on the
+ * context stream: xs and ys must stay unchanged until the stream has passed
+ * them (any synchronising call, e.g. mlp_sync). */
+int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                              int32_t round, int32_t* inst);
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst);
 /* Device-computed client-indexed distance matrix (n*n int32) -- used to check
  * the device distance function against the host one. */

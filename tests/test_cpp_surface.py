@@ -48,7 +48,7 @@ def _declared_c_functions(path):
 def test_header_symbols_exported():
     lib = C.CDLL(wm.LIB_PATH)
     seen = {}
-    for h in ("mlp_gpu.h", "mlp_host.h", "dvnd.h"):
+    for h in ("mlp_gpu.h", "mlp_host.h", "dvnd.h", "mlp_serve.h"):
         names = _declared_c_functions(os.path.join(INCLUDE, h))
         assert names, h
         seen[h] = names

@@ -755,6 +755,16 @@ __global__ void k_weights(const int32_t* xs, const int32_t* ys, int n, int round
   }
 }
 
+__global__ void k_fill_i64(int64_t* p, int64_t count, int64_t v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+       t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = v;
+}
+
+__global__ void k_set_slot(SlotDev* table, int id, SlotDev d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) table[id] = d;
+}
+
 __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* ids, int count,
                               int word, int64_t* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1392,6 +1402,18 @@ hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* 
   const int64_t total = (int64_t)n * n;
   hipLaunchKernelGGL(k_weights, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                      st, xs, ys, n, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_fill_i64(hipStream_t st, int64_t* p, int64_t count, int64_t v) {
+  const int64_t blocks = std::min<int64_t>((count + kBlock - 1) / kBlock, 1024);
+  hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(kBlock), 0, st,
+                     p, count, v);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_set_slot(hipStream_t st, SlotDev* table, int id, const SlotDev& d) {
+  hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, st, table, id, d);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include <new>
 #include <string>
 #include <unordered_map>
+#include <map>
 #include <vector>
 
 #include "mlp_gpu.h"
@@ -25,8 +26,9 @@ namespace {
 struct Instance {
   bool alive = false;
   int n = 0, round = 0, maxd = 0;
-  int32_t* xs = nullptr;
+  int32_t* xs = nullptr;  // device, in one pool block with ys
   int32_t* ys = nullptr;
+  size_t cls = 0;         // pool class of the block
 };
 
 struct Slot {
@@ -34,6 +36,7 @@ struct Slot {
   int inst = -1;
   int N = 0;
   void* mem = nullptr;
+  size_t cls = 0;  // pool class of mem
   SlotDev dev{};
   int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
   int parity[MLP_NB_COUNT] = {0, 0, 0, 0, 0};  // row buffer of the last eval
@@ -134,6 +137,11 @@ struct mlp_ctx {
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
+  // device memory pool: released blocks stay on per-class free lists (no
+  // hipFree, so no implicit device synchronisation, while jobs come and go)
+  std::map<size_t, std::vector<void*>> pool_free;
+  std::vector<void*> pool_all;
+  size_t pool_reserved = 0, pool_live = 0;
   // DVND rounds: launches are enqueued `dvnd_chunk` rounds at a time between
   // convergence checks; with dvnd_graph the chunk is captured once per active
   // set and replayed as a HIP graph; dvnd_persistent drains work counters.
@@ -254,6 +262,41 @@ int grow_table(mlp_ctx* ctx, int need) {
   return MLP_OK;
 }
 
+size_t pool_class(size_t bytes) {
+  size_t c = 4096;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+int pool_alloc(mlp_ctx* ctx, size_t bytes, void** out, size_t* cls) {
+  const size_t c = pool_class(bytes);
+  auto it = ctx->pool_free.find(c);
+  if (it != ctx->pool_free.end() && !it->second.empty()) {
+    *out = it->second.back();
+    it->second.pop_back();
+  } else {
+    void* p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ctx, MLP_ENOMEM, "device allocation of %zu bytes failed", c);
+    }
+    ctx->pool_all.push_back(p);
+    ctx->pool_reserved += c;
+    *out = p;
+  }
+  ctx->pool_live += c;
+  *cls = c;
+  return MLP_OK;
+}
+
+// Stream-ordered release: work already queued on the context stream may still
+// use the block; a later allocation reuses it only for work queued after.
+void pool_release(mlp_ctx* ctx, void* p, size_t cls) {
+  if (!p) return;
+  ctx->pool_free[cls].push_back(p);
+  ctx->pool_live -= cls;
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,12 +351,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   if (!ctx) return MLP_EINVAL;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  for (auto& s : ctx->slots)
-    if (s.mem) hipFree(s.mem);
-  for (auto& in : ctx->insts) {
-    if (in.xs) hipFree(in.xs);
-    if (in.ys) hipFree(in.ys);
-  }
+  for (void* p : ctx->pool_all) hipFree(p);
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
@@ -335,6 +373,13 @@ int mlp_sync(mlp_ctx* ctx) {
   return MLP_OK;
 }
 
+int mlp_pool_stats(mlp_ctx* ctx, int64_t* live_bytes, int64_t* reserved_bytes) {
+  if (!ctx) return MLP_EINVAL;
+  if (live_bytes) *live_bytes = (int64_t)ctx->pool_live;
+  if (reserved_bytes) *reserved_bytes = (int64_t)ctx->pool_reserved;
+  return MLP_OK;
+}
+
 int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* name,
                     int32_t name_len) {
   if (!ctx) return MLP_EINVAL;
@@ -344,8 +389,8 @@ int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* n
   return MLP_OK;
 }
 
-int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
-                        int32_t round, int32_t* inst) {
+static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                           int32_t round, int32_t* inst, bool async) {
   if (!ctx || !inst) return MLP_EINVAL;
   if (!xs || !ys || n < 2) return fail(ctx, MLP_EINVAL, "need at least 2 clients");
   if (n + 1 > MLP_MAX_N) return fail(ctx, MLP_ERANGE, "n = %d exceeds %d", n, MLP_MAX_N - 1);
@@ -365,10 +410,19 @@ int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int3
   const double dx = xmax - xmin, dy = ymax - ymin;
   in.maxd = (int)std::ceil(std::sqrt(dx * dx + dy * dy)) + 1;
   HCK(hipSetDevice(ctx->device));
-  HCK(hipMalloc(&in.xs, 4 * (size_t)n));
-  HCK(hipMalloc(&in.ys, 4 * (size_t)n));
-  HCK(hipMemcpy(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice));
-  HCK(hipMemcpy(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice));
+  const size_t half = align_up(4 * (size_t)n, 256);
+  void* mem = nullptr;
+  int st = pool_alloc(ctx, 2 * half, &mem, &in.cls);
+  if (st) return st;
+  in.xs = (int32_t*)mem;
+  in.ys = (int32_t*)((char*)mem + half);
+  if (async) {
+    HCK(hipMemcpyAsync(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    HCK(hipMemcpyAsync(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    HCK(hipMemcpy(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice));
+    HCK(hipMemcpy(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice));
+  }
   in.alive = true;
   for (size_t q = 0; q < ctx->insts.size(); ++q)
     if (!ctx->insts[q].alive) {
@@ -381,6 +435,16 @@ int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int3
   return MLP_OK;
 }
 
+int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                        int32_t round, int32_t* inst) {
+  return instance_create(ctx, xs, ys, n, round, inst, false);
+}
+
+int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                              int32_t round, int32_t* inst) {
+  return instance_create(ctx, xs, ys, n, round, inst, true);
+}
+
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
   if (!ctx) return MLP_EINVAL;
   if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
@@ -388,10 +452,8 @@ int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
   for (auto& s : ctx->slots)
     if (s.alive && s.inst == inst)
       return fail(ctx, MLP_ESTATE, "instance %d still has slots", inst);
-  HCK(hipStreamSynchronize(ctx->stream));
   Instance& in = ctx->insts[inst];
-  hipFree(in.xs);
-  hipFree(in.ys);
+  pool_release(ctx, in.xs, in.cls);
   in = Instance();
   return MLP_OK;
 }
@@ -439,9 +501,9 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   Slot s;
   s.inst = inst;
   s.N = N;
-  if (hipMalloc(&s.mem, L.total) != hipSuccess)
-    return fail(ctx, MLP_ENOMEM, "slot allocation of %zu bytes failed", L.total);
-  HCK(hipMemset(s.mem, 0, L.total));
+  st = pool_alloc(ctx, L.total, &s.mem, &s.cls);
+  if (st) return st;
+  HCK(hipMemsetAsync(s.mem, 0, L.total, ctx->stream));
   char* b = (char*)s.mem;
   SlotDev& d = s.dev;
   d.N = N;
@@ -466,12 +528,9 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
     d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
     d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
   }
-  {
-    std::vector<int64_t> none(2 * (size_t)d.rowcap, MLP_KEY_NONE);
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
-      HCK(hipMemcpy(d.rows[nb], none.data(), 8 * none.size(), hipMemcpyHostToDevice));
-  }
-  HCK(hipMemcpy(ctx->d_table + id, &d, sizeof(SlotDev), hipMemcpyHostToDevice));
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    HCK(mlp_launch_fill_i64(ctx->stream, d.rows[nb], 2 * (int64_t)d.rowcap, MLP_KEY_NONE));
+  HCK(mlp_launch_set_slot(ctx->stream, ctx->d_table, id, d));
   s.alive = true;
   ctx->slots[id] = s;
   *slot = id;
@@ -481,8 +540,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
 int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot) {
   if (!ctx) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
-  HCK(hipStreamSynchronize(ctx->stream));
-  hipFree(ctx->slots[slot].mem);
+  pool_release(ctx, ctx->slots[slot].mem, ctx->slots[slot].cls);
   ctx->slots[slot] = Slot();
   return MLP_OK;
 }

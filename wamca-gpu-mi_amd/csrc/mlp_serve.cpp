@@ -1,0 +1,212 @@
+// mlp_serve.cpp -- continuous-batching DVND server (include/mlp_serve.h) over
+// the public C ABI of the engine.  Host-side bookkeeping only: admission by
+// device footprint, one chunk of DVND rounds per step over every resident
+// job, retirement and result hand-off.
+#include "mlp_serve.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <vector>
+
+namespace {
+
+struct Job {
+  int64_t id = 0, tag = 0;
+  int32_t n = 0, round = 0;
+  std::vector<int32_t> xs, ys, tour;
+  double t_submit = 0, t_admit = 0, t_finish = 0;
+  int32_t inst = -1, slot = -1;
+  int64_t bytes = 0, latency = 0;
+  int32_t rounds = 0;
+};
+
+// The pool rounds every block up to a power of two of at least 4 KiB.
+int64_t pow2_class(int64_t b) {
+  int64_t c = 4096;
+  while (c < b) c <<= 1;
+  return c;
+}
+
+}  // namespace
+
+struct mlp_server {
+  mlp_ctx* ctx = nullptr;
+  mlp_server_config cfg{};
+  std::deque<Job*> queued;
+  std::vector<Job*> active;
+  std::vector<Job*> done;    // finished, not yet polled
+  std::vector<Job*> handed;  // returned by the last poll
+  int64_t used = 0, next_id = 1;
+  bool chunk_pending = false;
+  std::chrono::steady_clock::time_point t0;
+};
+
+extern "C" {
+
+double mlp_server_now(const mlp_server* srv) {
+  if (!srv) return 0.0;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - srv->t0).count();
+}
+
+int64_t mlp_server_job_bytes(int32_t n) {
+  const int64_t slot = mlp_slot_footprint(n);
+  if (slot < 0) return -1;
+  const int64_t half = (4 * (int64_t)n + 255) / 256 * 256;
+  return pow2_class(slot) + pow2_class(2 * half);
+}
+
+int mlp_server_create(mlp_ctx* ctx, const mlp_server_config* cfg, mlp_server** out) {
+  if (!ctx || !out) return MLP_EINVAL;
+  mlp_server* s = new (std::nothrow) mlp_server();
+  if (!s) return MLP_ENOMEM;
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
+  if (s->cfg.rounds_per_step <= 0) s->cfg.rounds_per_step = 2;
+  if (s->cfg.rowmap != MLP_ROWMAP_ADS && s->cfg.rowmap != MLP_ROWMAP_ODC) {
+    delete s;
+    return MLP_EINVAL;
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  *out = s;
+  return MLP_OK;
+}
+
+static void release(mlp_server* s, Job* j) {
+  if (j->slot >= 0) mlp_slot_destroy(s->ctx, j->slot);
+  if (j->inst >= 0) mlp_instance_destroy(s->ctx, j->inst);
+  j->slot = j->inst = -1;
+}
+
+int mlp_server_destroy(mlp_server* s) {
+  if (!s) return MLP_EINVAL;
+  mlp_sync(s->ctx);
+  for (Job* j : s->active) {
+    release(s, j);
+    delete j;
+  }
+  for (Job* j : s->queued) delete j;
+  for (Job* j : s->done) delete j;
+  for (Job* j : s->handed) delete j;
+  delete s;
+  return MLP_OK;
+}
+
+int mlp_server_submit(mlp_server* s, const int32_t* xs, const int32_t* ys, int32_t n,
+                      int32_t round, const int32_t* tour, int64_t tag, int64_t* job) {
+  if (!s || !xs || !ys || !tour || n < 2) return MLP_EINVAL;
+  const int64_t bytes = mlp_server_job_bytes(n);
+  if (bytes < 0) return MLP_ERANGE;
+  if (bytes > s->cfg.hbm_budget) return MLP_ENOMEM;
+  Job* j = new (std::nothrow) Job();
+  if (!j) return MLP_ENOMEM;
+  j->id = s->next_id++;
+  j->tag = tag;
+  j->n = n;
+  j->round = round;
+  j->xs.assign(xs, xs + n);
+  j->ys.assign(ys, ys + n);
+  j->tour.assign(tour, tour + n + 1);
+  j->bytes = bytes;
+  j->t_submit = mlp_server_now(s);
+  s->queued.push_back(j);
+  if (job) *job = j->id;
+  return MLP_OK;
+}
+
+int mlp_server_step(mlp_server* s, int32_t* active_out, int32_t* queued_out) {
+  if (!s) return MLP_EINVAL;
+  int st = MLP_OK;
+  // 1. retire the jobs the previous chunk finished
+  if (s->chunk_pending && !s->active.empty()) {
+    const int c = (int)s->active.size();
+    std::vector<int32_t> slots((size_t)c), act((size_t)c), rnd((size_t)c);
+    for (int q = 0; q < c; ++q) slots[(size_t)q] = s->active[(size_t)q]->slot;
+    st = mlp_dvnd_status(s->ctx, slots.data(), c, act.data(), rnd.data());
+    if (st) return st;
+    const double now = mlp_server_now(s);
+    std::vector<Job*> still;
+    for (int q = 0; q < c; ++q) {
+      Job* j = s->active[(size_t)q];
+      if (act[(size_t)q]) {
+        still.push_back(j);
+        continue;
+      }
+      j->rounds = rnd[(size_t)q];
+      st = mlp_get_tours(s->ctx, &j->slot, 1, j->tour.data());
+      if (st == MLP_OK) st = mlp_get_latency(s->ctx, &j->slot, 1, &j->latency);
+      if (st) return st;
+      j->t_finish = now;
+      release(s, j);
+      s->used -= j->bytes;
+      s->done.push_back(j);
+    }
+    s->active.swap(still);
+  }
+  s->chunk_pending = false;
+  // 2. admission in arrival order while the footprint fits
+  std::vector<Job*> admitted;
+  while (!s->queued.empty()) {
+    Job* j = s->queued.front();
+    if (s->cfg.max_active > 0 && (int)(s->active.size() + admitted.size()) >= s->cfg.max_active)
+      break;
+    if (s->used + j->bytes > s->cfg.hbm_budget) break;
+    st = mlp_instance_create_async(s->ctx, j->xs.data(), j->ys.data(), j->n, j->round, &j->inst);
+    if (st == MLP_OK) st = mlp_slot_create(s->ctx, j->inst, &j->slot);
+    if (st) {
+      release(s, j);
+      return st;
+    }
+    s->queued.pop_front();
+    s->used += j->bytes;
+    admitted.push_back(j);
+  }
+  if (!admitted.empty()) {
+    std::vector<int32_t> slots, tours;
+    for (Job* j : admitted) {
+      slots.push_back(j->slot);
+      tours.insert(tours.end(), j->tour.begin(), j->tour.end());
+    }
+    st = mlp_set_tours(s->ctx, slots.data(), (int32_t)slots.size(), tours.data());
+    if (st == MLP_OK) st = mlp_dvnd_start(s->ctx, slots.data(), (int32_t)slots.size());
+    if (st) return st;
+    const double now = mlp_server_now(s);
+    for (Job* j : admitted) {
+      j->t_admit = now;
+      s->active.push_back(j);
+    }
+  }
+  // 3. one chunk of rounds over every resident job
+  if (!s->active.empty()) {
+    std::vector<int32_t> slots;
+    for (Job* j : s->active) slots.push_back(j->slot);
+    st = mlp_dvnd_rounds(s->ctx, slots.data(), (int32_t)slots.size(), s->cfg.rowmap,
+                         s->cfg.rounds_per_step, s->cfg.max_rounds);
+    if (st) return st;
+    s->chunk_pending = true;
+  }
+  if (active_out) *active_out = (int32_t)s->active.size();
+  if (queued_out) *queued_out = (int32_t)s->queued.size();
+  return MLP_OK;
+}
+
+int mlp_server_poll(mlp_server* s, mlp_job_result* out, int32_t cap, int32_t* count) {
+  if (!s || !count || (cap > 0 && !out)) return MLP_EINVAL;
+  for (Job* j : s->handed) delete j;
+  s->handed.clear();
+  const int take = std::min<int>(cap, (int)s->done.size());
+  for (int q = 0; q < take; ++q) {
+    Job* j = s->done[(size_t)q];
+    out[q] = mlp_job_result{j->id, j->tag, j->latency, j->rounds, j->n,
+                            j->t_submit, j->t_admit, j->t_finish, j->tour.data()};
+    s->handed.push_back(j);
+  }
+  s->done.erase(s->done.begin(), s->done.begin() + take);
+  *count = take;
+  return MLP_OK;
+}
+
+}  // extern "C"
