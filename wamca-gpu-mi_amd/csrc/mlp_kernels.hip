@@ -304,77 +304,113 @@ __device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t*
 }
 
 // ---------------------------------------------------------------------------
-// ADS prefix build for one slot by one workgroup.  tour may live in LDS.
+// ADS prefix build for one slot by one workgroup (tour may live in LDS).
+// Positions go in chunks of kBlock (thread t owns position base + t); the
+// coordinate gathers of up to kPrepBatch chunks are issued before the first
+// wait; per chunk a block-wide scan turns w and p*w into T and Q.
 // ---------------------------------------------------------------------------
-__device__ void prep_wg(const SlotDev& s, const int32_t* tour, int64_t* sh /* >= 2*kBlock+4 */) {
-  const int N = s.N, tid = threadIdx.x;
+constexpr int kPrepBatch = 8;
+
+__device__ __forceinline__ int64_t wave_scan_incl(int64_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int64_t o = __shfl_up((long long)v, off, kWave);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor((long long)v, off, kWave);
+  return v;
+}
+
+struct PrepLds {
+  int32_t xy[kBlock + 2];  // xy of positions base-1 .. base+kBlock (offset 1)
+  int64_t wt[kBlock / kWave][2];
+  int64_t carry[2];        // T, Q before this chunk
+  int64_t lsum[kBlock / kWave];
+};
+
+__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
-  const int per = (N + kBlock - 1) / kBlock;
-  const int p0 = tid * per, p1 = min(N, p0 + per);
-  int64_t tsum = 0, qsum = 0;
-  for (int p = p0; p < p1; ++p) {
-    const int c = tour[p];
-    const int xy = (s.xs[c] << 16) | s.ys[c];
-    int w = 0;
-    if (p > 0) {
-      const int cp = tour[p - 1];
-      w = dist_xy((s.xs[cp] << 16) | s.ys[cp], xy, dr);
-    }
-    s.xy[p] = xy;
-    s.w[p] = w;
-    tsum += w;
-    qsum += (int64_t)p * w;
-  }
-  sh[tid] = tsum;
-  sh[kBlock + tid] = qsum;
-  __syncthreads();
-  if (tid == 0) {  // exclusive scan of the per-thread totals
-    int64_t at = 0, aq = 0;
-    for (int t = 0; t < kBlock; ++t) {
-      const int64_t vt = sh[t], vq = sh[kBlock + t];
-      sh[t] = at;
-      sh[kBlock + t] = aq;
-      at += vt;
-      aq += vq;
-    }
-    sh[2 * kBlock] = at;  // T[N-1]
-  }
-  __syncthreads();
-  int64_t T = sh[tid], Q = sh[kBlock + tid], lsum = 0;
-  for (int p = p0; p < p1; ++p) {
-    const int c = tour[p];
-    int w = 0;
-    if (p > 0) {
-      const int cp = tour[p - 1];
-      w = dist_xy((s.xs[cp] << 16) | s.ys[cp], (s.xs[c] << 16) | s.ys[c], dr);
-    }
-    T += w;
-    Q += (int64_t)p * w;
-    s.t64[p] = T;
-    s.q64[p] = Q;
-    s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
-    s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
-    lsum += T;  // L = sum_p T[p]
-    // 2-opt per-position constants (mod 2^32)
-    int wn = 0;
-    if (p + 1 < N) {
-      const int cn = tour[p + 1];
-      wn = dist_xy((s.xs[c] << 16) | s.ys[c], (s.xs[cn] << 16) | s.ys[cn], dr);
-    }
-    const uint32_t Z = 2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
-    s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w - Z);
-    s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)wn);
-  }
-  __syncthreads();
-  sh[tid] = lsum;
-  __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (tid < off) sh[tid] += sh[tid + off];
-    __syncthreads();
-  }
+  int64_t lacc = 0;  // this thread's share of L = sum_p T[p]
   if (tid == 0) {
-    const int64_t L = sh[0];
-    const int64_t Ttot = sh[2 * kBlock];
+    P.carry[0] = 0;
+    P.carry[1] = 0;
+    P.xy[0] = 0;
+  }
+  for (int b0 = 0; b0 < N; b0 += kBlock * kPrepBatch) {
+    // gathers of the batch: chunk k covers positions b0 + k*kBlock + [0, kBlock)
+    int xyv[kPrepBatch], xyn = 0;
+#pragma unroll
+    for (int k = 0; k < kPrepBatch; ++k) {
+      const int p = min(b0 + k * kBlock + tid, N - 1);
+      const int c = tour[p];
+      xyv[k] = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
+    }
+    if (tid == 0) {  // the first position after the batch
+      const int c = tour[min(b0 + kPrepBatch * kBlock, N - 1)];
+      xyn = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
+    }
+#pragma unroll
+    for (int k = 0; k < kPrepBatch; ++k) {
+      const int base = b0 + k * kBlock;
+      if (base >= N) break;
+      const int p = base + tid;
+      const bool valid = p < N;
+      P.xy[tid + 1] = xyv[k];
+      // the next chunk's first position: thread 0's next register of the batch
+      if (tid == 0) P.xy[kBlock + 1] = k + 1 < kPrepBatch ? xyv[k + 1] : xyn;
+      __syncthreads();
+      const int xy = xyv[k];
+      const int w = (valid && p > 0) ? dist_xy(P.xy[tid], xy, dr) : 0;
+      const int wn = (valid && p + 1 < N) ? dist_xy(xy, P.xy[tid + 2], dr) : 0;
+      int64_t vt = wave_scan_incl((int64_t)w, lane);
+      int64_t vq = wave_scan_incl((int64_t)p * w, lane);
+      if (lane == kWave - 1) {
+        P.wt[wave][0] = vt;
+        P.wt[wave][1] = vq;
+      }
+      __syncthreads();
+      int64_t T = P.carry[0] + vt, Q = P.carry[1] + vq;
+      for (int q = 0; q < wave; ++q) {
+        T += P.wt[q][0];
+        Q += P.wt[q][1];
+      }
+      if (valid) {
+        s.xy[p] = xy;
+        s.w[p] = w;
+        s.t64[p] = T;
+        s.q64[p] = Q;
+        s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
+        s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
+        const uint32_t Z =
+            2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
+        s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w - Z);
+        s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)wn);
+        lacc += T;
+      }
+      __syncthreads();  // every thread has read carry, wt and xy
+      if (tid == kBlock - 1) {
+        P.carry[0] = T;  // T, Q of the chunk's last position (base + kBlock - 1)
+        P.carry[1] = Q;
+        P.xy[0] = xy;
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t lw = wave_sum64(lacc);
+  if (lane == 0) P.lsum[wave] = lw;
+  __syncthreads();
+  if (tid == 0) {
+    int64_t L = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) L += P.lsum[q];
+    // T of the last position: the carry after the last chunk (positions past
+    // N - 1 add w = 0)
+    const int64_t Ttot = P.carry[0];
     // every |delta| <= N (Ttot + 6 maxd) + 3 Ttot (see the delta forms)
     const int64_t bound = (int64_t)N * (Ttot + 6 * (int64_t)s.maxd) + 3 * Ttot;
     s.info[MLP_INFO_LATENCY] = L;
@@ -392,10 +428,10 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, int64_t* sh /* >=
 
 __global__ __launch_bounds__(kBlock) void k_prep(const SlotDev* __restrict__ table,
                                                  SlotListInline list, const int32_t* list_dev) {
-  __shared__ int64_t sh[2 * kBlock + 4];
+  __shared__ PrepLds P;
   const int id = list_at(list, list_dev, blockIdx.x);
   const SlotDev& s = table[id];
-  prep_wg(s, s.tour, sh);
+  prep_wg(s, s.tour, P);
   if (threadIdx.x == 0) {
     s.info[MLP_INFO_ROUND_MOVES] = 0;
     s.info[MLP_INFO_TOTAL_MOVES] = 0;
@@ -505,6 +541,34 @@ __device__ __forceinline__ void bits_set(uint32_t* bits, int a, int b) {
   }
 }
 
+// Positions a kept move rewrites, and their new sources (mlkswap.cu:1066-1106,
+// mlk2opt.cu:725-768, mlkoropt.cu:886-950 restated as a position map).
+constexpr int kShortMove = 64;
+constexpr int kLongMoves = 256;
+static_assert(MLP_MAX_N / (kShortMove + 1) < kLongMoves, "disjoint long spans fit the list");
+
+__device__ __forceinline__ int move_span(int nb, int k, int i, int j) {
+  if (nb == 0) return 2;
+  if (nb == 1) return j - i + 1;
+  return i < j ? j + k - i : i + k - j;
+}
+
+__device__ __forceinline__ void apply_span(int nb, int k, int i, int j, const int32_t* told,
+                                           int32_t* tnew, int start, int stride) {
+  if (nb == 0) {
+    if (start == 0) {
+      tnew[i] = told[j];
+      tnew[j] = told[i];
+    }
+  } else if (nb == 1) {
+    for (int p = i + start; p <= j; p += stride) tnew[p] = told[i + j - p];
+  } else if (i < j) {
+    for (int p = i + start; p < j + k; p += stride) tnew[p] = p < j ? told[p + k] : told[i + p - j];
+  } else {
+    for (int p = j + start; p < i + k; p += stride) tnew[p] = p < j + k ? told[i + p - j] : told[p - k];
+  }
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
@@ -520,8 +584,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
                                                         int rowmap, int pcap, int ncap,
                                                         int skip_inactive) {
   extern __shared__ int64_t lds[];
-  __shared__ int s_m, s_kept;
-  __shared__ int64_t sh[2 * kBlock + 4];
+  __shared__ int s_m, s_kept, s_long;
+  __shared__ int s_longq[kLongMoves];
+  __shared__ PrepLds PL;
   const int id = list_at(list, list_dev, blockIdx.x);
   const SlotDev& s = table[id];
   if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
@@ -607,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
       }
       uint64_t cm = 0;
       for (int q = 0; q < kWave; ++q) {
-        const int iq = __shfl(i, q, kWave), jq = __shfl(j, q, kWave);
+        const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
         if (q < lane && conflicts(nb, k, iq, jq, i, j)) cm |= 1ull << q;
       }
       const uint64_t pre_mask = __ballot(pre);
@@ -641,32 +706,35 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   }
   __syncthreads();
   const int K = s_kept;
-  // 4. apply the kept moves (disjoint regions) through a position -> source map
+  // 4. apply the kept moves (disjoint regions) through a position -> source
+  // map: a move with a short region is applied by one thread, the few long
+  // ones (regions are disjoint, so at most N / kShortMove) by all threads.
   for (int p = tid; p < N; p += kBlock) {
     const int c = s.tour[p];
     told[p] = c;
     tnew[p] = c;
   }
+  if (tid == 0) s_long = 0;
   __syncthreads();
-  for (int q = 0; q < K; ++q) {
+  for (int q = tid; q < K; q += kBlock) {
     const int64_t v = keys[q];
     const int i = (int)(v >> 32), j = (int)(uint32_t)v;
-    if (nb == 0) {
-      if (tid == 0) {
-        tnew[i] = told[j];
-        tnew[j] = told[i];
-      }
-    } else if (nb == 1) {
-      for (int p = i + tid; p <= j; p += kBlock) tnew[p] = told[i + j - p];
-    } else if (i < j) {
-      for (int p = i + tid; p < j + k; p += kBlock) tnew[p] = p < j ? told[p + k] : told[i + p - j];
+    if (move_span(nb, k, i, j) <= kShortMove) {
+      apply_span(nb, k, i, j, told, tnew, 0, 1);
     } else {
-      for (int p = j + tid; p < i + k; p += kBlock) tnew[p] = p < j + k ? told[i + p - j] : told[p - k];
+      const int at = atomicAdd(&s_long, 1);
+      if (at < kLongMoves) s_longq[at] = q;
     }
   }
   __syncthreads();
+  const int nl = min(s_long, kLongMoves);
+  for (int t = 0; t < nl; ++t) {
+    const int64_t v = keys[s_longq[t]];
+    apply_span(nb, k, (int)(v >> 32), (int)(uint32_t)v, told, tnew, tid, kBlock);
+  }
+  __syncthreads();
   for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
-  prep_wg(s, tnew, sh);
+  prep_wg(s, tnew, PL);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
     s.info[MLP_INFO_ROUND_MOVES] += K;
