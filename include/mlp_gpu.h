@@ -175,6 +175,18 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
  * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024.  Results do not
  * depend on the shape. */
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
+/* Autotuner of the sweep tile shape: times every shape on an evaluation of
+ * nb_mask over the slots (reps launches each, after a warm-up; the slots' row
+ * buffers are overwritten), records the fastest for (nb_mask, log2 of the
+ * launch's move count) in the context's table, which the automatic shape
+ * choice consults (nearest tuned size of the same mask).  us_per_launch, if
+ * given, receives 3 timings (one per shape). */
+int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+                 int32_t reps, int32_t* best_tile, float* us_per_launch);
+/* The table: 32 masks x 48 log2-size buckets of int8 (-1 = untuned). */
+#define MLP_TUNE_TABLE_BYTES (32 * 48)
+int mlp_tune_get(mlp_ctx* ctx, int8_t* table);
+int mlp_tune_set(mlp_ctx* ctx, const int8_t* table);
 
 /* A timed region on the context stream: a HIP event at begin and one at
  * end; end synchronises and returns the elapsed milliseconds. */

@@ -19,6 +19,7 @@
 // square root (mlproblem.cpp:159-162).
 #include <hip/hip_runtime.h>
 
+#include "mlp_gpu.h"
 #include "mlp_internal.h"
 
 namespace {
@@ -305,11 +306,12 @@ __device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t*
 
 // ---------------------------------------------------------------------------
 // ADS prefix build for one slot by one workgroup (tour may live in LDS).
-// Positions go in chunks of kBlock (thread t owns position base + t); the
-// coordinate gathers of up to kPrepBatch chunks are issued before the first
-// wait; per chunk a block-wide scan turns w and p*w into T and Q.
+// Segments of kBlock * kPrepPer positions: thread t owns kPrepPer
+// consecutive positions, gathers their coordinates (all loads issued before
+// the first wait), forms w, T and Q prefixes in registers, and one
+// block-wide exclusive scan of the per-thread totals places them.
 // ---------------------------------------------------------------------------
-constexpr int kPrepBatch = 8;
+constexpr int kPrepPer = 8;
 
 __device__ __forceinline__ int64_t wave_scan_incl(int64_t v, int lane) {
 #pragma unroll
@@ -327,9 +329,8 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 }
 
 struct PrepLds {
-  int32_t xy[kBlock + 2];  // xy of positions base-1 .. base+kBlock (offset 1)
   int64_t wt[kBlock / kWave][2];
-  int64_t carry[2];        // T, Q before this chunk
+  int64_t carry[2];  // T, Q before this segment
   int64_t lsum[kBlock / kWave];
 };
 
@@ -340,67 +341,71 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
   if (tid == 0) {
     P.carry[0] = 0;
     P.carry[1] = 0;
-    P.xy[0] = 0;
   }
-  for (int b0 = 0; b0 < N; b0 += kBlock * kPrepBatch) {
-    // gathers of the batch: chunk k covers positions b0 + k*kBlock + [0, kBlock)
-    int xyv[kPrepBatch], xyn = 0;
+  for (int base = 0; base < N; base += kBlock * kPrepPer) {
+    const int p0 = base + tid * kPrepPer;
+    // xy of positions p0 - 1 .. p0 + kPrepPer (clamped into the tour)
+    int xy[kPrepPer + 2];
 #pragma unroll
-    for (int k = 0; k < kPrepBatch; ++k) {
-      const int p = min(b0 + k * kBlock + tid, N - 1);
-      const int c = tour[p];
-      xyv[k] = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
+    for (int e = 0; e < kPrepPer + 2; ++e) {
+      const int c = tour[min(max(p0 - 1 + e, 0), N - 1)];
+      xy[e] = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
     }
-    if (tid == 0) {  // the first position after the batch
-      const int c = tour[min(b0 + kPrepBatch * kBlock, N - 1)];
-      xyn = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
+    int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
+#pragma unroll
+    for (int e = 0; e <= kPrepPer; ++e) {
+      const int p = p0 + e;
+      w[e] = (p >= 1 && p < N) ? dist_xy(xy[e], xy[e + 1], dr) : 0;
+    }
+    int64_t lt[kPrepPer], lq[kPrepPer], at = 0, aq = 0;
+#pragma unroll
+    for (int e = 0; e < kPrepPer; ++e) {
+      at += w[e];
+      aq += (int64_t)(p0 + e) * w[e];
+      lt[e] = at;
+      lq[e] = aq;
+    }
+    const int64_t it = wave_scan_incl(at, lane), iq = wave_scan_incl(aq, lane);
+    if (lane == kWave - 1) {
+      P.wt[wave][0] = it;
+      P.wt[wave][1] = iq;
+    }
+    __syncthreads();
+    int64_t offT = P.carry[0] + it - at, offQ = P.carry[1] + iq - aq;
+    int64_t segT = 0, segQ = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; ++q) {
+      if (q < wave) {
+        offT += P.wt[q][0];
+        offQ += P.wt[q][1];
+      }
+      segT += P.wt[q][0];
+      segQ += P.wt[q][1];
     }
 #pragma unroll
-    for (int k = 0; k < kPrepBatch; ++k) {
-      const int base = b0 + k * kBlock;
-      if (base >= N) break;
-      const int p = base + tid;
-      const bool valid = p < N;
-      P.xy[tid + 1] = xyv[k];
-      // the next chunk's first position: thread 0's next register of the batch
-      if (tid == 0) P.xy[kBlock + 1] = k + 1 < kPrepBatch ? xyv[k + 1] : xyn;
-      __syncthreads();
-      const int xy = xyv[k];
-      const int w = (valid && p > 0) ? dist_xy(P.xy[tid], xy, dr) : 0;
-      const int wn = (valid && p + 1 < N) ? dist_xy(xy, P.xy[tid + 2], dr) : 0;
-      int64_t vt = wave_scan_incl((int64_t)w, lane);
-      int64_t vq = wave_scan_incl((int64_t)p * w, lane);
-      if (lane == kWave - 1) {
-        P.wt[wave][0] = vt;
-        P.wt[wave][1] = vq;
-      }
-      __syncthreads();
-      int64_t T = P.carry[0] + vt, Q = P.carry[1] + vq;
-      for (int q = 0; q < wave; ++q) {
-        T += P.wt[q][0];
-        Q += P.wt[q][1];
-      }
-      if (valid) {
-        s.xy[p] = xy;
-        s.w[p] = w;
+    for (int e = 0; e < kPrepPer; ++e) {
+      const int p = p0 + e;
+      if (p < N) {
+        const int64_t T = offT + lt[e], Q = offQ + lq[e];
+        s.xy[p] = xy[e + 1];
+        s.w[p] = w[e];
         s.t64[p] = T;
         s.q64[p] = Q;
         s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
         s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
         const uint32_t Z =
             2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
-        s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w - Z);
-        s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)wn);
+        s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w[e] - Z);
+        s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)w[e + 1]);
         lacc += T;
       }
-      __syncthreads();  // every thread has read carry, wt and xy
-      if (tid == kBlock - 1) {
-        P.carry[0] = T;  // T, Q of the chunk's last position (base + kBlock - 1)
-        P.carry[1] = Q;
-        P.xy[0] = xy;
-      }
-      __syncthreads();
     }
+    __syncthreads();  // every thread has read carry and wt
+    if (tid == 0) {
+      P.carry[0] += segT;
+      P.carry[1] += segQ;
+    }
+    __syncthreads();
   }
   const int64_t lw = wave_sum64(lacc);
   if (lane == 0) P.lsum[wave] = lw;
@@ -408,9 +413,7 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
   if (tid == 0) {
     int64_t L = 0;
     for (int q = 0; q < kBlock / kWave; ++q) L += P.lsum[q];
-    // T of the last position: the carry after the last chunk (positions past
-    // N - 1 add w = 0)
-    const int64_t Ttot = P.carry[0];
+    const int64_t Ttot = P.carry[0];  // T[N-1]: positions past N-1 add nothing
     // every |delta| <= N (Ttot + 6 maxd) + 3 Ttot (see the delta forms)
     const int64_t bound = (int64_t)N * (Ttot + 6 * (int64_t)s.maxd) + 3 * Ttot;
     s.info[MLP_INFO_LATENCY] = L;

@@ -44,6 +44,8 @@ struct Slot {
   bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
 };
 
+#define MLP_TUNE_BUCKETS 48
+
 struct DevList {
   void* p = nullptr;
 };
@@ -137,6 +139,7 @@ struct mlp_ctx {
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
+  int8_t tune[32][MLP_TUNE_BUCKETS];  // best shape per (mask, log2 moves), -1 untuned
   // device memory pool: released blocks stay on per-class free lists (no
   // hipFree, so no implicit device synchronisation, while jobs come and go)
   std::map<size_t, std::vector<void*>> pool_free;
@@ -323,6 +326,7 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
   if (device < 0 || device >= ndev) return MLP_EINVAL;
   mlp_ctx* ctx = new (std::nothrow) mlp_ctx();
   if (!ctx) return MLP_ENOMEM;
+  memset(ctx->tune, -1, sizeof ctx->tune);
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
@@ -669,9 +673,36 @@ hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
 // Resident workgroups per CU of each sweep tile shape (persistent grids).
 const int kTileWgPerCu[MLP_TILE_COUNT] = {5, 4, 3};
 
-// Shape for a launch of `count` tours: small tiles give a single tour enough
-// workgroups to fill the chip; large tiles evaluate fewer distances per move.
-int auto_tile(const mlp_ctx*, int count) {
+// Moves one evaluation of `mask` over the slots covers (ADS row map).
+int64_t launch_moves(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
+  int64_t m = 0;
+  for (int q = 0; q < count; ++q) {
+    const int64_t N = ctx->slots[slots[q]].N;
+    if (mask & 1u) m += std::max<int64_t>(0, (N - 4) * (N - 3) / 2);
+    if (mask & 2u) m += std::max<int64_t>(0, ((N + 1) / 2) * (N - 3));
+    for (int k = 1; k <= 3; ++k)
+      if (mask & (1u << (k + 1))) m += std::max<int64_t>(0, (N - k - 2) * (N - k - 1));
+  }
+  return m;
+}
+
+int work_bucket(int64_t moves) {
+  int b = 0;
+  while (b < MLP_TUNE_BUCKETS - 1 && (2LL << b) <= moves) ++b;
+  return b;
+}
+
+// Shape for a launch: the autotuned table entry of (mask, work bucket) or the
+// nearest tuned bucket of that mask; untuned masks use the launch size (small
+// tiles give a small launch enough workgroups, large tiles evaluate fewer
+// distances per move).
+int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
+  const int b = work_bucket(launch_moves(ctx, slots, count, mask));
+  const int8_t* row = ctx->tune[mask & 0x1fu];
+  for (int d = 0; d < MLP_TUNE_BUCKETS; ++d) {
+    if (b - d >= 0 && row[b - d] >= 0) return row[b - d];
+    if (b + d < MLP_TUNE_BUCKETS && row[b + d] >= 0) return row[b + d];
+  }
   if (count <= 2) return MLP_TILE_4x256;
   if (count <= 16) return MLP_TILE_8x512;
   return MLP_TILE_16x1024;
@@ -713,7 +744,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
   if (rowmap == 0) {
     // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
     int tile = ctx->tile_override;
-    if (tile < 0) tile = auto_tile(ctx, count);
+    if (tile < 0) tile = auto_tile(ctx, slots, count, nb_mask);
     int TR, TC;
     mlp_tile_shape(tile, &TR, &TC);
     std::vector<SweepSeg> segs;
@@ -816,6 +847,64 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
   if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
   return eval_impl(ctx, slots, count, nb_mask, rowmap, persistent, 0, nullptr);
+}
+
+int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
+                 int32_t reps, int32_t* best_tile, float* us_per_launch) {
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (nb_mask == 0 || (nb_mask & ~0x1fu)) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  if (count == 0 || reps <= 0) return fail(ctx, MLP_EINVAL, "nothing to tune");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HCK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail(ctx, MLP_EDEVICE, "event");
+  }
+  const int saved = ctx->tile_override;
+  float best = 0;
+  int bt = -1;
+  for (int t = 0; t < MLP_TILE_COUNT && st == MLP_OK; ++t) {
+    ctx->tile_override = t;
+    st = eval_impl(ctx, slots, count, nb_mask, 0, 0, 0, nullptr);  // warm-up
+    if (st) break;
+    hipError_t e = hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps && st == MLP_OK && e == hipSuccess; ++r)
+      st = eval_impl(ctx, slots, count, nb_mask, 0, 0, 0, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) st = fail(ctx, MLP_EDEVICE, "autotune: %s", hipGetErrorString(e));
+    const float us = ms * 1e3f / (float)reps;
+    if (us_per_launch) us_per_launch[t] = us;
+    if (bt < 0 || us < best) {
+      best = us;
+      bt = t;
+    }
+  }
+  ctx->tile_override = saved;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st) return st;
+  ctx->tune[nb_mask][work_bucket(launch_moves(ctx, slots, count, nb_mask))] = (int8_t)bt;
+  if (best_tile) *best_tile = bt;
+  return MLP_OK;
+}
+
+int mlp_tune_get(mlp_ctx* ctx, int8_t* table) {
+  if (!ctx || !table) return MLP_EINVAL;
+  memcpy(table, ctx->tune, sizeof ctx->tune);
+  return MLP_OK;
+}
+
+int mlp_tune_set(mlp_ctx* ctx, const int8_t* table) {
+  if (!ctx || !table) return MLP_EINVAL;
+  for (size_t q = 0; q < sizeof ctx->tune; ++q)
+    if (table[q] < -1 || table[q] >= MLP_TILE_COUNT)
+      return fail(ctx, MLP_EINVAL, "tune entry %zu = %d is not a tile shape", q, (int)table[q]);
+  memcpy(ctx->tune, table, sizeof ctx->tune);
+  return MLP_OK;
 }
 
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
@@ -1067,8 +1156,7 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
     }
     return MLP_OK;
   }
-  int tile = ctx->tile_override;
-  if (tile < 0) tile = auto_tile(ctx, count);
+  int tile = ctx->tile_override;  // (the per-neighborhood shapes follow from the slots)
   // the captured launches embed the slot table address, every slot's id and
   // size (segment tables, merge capacities) and the settings
   const int32_t hdr[6] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count};

@@ -13,6 +13,7 @@ import numpy as np
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libwamca_mi.so")
 
+I8P = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 I64P = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -97,6 +98,10 @@ _SIGS = {
     "mlp_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
     "mlp_sqrt_selfcheck": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mlp_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_autotune": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_uint32, C.c_int32,
+                               C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
+    "mlp_tune_get": (C.c_int, [C.c_void_p, I8P]),
+    "mlp_tune_set": (C.c_int, [C.c_void_p, I8P]),
     "mlp_timing_begin": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_timing_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     # host-side inputs (include/mlp_host.h)

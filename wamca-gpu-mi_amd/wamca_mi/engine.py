@@ -26,7 +26,9 @@ def rows_of(nb: int, rowmap: int, N: int) -> int:
 
 
 class Engine:
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuned: bool = True):
+        """Context on one GPU.  tuned: load the committed tile-shape table of
+        this architecture (wamca_mi/tune_<arch>.json) when there is one."""
         self._L = lib()
         h = C.c_void_p()
         st = self._L.mlp_ctx_create(device, C.byref(h))
@@ -34,6 +36,11 @@ class Engine:
             raise MLPError(st, self._L.mlp_strerror(st).decode())
         self.h = h
         self.N = {}  # slot -> N
+        if tuned:
+            from .tune import load_table
+            t = load_table(self.device_info()["arch"])
+            if t is not None:
+                self.set_tune_table(t)
 
     def close(self):
         if self.h:
@@ -211,6 +218,24 @@ class Engine:
     def set_tile(self, tile: int):
         """Sweep tile shape: -1 automatic, 0 = 4x256, 1 = 8x512, 2 = 16x1024."""
         self._ck(self._L.mlp_set_tile(self.h, tile))
+
+    def autotune(self, slots, nb_mask=0x1F, reps=3):
+        """Time every sweep tile shape on this launch; record the best in the
+        context's table.  Returns (best shape, [us per launch per shape])."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        best = C.c_int32()
+        us = (C.c_float * 3)()
+        self._ck(self._L.mlp_autotune(self.h, slots, len(slots), nb_mask, reps, C.byref(best), us))
+        return best.value, list(us)
+
+    def tune_table(self) -> np.ndarray:
+        t = np.empty((32, 48), np.int8)
+        self._ck(self._L.mlp_tune_get(self.h, t))
+        return t
+
+    def set_tune_table(self, t):
+        t = np.ascontiguousarray(t, np.int8).reshape(32, 48)
+        self._ck(self._L.mlp_tune_set(self.h, t))
 
     def timing_begin(self, capacity=8192):
         self._ck(self._L.mlp_timing_begin(self.h, capacity))
