@@ -514,15 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ tab
 
 namespace {
 
-// Conflict predicates of one neighborhood (mlkernel.cu:51-71).
-__device__ __forceinline__ bool conflicts(int nb, int k, int i1, int j1, int i2, int j2) {
-  if (nb == 0)
-    return !((abs(i1 - i2) > 1) && (abs(i1 - j2) > 1) && (abs(j1 - i2) > 1) && (abs(j1 - j2) > 1));
-  if (nb == 1) return !((j1 < i2 - 1) || (i1 > j2 + 1) || (j2 < i1 - 1) || (i2 > j1 + 1));
-  const int mn1 = min(i1, j1), mx1 = max(i1, j1), mn2 = min(i2, j2), mx2 = max(i2, j2);
-  return !((mx1 + k < mn2) || (mn1 > mx2 + k));
-}
-
+// Interval bitmaps of positions claimed by kept moves.
 __device__ __forceinline__ bool bits_any(const uint32_t* bits, int a, int b) {
   const int wa = a >> 5, wb = b >> 5;
   for (int w = wa; w <= wb; ++w) {
@@ -572,15 +564,80 @@ __device__ __forceinline__ void apply_span(int nb, int k, int i, int j, const in
   }
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
+// Same-id conflict predicates of mlkernel.cu:51-71 without short-circuit
+// branches.
+template <int NB>
+__device__ __forceinline__ bool conflicts_nb(int i1, int j1, int i2, int j2) {
+  constexpr int K = NB >= 2 ? NB - 1 : 0;
+  if (NB == 0)
+    return ((int)((unsigned)(i1 - i2 + 1) <= 2u) | (int)((unsigned)(i1 - j2 + 1) <= 2u) |
+            (int)((unsigned)(j1 - i2 + 1) <= 2u) | (int)((unsigned)(j1 - j2 + 1) <= 2u)) != 0;
+  if (NB == 1)
+    return ((int)(j1 < i2 - 1) | (int)(i1 > j2 + 1) | (int)(j2 < i1 - 1) | (int)(i2 > j1 + 1)) == 0;
+  return ((int)(max(i1, j1) + K < min(i2, j2)) | (int)(min(i1, j1) > max(i2, j2) + K)) == 0;
 }
+
+// Greedy over the M candidates sorted by (cost, row), on one wave, 64 at a
+// time.  The bitmap of positions claimed by kept moves is an exact conflict
+// test against every move kept so far (swap: positions i-1..i+1, j-1..j+1;
+// 2-opt: [i, j+1]; or-opt: [min(i,j), max(i,j)+k]); within a batch the
+// candidates that pass it are decided in order by a scalar loop, each
+// against the batch's kept set through one ballot.  Kept moves go to keys[q]
+// as (i << 32 | j) and to the slot's kept arrays.  Returns their count.
+template <int NB>
+__device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* keys,
+                           const uint16_t* ctxL, uint32_t* bits, int lane) {
+  constexpr int K = NB >= 2 ? NB - 1 : 0;
+  int kept_total = 0;
+  for (int base = 0; base < M; base += kWave) {
+    const int t = base + lane;
+    const bool valid = t < M;
+    int i = 0, j = 0, row = 0;
+    int64_t cost = 0;
+    if (valid) {
+      const int64_t sk = keys[t];
+      cost = sk >> MLP_KEY_SHIFT;
+      row = (int)(sk & MLP_KEY_MASK);
+      move_ij(NB, rowmap, N, row, (int)ctxL[row], i, j);
+    }
+    const int a = NB == 1 ? i : min(i, j);
+    const int b = NB == 1 ? j + 1 : max(i, j) + K;
+    bool pre = valid;
+    if (valid) {
+      if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+      else pre = !bits_any(bits, a, b);
+    }
+    uint64_t m = __ballot(pre), kept = 0;
+    while (m) {
+      const int q = __builtin_ctzll(m);
+      m &= m - 1;
+      const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
+      const bool c = conflicts_nb<NB>(iq, jq, i, j);
+      if ((__ballot(c) & kept) == 0) kept |= 1ull << q;
+    }
+    if ((kept >> lane) & 1ull) {
+      if (NB == 0) {
+        bits_set(bits, i - 1, i + 1);
+        bits_set(bits, j - 1, j + 1);
+      } else {
+        bits_set(bits, a, b);
+      }
+      const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
+      keys[q] = ((int64_t)i << 32) | (uint32_t)j;  // q <= t: already consumed
+      s.kept_cost[NB][q] = cost;
+      s.kept_ij[NB][3 * q + 0] = i;
+      s.kept_ij[NB][3 * q + 1] = j;
+      s.kept_ij[NB][3 * q + 2] = row;
+    }
+    kept_total += __popcll(kept);
+  }
+  return kept_total;
+}
+
 
 }  // namespace
 
-// One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits.
+// One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
@@ -601,6 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   int32_t* tnew = told + ncap;
   uint32_t* bits = (uint32_t*)(tnew + ncap);
   const int nwords = (N + 32) / 32 + 1;
+  uint16_t* ctxL = (uint16_t*)(bits + (ncap + 32) / 32 + 1);  // ctx of each improving row
 
   if (tid == 0) s_m = 0;
   for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
@@ -618,6 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
       const int at = atomicAdd(&s_m, 1);
       keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + r;
+      ctxL[r] = (uint16_t)(key & MLP_KEY_MASK);
     }
   }
   __syncthreads();
@@ -647,65 +706,17 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
       __syncthreads();
     }
   }
-  // 3. greedy filter, 64 candidates per step on wave 0
+  // 3. greedy filter on wave 0
   if (tid < kWave) {
-    const int lane = tid;
-    int kept_total = 0;
-    for (int base = 0; base < M; base += kWave) {
-      const int t = base + lane;
-      const bool valid = t < M;
-      int i = 0, j = 0, row = 0;
-      int64_t cost = 0;
-      if (valid) {
-        const int64_t sk = keys[t];
-        cost = sk >> MLP_KEY_SHIFT;
-        row = (int)(sk & MLP_KEY_MASK);
-        const int ctx = (int)(rk[row] & MLP_KEY_MASK);
-        move_ij(nb, rowmap, N, row, ctx, i, j);
-      }
-      bool pre = valid;
-      if (valid) {
-        if (nb == 0) {
-          pre = !(((bits[i >> 5] >> (i & 31)) & 1u) || ((bits[j >> 5] >> (j & 31)) & 1u));
-        } else {
-          const int a = nb == 1 ? i : min(i, j);
-          const int b = nb == 1 ? j + 1 : max(i, j) + k;
-          pre = !bits_any(bits, a, b);
-        }
-      }
-      uint64_t cm = 0;
-      for (int q = 0; q < kWave; ++q) {
-        const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
-        if (q < lane && conflicts(nb, k, iq, jq, i, j)) cm |= 1ull << q;
-      }
-      const uint64_t pre_mask = __ballot(pre);
-      uint64_t kept = 0;
-      for (int q = 0; q < kWave; ++q) {
-        if ((pre_mask >> q) & 1ull) {
-          const uint64_t cq = readlane64(cm, q);
-          if ((cq & kept) == 0) kept |= 1ull << q;
-        }
-      }
-      const bool mine = (kept >> lane) & 1ull;
-      if (mine) {
-        if (nb == 0) {
-          bits_set(bits, i - 1, i + 1);
-          bits_set(bits, j - 1, j + 1);
-        } else {
-          const int a = nb == 1 ? i : min(i, j);
-          const int b = nb == 1 ? j + 1 : max(i, j) + k;
-          bits_set(bits, a, b);
-        }
-        const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
-        keys[q] = ((int64_t)i << 32) | (uint32_t)j;  // q <= t: already consumed
-        s.kept_cost[nb][q] = cost;
-        s.kept_ij[nb][3 * q + 0] = i;
-        s.kept_ij[nb][3 * q + 1] = j;
-        s.kept_ij[nb][3 * q + 2] = row;
-      }
-      kept_total += __popcll(kept);
+    int kept = 0;
+    switch (nb) {
+      case 0: kept = greedy_wave<0>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
+      case 1: kept = greedy_wave<1>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
+      case 2: kept = greedy_wave<2>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
+      case 3: kept = greedy_wave<3>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
+      default: kept = greedy_wave<4>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
     }
-    if (lane == 0) s_kept = kept_total;
+    if (tid == 0) s_kept = kept;
   }
   __syncthreads();
   const int K = s_kept;
@@ -1421,9 +1432,10 @@ void mlp_tile_shape(int tile, int* rows, int* cols) {
   *cols = tile == MLP_TILE_4x256 ? 256 : tile == MLP_TILE_8x512 ? 512 : 1024;
 }
 
+// keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {
   const int nwords = (ncap + 32) / 32 + 1;
-  return (size_t)pcap * 8 + (size_t)ncap * 8 + (size_t)nwords * 4;
+  return (size_t)pcap * 8 + (size_t)ncap * 8 + (size_t)nwords * 4 + (size_t)pcap * 2;
 }
 
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
