@@ -56,6 +56,56 @@ def cpu_baseline(xs, ys, tours, budget_s=12.0):
                       f"({moves} moves, {dt:.1f} s, oracle eval_rows, 1 thread)"}
 
 
+def dvnd_legs(eng, args):
+    """DVND solves: n=100 alone; 64 uneven instances batched vs one at a time;
+    a Poisson job stream served (continuous batching) vs plain."""
+    import numpy as np
+    from wamca_mi import inputs as I
+    from wamca_mi import serve
+    from wamca_mi import workloads as WL
+    res = {}
+    # n = 100 uniform [0,1000]^2 seed 1, nint, random-insertion start (seed 1)
+    xs, ys = I.uniform(100, 1, 1000)
+    t0 = I.random_tour(xs, ys, 1, 1)
+    inst = eng.instance(xs, ys, 1)
+    s = eng.slot(inst)
+    times = []
+    for _ in range(5):
+        eng.set_tour(s, t0)
+        eng.sync()
+        t = time.perf_counter()
+        rounds = eng.dvnd([s], 0)
+        times.append(time.perf_counter() - t)
+    res["n100_alone"] = {"ms_per_solve": min(times) * 1e3, "latency": int(eng.latency([s])[0]),
+                         "rounds": int(rounds[0])}
+    eng.destroy_slot(s)
+    eng.destroy_instance(inst)
+    # 64 instances, n log-uniform [50, 2000], half uniform / half clustered
+    jobs, meta = WL.dvnd_jobs(64, 3)
+    serve.run_served(eng, jobs[:8], [0.0] * 8)  # warm-up
+    sv = serve.run_served(eng, jobs, [0.0] * 64, rounds_per_step=4)
+    pl = serve.run_plain(eng, jobs, [0.0] * 64)
+    res["batched_64"] = {
+        "jobs": 64, "mean_n": float(np.mean([m["n"] for m in meta])),
+        "served_solves_per_s": sv["solves_per_s"], "plain_solves_per_s": pl["solves_per_s"],
+        "ratio": sv["solves_per_s"] / pl["solves_per_s"],
+        "served_makespan_s": sv["makespan_s"], "plain_makespan_s": pl["makespan_s"],
+        "identical_results": sv["latency"] == pl["latency"]}
+    # Poisson stream
+    jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4)
+    arr = WL.poisson_arrivals(args.stream_rate, args.stream_jobs, 40)
+    sv = serve.run_served(eng, jobs, arr)
+    pl = serve.run_plain(eng, jobs, arr)
+    res["stream"] = {
+        "jobs": args.stream_jobs, "rate_per_s": args.stream_rate,
+        "n": "log-uniform [50, 2000], half uniform [0,1000]^2, half clustered",
+        "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+        "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+        "ratio": sv["solves_per_s"] / pl["solves_per_s"],
+        "identical_results": sv["latency"] == pl["latency"]}
+    return res
+
+
 def traffic_per_launch(tile):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (calibrated against a copy kernel), if present."""
@@ -77,6 +127,9 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="-1: autotune over the three shapes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-dvnd", action="store_true")
+    ap.add_argument("--stream-jobs", type=int, default=300)
+    ap.add_argument("--stream-rate", type=float, default=2000.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run device copies of this size (counter calibration)")
     args = ap.parse_args()
@@ -187,6 +240,8 @@ def main():
             if best is None or rec["moves_per_s"] > best["moves_per_s"]:
                 best = rec
         out["batched"] = best
+    if not args.no_dvnd:
+        out["dvnd"] = dvnd_legs(eng, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(xs, ys, tours)
     eng.close()

@@ -689,21 +689,35 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   while (P < M) P <<= 1;
   for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
   __syncthreads();
-  // 2. bitonic sort by (cost, row) == stable sort by cost of the row list
-  for (int kk = 2; kk <= P; kk <<= 1) {
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      for (int t = tid; t < P; t += kBlock) {
-        const int u = t ^ jj;
-        if (u > t) {
-          const int64_t a = keys[t], b = keys[u];
-          const bool up = (t & kk) == 0;
-          if (up ? (a > b) : (a < b)) {
-            keys[t] = b;
-            keys[u] = a;
+  // 2. bitonic sort by (cost, row) == stable sort by cost of the row list.
+  // Each phase has P/2 compare-swaps; a thread takes up to four pairs at a
+  // time, loading all of them before comparing.
+  {
+    const int half = P >> 1;
+    for (int kk = 2; kk <= P; kk <<= 1) {
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        const int lj = 31 - __clz(jj);
+        for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
+          int lo[4];
+          int64_t va[4], vb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int pr = min(p0 + u * kBlock, half - 1);
+            lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
+            va[u] = keys[lo[u]];
+            vb[u] = keys[lo[u] + jj];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool up = (lo[u] & kk) == 0;
+            if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
+              keys[lo[u]] = vb[u];
+              keys[lo[u] + jj] = va[u];
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
   // 3. greedy filter on wave 0
