@@ -55,7 +55,7 @@ enum { MLP_ROWMAP_ADS = 0, MLP_ROWMAP_ODC = 1 };
 
 /* Largest supported tour size N (positions).  Per-row keys pack the row-local
  * index in 20 bits and the delta in the remaining 43. */
-#define MLP_MAX_N 16384
+#define MLP_MAX_N 32768
 
 typedef struct mlp_ctx mlp_ctx;
 
@@ -174,6 +174,10 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
  * from the launch's total move count, or by the table mlp_autotune fills),
  * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024.  Results do not
  * depend on the shape. */
+/* The merge keeps its working arrays in LDS when they fit (about N <= 8190)
+ * and in the slot's device memory otherwise; on = 1 uses device memory for
+ * every size (same results; for checking the large-N path on small tours). */
+int mlp_merge_force_global(mlp_ctx* ctx, int32_t on);
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
 /* Autotuner of the sweep tile shape: times every shape on an evaluation of
  * nb_mask over the slots (reps launches each, after a warm-up; the slots' row

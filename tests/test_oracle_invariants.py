@@ -143,3 +143,45 @@ def test_nint_differs_from_exact_rounding_above_2048():
     k = 3000
     assert O.lib().or_dist_s(k * k + k, 1) == k + 1
     assert O.lib().or_dist_s(1000 * 1000 + 1000, 1) == 1000
+
+
+def _extreme_instance(n, seed, span):
+    """Two far clusters; the tour alternates between them (long edges, large
+    total duration) -- the worst case for the delta magnitudes."""
+    rng = np.random.RandomState(seed)
+    xs = np.where(np.arange(n) % 2 == 0, rng.randint(0, 20, n), span - rng.randint(0, 20, n))
+    ys = rng.randint(0, span + 1, n)
+    return xs.astype(np.int32), ys.astype(np.int32)
+
+
+@pytest.mark.parametrize("n,seed,extreme", [(40, 1, False), (90, 2, False), (60, 3, True),
+                                            (120, 4, True)])
+def test_int32_exactness_bounds(n, seed, extreme):
+    """The bounds the device uses to choose the int32 path (per neighborhood):
+    |delta| <= 4 N maxd (swap), 3 Ttot + 7 N maxd (or-opt-k, k <= 3; the
+    device adds one more N maxd of margin), N (Ttot + 6 maxd) + 3 Ttot (2-opt),
+    checked against every move of random and adversarial tours."""
+    span = 3000
+    xs, ys = _extreme_instance(n, seed, span) if extreme else O.instance_uniform(n, seed, span)
+    W = O.weights(xs, ys, 1)
+    maxd = int(W.max())
+    tours = [O.perm_tour(seed + q, n) for q in range(3)]
+    if extreme:
+        tours.append(np.array([0] + list(range(1, n)) + [0], np.int32))
+    for t in tours:
+        N = len(t)
+        w = np.array([W[t[p - 1], t[p]] for p in range(1, N)], np.int64)
+        Ttot = int(w.sum())
+        for nb in range(5):
+            m = O.enumerate_moves(nb, 0, W, t)
+            top = int(np.abs(m["ads"]).max())
+            if nb == 0:
+                bound = 4 * N * maxd
+            elif nb == 1:
+                bound = N * (Ttot + 6 * maxd) + 3 * Ttot
+            else:
+                bound = 3 * Ttot + 7 * N * maxd
+            assert top <= bound, (nb, top, bound)
+            # the or-opt bound is not loose by orders of magnitude
+            if nb >= 2 and extreme:
+                assert top * 20 >= bound, (nb, top, bound)

@@ -18,13 +18,12 @@
 #define MLP_KEY_SHIFT 20                       // row-local index bits in a key
 #define MLP_KEY_MASK ((1LL << MLP_KEY_SHIFT) - 1)
 #define MLP_KEY_NONE INT64_MAX                 // row without a move
-#define MLP_MERGE_MAX_ROWS 8192                // LDS-resident merge capacity
 
 // info[] words of a slot
 enum {
   MLP_INFO_LATENCY = 0,   // int64 latency of the current tour
   MLP_INFO_TTOT = 1,      // T[N-1]
-  MLP_INFO_PATH32 = 2,    // 1 if the int32 path is exact for this tour
+  MLP_INFO_PATH32 = 2,    // MLP_P32_* flags: neighborhoods whose int32 path is exact
   MLP_INFO_ROUND_MOVES = 3,  // moves applied since the last round reset
   MLP_INFO_TOTAL_MOVES = 4,  // moves applied since the tour was set
   MLP_INFO_ACTIVE = 5,    // 1 while a DVND run on this slot continues
@@ -33,6 +32,8 @@ enum {
   MLP_INFO_PARITY0 = 16,  // row buffer (0/1) holding the last eval of nb (16 + nb)
   MLP_INFO_WORDS = 24
 };
+
+enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2 };
 
 struct SlotDev {
   int32_t N;      // positions (n + 1)
@@ -61,6 +62,7 @@ struct SlotDev {
   int64_t* rows[MLP_NB_COUNT];
   int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
   int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
+  void* mscratch;  // the merge's working arrays when they exceed LDS (layout of its LDS)
 };
 
 // Z[p] = 2 Q[p] - (2p + 1) T[p] turns the 2-opt internal sum into

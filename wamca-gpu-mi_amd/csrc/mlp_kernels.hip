@@ -291,7 +291,9 @@ __device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, 
   }
   if (row >= sg.rows) return;
   if (lane == 0) oth[row] = MLP_KEY_NONE;
-  if (s.info[MLP_INFO_PATH32]) eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
+  const int64_t f = s.info[MLP_INFO_PATH32];
+  if (f & (sg.nb == 1 ? MLP_P32_2OPT : MLP_P32_SWAP_OROPT))
+    eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
   else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane, cur + row);
 }
 
@@ -414,11 +416,19 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
     int64_t L = 0;
     for (int q = 0; q < kBlock / kWave; ++q) L += P.lsum[q];
     const int64_t Ttot = P.carry[0];  // T[N-1]: positions past N-1 add nothing
-    // every |delta| <= N (Ttot + 6 maxd) + 3 Ttot (see the delta forms)
-    const int64_t bound = (int64_t)N * (Ttot + 6 * (int64_t)s.maxd) + 3 * Ttot;
+    // Moves computed mod 2^32 are exact while |delta| < 2^31.  Node-wise:
+    // a 2-opt reversal moves each reversed node's arrival by at most
+    // Ttot + 2 maxd, so |delta| <= N (Ttot + 6 maxd) + 3 Ttot; a swap changes
+    // 4 edges (|delta| <= 4 N maxd); an or-opt-k moves k nodes by at most
+    // Ttot each and every other node by at most (k + 4) maxd, so
+    // |delta| <= 3 Ttot + 7 N maxd for k <= 3.
+    const int64_t md = s.maxd;
+    const int64_t b2 = (int64_t)N * (Ttot + 6 * md) + 3 * Ttot;
+    const int64_t bso = 3 * Ttot + 8 * (int64_t)N * md;
     s.info[MLP_INFO_LATENCY] = L;
     s.info[MLP_INFO_TTOT] = Ttot;
-    s.info[MLP_INFO_PATH32] = bound < 2147483647LL ? 1 : 0;
+    s.info[MLP_INFO_PATH32] = (b2 < 2147483647LL ? MLP_P32_2OPT : 0) |
+                              (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0);
   }
   __syncthreads();
 }
@@ -539,7 +549,7 @@ __device__ __forceinline__ void bits_set(uint32_t* bits, int a, int b) {
 // Positions a kept move rewrites, and their new sources (mlkswap.cu:1066-1106,
 // mlk2opt.cu:725-768, mlkoropt.cu:886-950 restated as a position map).
 constexpr int kShortMove = 64;
-constexpr int kLongMoves = 256;
+constexpr int kLongMoves = 512;
 static_assert(MLP_MAX_N / (kShortMove + 1) < kLongMoves, "disjoint long spans fit the list");
 
 __device__ __forceinline__ int move_span(int nb, int k, int i, int j) {
@@ -638,10 +648,11 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
 }  // namespace
 
 // One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
+template <bool GMEM>
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
-                                                        int rowmap, int pcap, int ncap,
+                                                        int rowmap, int pcap_l, int ncap_l,
                                                         int skip_inactive) {
   extern __shared__ int64_t lds[];
   __shared__ int s_m, s_kept, s_long;
@@ -653,8 +664,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   const int N = s.N, tid = threadIdx.x;
   const int R = rows_of(nb, rowmap, N);
   const int k = nb_k(nb);
-  int64_t* keys = lds;
-  int32_t* told = (int32_t*)(lds + pcap);
+  // working arrays in LDS (capacities of the launch), or in the slot's global
+  // scratch laid out for its own size
+  int pcap = pcap_l, ncap = ncap_l;
+  if (GMEM) {
+    pcap = 1;
+    while (pcap < s.rowcap) pcap <<= 1;
+    ncap = N;
+  }
+  int64_t* keys = GMEM ? (int64_t*)s.mscratch : lds;
+  int32_t* told = (int32_t*)(keys + pcap);
   int32_t* tnew = told + ncap;
   uint32_t* bits = (uint32_t*)(tnew + ncap);
   const int nwords = (N + 32) / 32 + 1;
@@ -1154,7 +1173,7 @@ __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int pari
   const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
   const int cend = min(q1 + 3, N);
   // every global load of the tile is issued before the first wait
-  const bool p32 = ldg(s.info + MLP_INFO_PATH32) != 0;
+  const bool p32 = (ldg(s.info + MLP_INFO_PATH32) & MLP_P32_SWAP_OROPT) != 0;
   int xa[R + 4];
 #pragma unroll
   for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
@@ -1223,7 +1242,7 @@ __device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds
   const float dr = round_dr(s.round);
   const int rows = (N + 1) >> 1;
   const int par = (parity >> 1) & 1;
-  const bool p32 = ldg(s.info + MLP_INFO_PATH32) != 0;
+  const bool p32 = (ldg(s.info + MLP_INFO_PATH32) & MLP_P32_2OPT) != 0;
   const int by = rb * MLP_2OPT_ROWS + wave;
   const bool live = by < rows;
   const int c0 = ct * C, c1 = min(c0 + C, N - 3);  // row-local indices [c0, c1)
@@ -1453,15 +1472,20 @@ size_t mlp_merge_lds_bytes(int pcap, int ncap) {
 }
 
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
-  return hipFuncSetAttribute((const void*)k_merge_apply,
+  return hipFuncSetAttribute((const void*)k_merge_apply<false>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
 }
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive) {
-  hipLaunchKernelGGL(k_merge_apply, dim3(count), dim3(kBlock), mlp_merge_lds_bytes(pcap, ncap),
-                     st, table, inl, list_dev, nb, rowmap, pcap, ncap, skip_inactive);
+                                  int pcap, int ncap, int skip_inactive, int gmem) {
+  if (gmem)
+    hipLaunchKernelGGL(k_merge_apply<true>, dim3(count), dim3(kBlock), 0, st, table, inl,
+                       list_dev, nb, rowmap, pcap, ncap, skip_inactive);
+  else
+    hipLaunchKernelGGL(k_merge_apply<false>, dim3(count), dim3(kBlock),
+                       mlp_merge_lds_bytes(pcap, ncap), st, table, inl, list_dev, nb, rowmap, pcap,
+                       ncap, skip_inactive);
   return hipGetLastError();
 }
 

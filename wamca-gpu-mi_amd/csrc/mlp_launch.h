@@ -28,7 +28,7 @@ size_t mlp_merge_lds_bytes(int pcap, int ncap);
 hipError_t mlp_merge_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive);
+                                  int pcap, int ncap, int skip_inactive, int gmem);
 hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, uint32_t* S,
                           uint32_t* out);
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);

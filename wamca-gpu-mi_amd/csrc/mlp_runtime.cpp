@@ -81,10 +81,19 @@ int row_cap(int N) {
   return r;
 }
 
+// The merge's working arrays for one slot of N positions (capacity of the
+// sorted-key array: the power of two at or above the row capacity).
+int merge_pcap(int N) {
+  int p = 1;
+  while (p < row_cap(N)) p <<= 1;
+  return p;
+}
+size_t merge_bytes(int N) { return mlp_merge_lds_bytes(merge_pcap(N), N); }
+
 // Byte layout of one slot's device block.
 struct SlotLayout {
   size_t tour, xy, w, t32, q32, g32, h32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
-      kij[MLP_NB_COUNT], total;
+      kij[MLP_NB_COUNT], ms, total;
 };
 
 SlotLayout slot_layout(int N) {
@@ -111,6 +120,7 @@ SlotLayout slot_layout(int N) {
     L.kc[nb] = take(8 * r);
     L.kij[nb] = take(12 * r);
   }
+  L.ms = take(merge_bytes(N));  // merge working arrays in global memory
   L.total = off;
   return L;
 }
@@ -139,6 +149,7 @@ struct mlp_ctx {
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
+  bool merge_force_global = false;  // merge working arrays in global memory for every size
   int8_t tune[32][MLP_TUNE_BUCKETS];  // best shape per (mask, log2 moves), -1 untuned
   // device memory pool: released blocks stay on per-class free lists (no
   // hipFree, so no implicit device synchronisation, while jobs come and go)
@@ -532,6 +543,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
     d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
     d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
   }
+  d.mscratch = b + L.ms;
   for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
     HCK(mlp_launch_fill_i64(ctx->stream, d.rows[nb], 2 * (int64_t)d.rowcap, MLP_KEY_NONE));
   HCK(mlp_launch_set_slot(ctx->stream, ctx->d_table, id, d));
@@ -849,6 +861,12 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   return eval_impl(ctx, slots, count, nb_mask, rowmap, persistent, 0, nullptr);
 }
 
+int mlp_merge_force_global(mlp_ctx* ctx, int32_t on) {
+  if (!ctx) return MLP_EINVAL;
+  ctx->merge_force_global = on != 0;
+  return MLP_OK;
+}
+
 int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                  int32_t reps, int32_t* best_tile, float* us_per_launch) {
   int st = check_slots(ctx, slots, count);
@@ -1027,7 +1045,7 @@ int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t*
 int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int32_t* rows) {
   if (!ctx || !moves) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
-  if (ctx->slots[slot].N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j)");
+  if (ctx->slots[slot].N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j of MLMove64)");
   std::vector<int64_t> c(MLP_MAX_N);
   std::vector<int32_t> ii(MLP_MAX_N), jj(MLP_MAX_N);
   int R = 0;
@@ -1055,6 +1073,7 @@ int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int
 static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb, int skip) {
   if (count == 0) return MLP_OK;
   int rowmap = -1, pcap = 1, ncap = 1;
+  std::vector<int32_t> in_lds, in_gmem;
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
     const int rm = s.eval_rowmap[nb];
@@ -1065,23 +1084,32 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
     if (rowmap >= 0 && rm != rowmap)
       return fail(ctx, MLP_EINVAL, "slots evaluated with different row maps");
     rowmap = rm;
-    const int R = std::max(1, rows_of(nb, rm, s.N));
-    if (R > MLP_MERGE_MAX_ROWS)
-      return fail(ctx, MLP_ERANGE, "merge supports at most %d rows", MLP_MERGE_MAX_ROWS);
-    while (pcap < R) pcap <<= 1;
-    ncap = std::max(ncap, s.N);
+    // a slot whose working arrays fit LDS joins the LDS launch
+    if (!ctx->merge_force_global && merge_bytes(s.N) <= ctx->max_merge_lds) {
+      in_lds.push_back(slots[q]);
+      pcap = std::max(pcap, merge_pcap(s.N));
+      ncap = std::max(ncap, s.N);
+    } else {
+      in_gmem.push_back(slots[q]);
+    }
   }
-  if (mlp_merge_lds_bytes(pcap, ncap) > ctx->max_merge_lds)
-    return fail(ctx, MLP_ERANGE, "merge of N = %d exceeds LDS", ncap);
+  if (!in_lds.empty() && mlp_merge_lds_bytes(pcap, ncap) > ctx->max_merge_lds) {
+    // mixed sizes whose joint capacities exceed LDS: largest ones go global
+    in_gmem.insert(in_gmem.end(), in_lds.begin(), in_lds.end());
+    in_lds.clear();
+  }
   for (int q = 0; q < count; ++q) ctx->slots[slots[q]].merged[nb] = true;
-  for (int q0 = 0; q0 < count; q0 += 65535) {
-    const int c = std::min(65535, count - q0);
-    SlotListInline inl;
-    const int32_t* dev = nullptr;
-    int st = slot_list(ctx, slots + q0, c, inl, &dev);
-    if (st) return st;
-    HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
-                               skip));
+  for (int g = 0; g < 2; ++g) {
+    const std::vector<int32_t>& list = g ? in_gmem : in_lds;
+    for (size_t q0 = 0; q0 < list.size(); q0 += 65535) {
+      const int c = (int)std::min<size_t>(65535, list.size() - q0);
+      SlotListInline inl;
+      const int32_t* dev = nullptr;
+      int st = slot_list(ctx, list.data() + q0, c, inl, &dev);
+      if (st) return st;
+      HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
+                                 skip, g));
+    }
   }
   return MLP_OK;
 }

@@ -15,6 +15,9 @@ NB_NAMES = ("swap", "2opt", "oropt1", "oropt2", "oropt3")
 ROWMAP_ADS, ROWMAP_ODC = 0, 1
 
 
+MAX_N = 32768  # MLP_MAX_N: largest tour size N (positions)
+
+
 def rows_of(nb: int, rowmap: int, N: int) -> int:
     """Rows of a neighborhood's launch grid (mlkswap.cu:991,1003; mlk2opt.cu:661;
     mlkoropt.cu:819)."""
@@ -139,7 +142,7 @@ class Engine:
         self._ck(self._L.mlp_eval(self.h, slots, len(slots), nb_mask, rowmap, int(persistent)))
 
     def rows(self, s, nb):
-        cap = 16384
+        cap = MAX_N
         cost = np.empty(cap, np.int64)
         i = np.empty(cap, np.int32)
         j = np.empty(cap, np.int32)
@@ -149,7 +152,7 @@ class Engine:
         return {"cost": cost[:r].copy(), "i": i[:r].copy(), "j": j[:r].copy()}
 
     def moves64(self, s, nb):
-        out = np.empty(16384, np.uint64)
+        out = np.empty(MAX_N, np.uint64)
         R = C.c_int32()
         self._ck(self._L.mlp_get_moves64(self.h, s, nb, out, C.byref(R)))
         return out[:R.value].copy()
@@ -159,11 +162,15 @@ class Engine:
         self._ck(self._L.mlp_merge_apply(self.h, slots, len(slots), nb))
 
     def applied(self, s, nb):
-        cap = 16384
+        cap = MAX_N
         buf = (MoveRec * cap)()
         cnt = C.c_int32()
         self._ck(self._L.mlp_get_applied(self.h, s, nb, buf, cap, C.byref(cnt)))
         return [(buf[q].cost, buf[q].id, buf[q].i, buf[q].j, buf[q].row) for q in range(cnt.value)]
+
+    def merge_force_global(self, on: bool):
+        """Merge working arrays in device memory for every tour size."""
+        self._ck(self._L.mlp_merge_force_global(self.h, int(on)))
 
     def dvnd(self, slots, rowmap=ROWMAP_ADS, max_rounds=0):
         slots = np.ascontiguousarray(slots, np.int32)
