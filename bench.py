@@ -92,13 +92,14 @@ def dvnd_legs(eng, args):
         "served_makespan_s": sv["makespan_s"], "plain_makespan_s": pl["makespan_s"],
         "identical_results": sv["latency"] == pl["latency"]}
     # Poisson stream
-    jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4)
+    jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4, big_frac=args.stream_big_frac)
     arr = WL.poisson_arrivals(args.stream_rate, args.stream_jobs, 40)
     sv = serve.run_served(eng, jobs, arr)
     pl = serve.run_plain(eng, jobs, arr)
     res["stream"] = {
         "jobs": args.stream_jobs, "rate_per_s": args.stream_rate,
-        "n": "log-uniform [50, 2000], half uniform [0,1000]^2, half clustered",
+        "n": "log-uniform [50, 2000], half uniform [0,1000]^2, half clustered; "
+             f"{sum(m['n'] == 18000 for m in meta)} jobs clustered n = 18000",
         "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
         "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
         "ratio": sv["solves_per_s"] / pl["solves_per_s"],
@@ -130,6 +131,8 @@ def main():
     ap.add_argument("--no-dvnd", action="store_true")
     ap.add_argument("--stream-jobs", type=int, default=300)
     ap.add_argument("--stream-rate", type=float, default=2000.0)
+    ap.add_argument("--stream-big-frac", type=float, default=0.01,
+                    help="fraction of stream jobs replaced by clustered n = 18000")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run device copies of this size (counter calibration)")
     args = ap.parse_args()

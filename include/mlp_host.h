@@ -48,6 +48,12 @@ int mlp_tour_perm(uint32_t seed, int32_t n, int32_t* tour);
  * MT19937(seed); N = n + 1 entries. */
 int mlp_tour_random(const int32_t* xs, const int32_t* ys, int32_t n, int32_t round, uint32_t seed,
                     int32_t skip, float alpha, int32_t* tour);
+/* The same randomized construction with the rank-`pick` candidate found by
+ * selection (ties in distance ordered by client id): O(n^2), for large
+ * synthetic instances.  Tours differ from mlp_tour_random's where distances
+ * tie. */
+int mlp_tour_random_fast(const int32_t* xs, const int32_t* ys, int32_t n, int32_t round,
+                         uint32_t seed, float alpha, int32_t* tour);
 
 #ifdef __cplusplus
 }

@@ -252,4 +252,45 @@ int mlp_tour_random(const int32_t* xs, const int32_t* ys, int32_t n, int32_t rou
   return 0;
 }
 
+// Same randomized construction with the candidate of rank `pick` found by
+// selection instead of a full sort, ties in distance ordered by client id:
+// O(n^2) instead of O(n^2 log n), for synthetic tours of large instances.
+// (Tie order differs from std::sort's, so tours differ from
+// mlp_tour_random's wherever distances tie.)
+int mlp_tour_random_fast(const int32_t* xs, const int32_t* ys, int32_t n, int32_t round,
+                         uint32_t seed, float alpha, int32_t* tour) {
+  if (n < 2 || !xs || !ys || !tour) return -1;
+  MT m(seed);
+  std::vector<int> cand((size_t)n - 1);
+  std::vector<int64_t> key((size_t)n - 1);
+  for (int c = 1; c < n; ++c) cand[(size_t)c - 1] = c;
+  unsigned count = (unsigned)(n - 1);
+  int last = 0, pos = 0;
+  tour[pos++] = 0;
+  while (count > 0) {
+    unsigned pick = 0;
+    if (count > 1) {
+      const unsigned rcl = (unsigned)std::ceil(alpha * count);
+      pick = rcl > 0 ? m.upto(rcl - 1) : 0;
+      if (alpha < 0.5F) pick = count - pick - 1;
+    }
+    for (unsigned q = 0; q < count; ++q) {
+      const int c = cand[q];
+      key[q] = ((int64_t)host_dist(xs[last], ys[last], xs[c], ys[c], round) << 32) | (uint32_t)c;
+    }
+    std::nth_element(key.begin(), key.begin() + pick, key.begin() + count);
+    const int cur = (int)(uint32_t)key[pick];
+    tour[pos++] = cur;
+    last = cur;
+    for (unsigned q = 0; q < count; ++q)
+      if (cand[q] == cur) {
+        cand[q] = cand[count - 1];
+        break;
+      }
+    --count;
+  }
+  tour[pos] = 0;
+  return 0;
+}
+
 }  // extern "C"

@@ -116,6 +116,8 @@ _SIGS = {
     "mlp_tour_perm": (C.c_int, [C.c_uint32, C.c_int32, I32P]),
     "mlp_tour_random": (C.c_int, [I32P, I32P, C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
                                   C.c_float, I32P]),
+    "mlp_tour_random_fast": (C.c_int, [I32P, I32P, C.c_int32, C.c_int32, C.c_uint32, C.c_float,
+                                       I32P]),
 }
 
 

@@ -52,3 +52,13 @@ def random_tour(xs, ys, rnd: int, seed: int, skip: int = 0, alpha: float = 0.5):
     t = np.empty(len(xs) + 1, np.int32)
     _ck(lib().mlp_tour_random(xs, ys, len(xs), rnd, seed, skip, alpha, t), "random tour")
     return t
+
+
+def random_tour_fast(xs, ys, rnd: int, seed: int, alpha: float = 0.5):
+    """The randomized construction with selection instead of sorting (O(n^2);
+    ties in distance ordered by client id) for large instances."""
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    t = np.empty(len(xs) + 1, np.int32)
+    _ck(lib().mlp_tour_random_fast(xs, ys, len(xs), rnd, seed, alpha, t), "random tour (fast)")
+    return t

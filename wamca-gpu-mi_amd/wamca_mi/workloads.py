@@ -5,7 +5,9 @@
   (sigma 40, centres uniform in [0,2000]^2), each with its own seed, and one
   random-insertion start tour each (greedy randomized construction, alpha
   0.5, MLSolution::random of the reference, mlsolution.cpp:753-874).
-  A fraction ``big_frac`` of the jobs is replaced by clustered n = big_n.
+  A fraction ``big_frac`` of the jobs is replaced by clustered n = big_n
+  (their start tours come from the selection-based variant of the
+  construction, ``inputs.random_tour_fast``).
 - ``poisson_arrivals``: arrival times of a Poisson process of a given rate.
 All draws come from MT19937 streams of the library's host code (mlp_host.h).
 """
@@ -39,7 +41,9 @@ def dvnd_jobs(count: int, seed: int, lo: int = 50, hi: int = 2000, big_frac: flo
             else:
                 xs, ys = I.clustered(n, iseed)
                 kind = "clustered"
-        tour = I.random_tour(xs, ys, rnd, iseed)
+        # the reference's construction sorts per step (O(n^2 log n)); large
+        # jobs use the selection-based variant
+        tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, rnd, iseed)
         jobs.append((xs, ys, tour))
         meta.append({"n": n, "kind": kind, "seed": iseed})
     return jobs, meta
