@@ -91,19 +91,22 @@ def dvnd_legs(eng, args):
         "ratio": sv["solves_per_s"] / pl["solves_per_s"],
         "served_makespan_s": sv["makespan_s"], "plain_makespan_s": pl["makespan_s"],
         "identical_results": sv["latency"] == pl["latency"]}
-    # Poisson stream
-    jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4, big_frac=args.stream_big_frac)
-    arr = WL.poisson_arrivals(args.stream_rate, args.stream_jobs, 40)
-    sv = serve.run_served(eng, jobs, arr)
-    pl = serve.run_plain(eng, jobs, arr)
-    res["stream"] = {
-        "jobs": args.stream_jobs, "rate_per_s": args.stream_rate,
-        "n": "log-uniform [50, 2000], half uniform [0,1000]^2, half clustered; "
-             f"{sum(m['n'] == 18000 for m in meta)} jobs clustered n = 18000",
-        "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
-        "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
-        "ratio": sv["solves_per_s"] / pl["solves_per_s"],
-        "identical_results": sv["latency"] == pl["latency"]}
+    # Poisson streams: n log-uniform in [100, 1000]; and [50, 2000] with a
+    # fraction of clustered n = 18000 jobs
+    for name, lo, hi, big in (("stream_100_1000", 100, 1000, 0.0),
+                              ("stream_50_2000_big", 50, 2000, args.stream_big_frac)):
+        jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4, lo=lo, hi=hi, big_frac=big)
+        arr = WL.poisson_arrivals(args.stream_rate, args.stream_jobs, 40)
+        sv = serve.run_served(eng, jobs, arr)
+        pl = serve.run_plain(eng, jobs, arr)
+        res[name] = {
+            "jobs": args.stream_jobs, "rate_per_s": args.stream_rate,
+            "n": f"log-uniform [{lo}, {hi}], half uniform [0,1000]^2, half clustered; "
+                 f"{sum(m['n'] == 18000 for m in meta)} jobs clustered n = 18000",
+            "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+            "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+            "ratio": sv["solves_per_s"] / pl["solves_per_s"],
+            "identical_results": sv["latency"] == pl["latency"]}
     return res
 
 

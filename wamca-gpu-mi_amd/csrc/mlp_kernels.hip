@@ -1202,6 +1202,8 @@ __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int pari
       if (nb != 1 && ((mask >> nb) & 1))
         reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
   }
+  // swap alone covers j >= i + 2 only: a tile left of that diagonal is empty
+  if (mask == 1 && q1 <= b0 + 3) return;
 #pragma unroll
   for (int it = 0; it < FI; ++it) {
     const int c = cbase + tid + it * kBlock;
