@@ -139,3 +139,30 @@ def run_plain(eng, jobs, arrivals, rnd=1, rowmap=ROWMAP_ADS):
     out["tours"] = tours
     out["rounds"] = rounds
     return out
+
+
+def main():
+    """python -m wamca_mi.serve --job KIND N SEED: one DVND job alone on the GPU
+    (KIND uniform or clustered; start tour from the randomized construction,
+    the selection-based variant above n = 4000)."""
+    import argparse
+    import json
+
+    from . import inputs as I
+    from .engine import Engine
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--job", nargs=3, metavar=("KIND", "N", "SEED"), required=True)
+    args = ap.parse_args()
+    kind, n, seed = args.job[0], int(args.job[1]), int(args.job[2])
+    xs, ys = I.clustered(n, seed) if kind == "clustered" else I.uniform(n, seed, 1000)
+    tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, 1, seed)
+    eng = Engine(0)
+    t0 = time.perf_counter()
+    out = run_plain(eng, [(xs, ys, tour)], [0.0])
+    print(json.dumps({"kind": kind, "n": n, "seed": seed, "seconds": time.perf_counter() - t0,
+                      "rounds": out["rounds"][0], "latency": out["latency"][0]}))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()
