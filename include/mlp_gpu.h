@@ -174,6 +174,17 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
  * from the launch's total move count, or by the table mlp_autotune fills),
  * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024.  Results do not
  * depend on the shape. */
+/* Capture of the context stream into a replayable HIP graph: between
+ * mlp_graph_begin and mlp_graph_end, mlp_eval, mlp_eval_each, mlp_merge_apply
+ * and mlp_dvnd_rounds are recorded instead of run (an eval also records the
+ * clearing of the row buffers it folds into, so every replay starts from
+ * empty rows).  Calls that synchronise or copy to the host fail while a
+ * capture is open.  A graph keeps the slots it was recorded on; replays run
+ * in stream order with other work. */
+int mlp_graph_begin(mlp_ctx* ctx);
+int mlp_graph_end(mlp_ctx* ctx, int32_t* graph);
+int mlp_graph_launch(mlp_ctx* ctx, int32_t graph);
+int mlp_graph_destroy(mlp_ctx* ctx, int32_t graph);
 /* The merge keeps its working arrays in LDS when they fit (about N <= 8190)
  * and in the slot's device memory otherwise; on = 1 uses device memory for
  * every size (same results; for checking the large-N path on small tours). */

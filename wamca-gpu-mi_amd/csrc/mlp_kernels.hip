@@ -870,6 +870,28 @@ __global__ void k_weights(const int32_t* xs, const int32_t* ys, int n, int round
   }
 }
 
+// Clears the row buffers a sweep segment is about to fold into (one block
+// per segment).
+__global__ __launch_bounds__(kBlock) void k_clear_rows(const SlotDev* __restrict__ table,
+                                                       SweepSegsInline inl,
+                                                       const SweepSeg* segs_dev, int nsegs) {
+  if ((int)blockIdx.x >= nsegs) return;
+  SweepSeg sg = inl.s[0];
+  if (segs_dev) {
+    sg = segs_dev[blockIdx.x];
+  } else {
+#pragma unroll
+    for (int q = 1; q < MLP_SEGS_INLINE; ++q)
+      if (q == (int)blockIdx.x) sg = inl.s[q];
+  }
+  const SlotDev& s = table[sg.slot];
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    if ((sg.mask >> nb) & 1) {
+      int64_t* b = s.rows[nb] + (((sg.parity >> nb) & 1) ? s.rowcap : 0);
+      for (int r = threadIdx.x; r < s.rowcap; r += kBlock) b[r] = MLP_KEY_NONE;
+    }
+}
+
 __global__ void k_fill_i64(int64_t* p, int64_t count, int64_t v) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
        t += (int64_t)gridDim.x * blockDim.x)
@@ -1525,6 +1547,12 @@ hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* 
   const int64_t total = (int64_t)n * n;
   hipLaunchKernelGGL(k_weights, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                      st, xs, ys, n, round, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_clear_rows(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
+                                 const SweepSeg* segs_dev, int nsegs) {
+  hipLaunchKernelGGL(k_clear_rows, dim3(nsegs), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs);
   return hipGetLastError();
 }
 

@@ -162,7 +162,11 @@ struct mlp_ctx {
   int dvnd_chunk = 4;
   int dvnd_graph = 1;
   int dvnd_persistent = 0;
-  bool capturing = false;
+  bool capturing = false;     // the stream is being captured into a graph
+  bool user_capture = false;  // ... by mlp_graph_begin
+  std::vector<hipGraphExec_t> user_graphs;
+  std::vector<std::vector<int32_t>> user_graph_slots;  // slots each graph evaluates
+  std::vector<int32_t> capture_slots;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
   int graph_uses = 0;
@@ -191,6 +195,13 @@ int fail(mlp_ctx* c, int st, const char* fmt, ...) {
     hipError_t e_ = (call);                                                                   \
     if (e_ != hipSuccess)                                                                     \
       return fail(ctx, MLP_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+// Calls that synchronise or copy to the host are refused inside a capture.
+#define NO_CAPTURE(ctx)                                                                       \
+  do {                                                                                        \
+    if ((ctx) && (ctx)->capturing)                                                            \
+      return fail((ctx), MLP_ESTATE, "not allowed while a graph capture is open");            \
   } while (0)
 
 bool slot_ok(mlp_ctx* ctx, int s) {
@@ -363,6 +374,11 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
 }
 
 int mlp_ctx_destroy(mlp_ctx* ctx) {
+  if (ctx && ctx->capturing) {  // abandon an open capture
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(ctx->stream, &g) == hipSuccess && g) hipGraphDestroy(g);
+    ctx->capturing = ctx->user_capture = false;
+  }
   if (!ctx) return MLP_EINVAL;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
@@ -370,6 +386,8 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  for (hipGraphExec_t g : ctx->user_graphs)
+    if (g) hipGraphExecDestroy(g);
   if (ctx->d_counter) hipFree(ctx->d_counter);
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
@@ -383,6 +401,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
 }
 
 int mlp_sync(mlp_ctx* ctx) {
+  NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   HCK(hipStreamSynchronize(ctx->stream));
   return MLP_OK;
@@ -452,15 +471,18 @@ static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, i
 
 int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                         int32_t round, int32_t* inst) {
+  NO_CAPTURE(ctx);
   return instance_create(ctx, xs, ys, n, round, inst, false);
 }
 
 int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                               int32_t round, int32_t* inst) {
+  NO_CAPTURE(ctx);
   return instance_create(ctx, xs, ys, n, round, inst, true);
 }
 
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
+  NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
     return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
@@ -474,6 +496,7 @@ int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
 }
 
 int mlp_instance_weights(mlp_ctx* ctx, int32_t inst, int32_t* w_out) {
+  NO_CAPTURE(ctx);
   if (!ctx || !w_out) return MLP_EINVAL;
   if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
     return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
@@ -495,6 +518,7 @@ int64_t mlp_slot_footprint(int32_t n) {
 }
 
 int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
+  NO_CAPTURE(ctx);
   if (!ctx || !slot) return MLP_EINVAL;
   if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
     return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
@@ -554,6 +578,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
 }
 
 int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot) {
+  NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   pool_release(ctx, ctx->slots[slot].mem, ctx->slots[slot].cls);
@@ -562,6 +587,7 @@ int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot) {
 }
 
 int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32_t* tours) {
+  NO_CAPTURE(ctx);
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count == 0) return MLP_OK;
@@ -614,6 +640,7 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
 }
 
 int mlp_get_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* tours) {
+  NO_CAPTURE(ctx);
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count && !tours) return fail(ctx, MLP_EINVAL, "tours is NULL");
@@ -645,6 +672,7 @@ static int gather_info(mlp_ctx* ctx, const int32_t* slots, int count, int word, 
 }
 
 int mlp_get_latency(mlp_ctx* ctx, const int32_t* slots, int32_t count, int64_t* latency) {
+  NO_CAPTURE(ctx);
   if (!latency && count) return fail(ctx, MLP_EINVAL, "latency is NULL");
   return gather_info(ctx, slots, count, MLP_INFO_LATENCY, latency);
 }
@@ -720,6 +748,12 @@ int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask
   return MLP_TILE_16x1024;
 }
 
+// A work counter cleared by a kernel (graph-capturable, unlike a memset).
+hipError_t zero_counter(mlp_ctx* ctx, int* counter) {
+  SlotListInline none{};
+  return mlp_launch_round_begin(ctx->stream, ctx->d_table, none, nullptr, 0, counter, 1);
+}
+
 hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
   EvalArgs& a = *(EvalArgs*)p;
   return mlp_launch_sweep(ctx->stream, a.rowmap, ctx->d_table, a.sinl, a.sdev, a.nsegs, a.total,
@@ -733,6 +767,7 @@ hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
 // a DVND round passes its own counter, cleared by k_round_begin.
 static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                      int32_t rowmap, int32_t persistent, int skip, int* round_counter) {
+  if (ctx->user_capture) ctx->capture_slots.insert(ctx->capture_slots.end(), slots, slots + count);
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
     if (skip) {
@@ -803,13 +838,15 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
       if (st) return st;
     }
     a.grid = total;
+    if (ctx->user_capture)  // replays fold into the same buffers: clear them in the graph
+      HCK(mlp_launch_clear_rows(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs));
     if (persistent) {
       a.grid = std::min(total, ctx->cu_count * kTileWgPerCu[tile]);
       if (round_counter) {
         a.counter = round_counter;
       } else {
         a.counter = ctx->d_counter;
-        HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+        HCK(zero_counter(ctx, a.counter));
       }
     }
     return timed_launch(ctx, launch_sweep_cb, &a);
@@ -846,7 +883,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
       a.counter = round_counter;
     } else {
       a.counter = ctx->d_counter;
-      HCK(hipMemsetAsync(a.counter, 0, 4, ctx->stream));
+      HCK(zero_counter(ctx, a.counter));
     }
   }
   return timed_launch(ctx, launch_eval_cb, &a);
@@ -861,6 +898,72 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   return eval_impl(ctx, slots, count, nb_mask, rowmap, persistent, 0, nullptr);
 }
 
+int mlp_graph_begin(mlp_ctx* ctx) {
+  if (!ctx) return MLP_EINVAL;
+  if (ctx->capturing) return fail(ctx, MLP_ESTATE, "a capture is already open");
+  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  ctx->user_capture = true;
+  ctx->capture_slots.clear();
+  return MLP_OK;
+}
+
+int mlp_graph_end(mlp_ctx* ctx, int32_t* graph) {
+  if (!ctx || !graph) return MLP_EINVAL;
+  if (!ctx->user_capture) return fail(ctx, MLP_ESTATE, "no capture open");
+  ctx->capturing = false;
+  ctx->user_capture = false;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  if (ec != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return fail(ctx, MLP_EDEVICE, "graph capture: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(ctx, MLP_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+  std::vector<int32_t> used = ctx->capture_slots;
+  std::sort(used.begin(), used.end());
+  used.erase(std::unique(used.begin(), used.end()), used.end());
+  // the recorded evals changed the host's parity choice without running
+  for (int32_t id : used)
+    if (slot_ok(ctx, id)) ctx->slots[id].parity_stale = true;
+  for (size_t q = 0; q < ctx->user_graphs.size(); ++q)
+    if (!ctx->user_graphs[q]) {
+      ctx->user_graphs[q] = ex;
+      ctx->user_graph_slots[q] = used;
+      *graph = (int32_t)q;
+      return MLP_OK;
+    }
+  ctx->user_graphs.push_back(ex);
+  ctx->user_graph_slots.push_back(used);
+  *graph = (int32_t)ctx->user_graphs.size() - 1;
+  return MLP_OK;
+}
+
+int mlp_graph_launch(mlp_ctx* ctx, int32_t graph) {
+  if (!ctx) return MLP_EINVAL;
+  if (ctx->capturing) return fail(ctx, MLP_ESTATE, "capture open");
+  if (graph < 0 || graph >= (int)ctx->user_graphs.size() || !ctx->user_graphs[graph])
+    return fail(ctx, MLP_EINVAL, "graph %d does not exist", graph);
+  HCK(hipGraphLaunch(ctx->user_graphs[graph], ctx->stream));
+  // replays record row-buffer parities the host did not choose
+  for (int32_t id : ctx->user_graph_slots[graph])
+    if (slot_ok(ctx, id)) ctx->slots[id].parity_stale = true;
+  return MLP_OK;
+}
+
+int mlp_graph_destroy(mlp_ctx* ctx, int32_t graph) {
+  if (!ctx) return MLP_EINVAL;
+  if (graph < 0 || graph >= (int)ctx->user_graphs.size() || !ctx->user_graphs[graph])
+    return fail(ctx, MLP_EINVAL, "graph %d does not exist", graph);
+  HCK(hipStreamSynchronize(ctx->stream));
+  HCK(hipGraphExecDestroy(ctx->user_graphs[graph]));
+  ctx->user_graphs[graph] = nullptr;
+  return MLP_OK;
+}
+
 int mlp_merge_force_global(mlp_ctx* ctx, int32_t on) {
   if (!ctx) return MLP_EINVAL;
   ctx->merge_force_global = on != 0;
@@ -869,6 +972,7 @@ int mlp_merge_force_global(mlp_ctx* ctx, int32_t on) {
 
 int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                  int32_t reps, int32_t* best_tile, float* us_per_launch) {
+  NO_CAPTURE(ctx);
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (nb_mask == 0 || (nb_mask & ~0x1fu)) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
@@ -934,6 +1038,7 @@ int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
 }
 
 int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity) {
+  NO_CAPTURE(ctx);
   if (!ctx || capacity <= 0) return MLP_EINVAL;
   HCK(hipStreamSynchronize(ctx->stream));
   while (ctx->tev.size() < 2 * (size_t)capacity) {
@@ -947,6 +1052,7 @@ int mlp_timing_begin(mlp_ctx* ctx, int32_t capacity) {
 }
 
 int mlp_timing_end(mlp_ctx* ctx, double* total_ms, int32_t* launches) {
+  NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   HCK(hipStreamSynchronize(ctx->stream));
   double tot = 0;
@@ -970,6 +1076,7 @@ int mlp_region_begin(mlp_ctx* ctx) {
 }
 
 int mlp_region_end(mlp_ctx* ctx, float* ms) {
+  NO_CAPTURE(ctx);
   if (!ctx || !ms) return MLP_EINVAL;
   if (!ctx->region_open) return fail(ctx, MLP_ESTATE, "no timed region open");
   HCK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -993,6 +1100,7 @@ int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb
 }
 
 int mlp_copy_probe(mlp_ctx* ctx, int64_t bytes, int32_t reps) {
+  NO_CAPTURE(ctx);
   if (!ctx || bytes <= 0 || reps <= 0) return MLP_EINVAL;
   void *a = nullptr, *b = nullptr;
   HCK(hipMalloc(&a, (size_t)bytes));
@@ -1012,6 +1120,7 @@ int mlp_copy_probe(mlp_ctx* ctx, int64_t bytes, int32_t reps) {
 
 int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t* i, int32_t* j,
                  int32_t* rows) {
+  NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
@@ -1043,6 +1152,7 @@ int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t*
 }
 
 int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int32_t* rows) {
+  NO_CAPTURE(ctx);
   if (!ctx || !moves) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   if (ctx->slots[slot].N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j of MLMove64)");
@@ -1123,6 +1233,7 @@ int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t n
 
 int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int32_t cap,
                     int32_t* count) {
+  NO_CAPTURE(ctx);
   if (!ctx || !count) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
@@ -1177,7 +1288,7 @@ int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int
 // every device list it needs already exists (no copies inside a capture).
 int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
               int rounds) {
-  if (!ctx->dvnd_graph) {
+  if (!ctx->dvnd_graph || ctx->user_capture) {
     for (int r = 0; r < rounds; ++r) {
       int st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
       if (st) return st;
@@ -1274,6 +1385,7 @@ int mlp_dvnd_rounds(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t r
 
 int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* active,
                     int32_t* rounds) {
+  NO_CAPTURE(ctx);
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count == 0) return MLP_OK;
@@ -1302,6 +1414,7 @@ int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t pers
 
 int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
              int32_t max_rounds, int32_t* rounds_out) {
+  NO_CAPTURE(ctx);
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
@@ -1325,6 +1438,7 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
 }
 
 int mlp_ads_materialize(mlp_ctx* ctx, int32_t slot, uint32_t* host_out, int64_t bytes) {
+  NO_CAPTURE(ctx);
   if (!ctx || !host_out) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   const int N = ctx->slots[slot].N;
@@ -1347,6 +1461,7 @@ int mlp_ads_materialize(mlp_ctx* ctx, int32_t slot, uint32_t* host_out, int64_t 
 }
 
 int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32_t* out) {
+  NO_CAPTURE(ctx);
   if (!ctx || !out || count < 0 || s0 < 0) return MLP_EINVAL;
   if ((int64_t)s0 + count > INT32_MAX) return fail(ctx, MLP_ERANGE, "s range");
   int32_t* d = nullptr;
@@ -1361,6 +1476,7 @@ int mlp_dist_table(mlp_ctx* ctx, int32_t s0, int32_t count, int32_t round, int32
 }
 
 int mlp_sqrt_selfcheck(mlp_ctx* ctx, int64_t* mismatches) {
+  NO_CAPTURE(ctx);
   if (!ctx || !mismatches) return MLP_EINVAL;
   unsigned long long* d = nullptr;
   HCK(hipMalloc(&d, 8));
@@ -1376,6 +1492,7 @@ int mlp_sqrt_selfcheck(mlp_ctx* ctx, int64_t* mismatches) {
 }
 
 int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, int32_t* out) {
+  NO_CAPTURE(ctx);
   if (!ctx || !out || !s || count < 0) return MLP_EINVAL;
   for (int q = 0; q < count; ++q)
     if (s[q] < 0) return fail(ctx, MLP_ERANGE, "negative squared length");

@@ -99,6 +99,10 @@ _SIGS = {
     "mlp_sqrt_selfcheck": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mlp_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_merge_force_global": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_graph_begin": (C.c_int, [C.c_void_p]),
+    "mlp_graph_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "mlp_graph_launch": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_graph_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_autotune": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_uint32, C.c_int32,
                                C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
     "mlp_tune_get": (C.c_int, [C.c_void_p, I8P]),

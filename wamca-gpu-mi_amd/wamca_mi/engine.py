@@ -168,6 +168,22 @@ class Engine:
         self._ck(self._L.mlp_get_applied(self.h, s, nb, buf, cap, C.byref(cnt)))
         return [(buf[q].cost, buf[q].id, buf[q].i, buf[q].j, buf[q].row) for q in range(cnt.value)]
 
+    # -- HIP graphs -----------------------------------------------------------
+    def graph_begin(self):
+        """Start recording evals / merges / DVND rounds into a graph."""
+        self._ck(self._L.mlp_graph_begin(self.h))
+
+    def graph_end(self) -> int:
+        g = C.c_int32()
+        self._ck(self._L.mlp_graph_end(self.h, C.byref(g)))
+        return g.value
+
+    def graph_launch(self, g: int):
+        self._ck(self._L.mlp_graph_launch(self.h, g))
+
+    def graph_destroy(self, g: int):
+        self._ck(self._L.mlp_graph_destroy(self.h, g))
+
     def merge_force_global(self, on: bool):
         """Merge working arrays in device memory for every tour size."""
         self._ck(self._L.mlp_merge_force_global(self.h, int(on)))
