@@ -2,6 +2,8 @@
 replay equals the eager result, with eager work (new tours, other launches,
 synchronisations) between replays; synchronising calls are refused while a
 capture is open."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -19,6 +21,21 @@ def eng():
     e.close()
 
 
+@contextlib.contextmanager
+def capture(eng, out):
+    """Record into a graph; the capture is closed even when a call fails."""
+    eng.graph_begin()
+    try:
+        yield
+    except Exception:
+        try:
+            eng.graph_end()
+        except wm.MLPError:
+            pass
+        raise
+    out.append(eng.graph_end())
+
+
 @pytest.mark.parametrize("persistent", [False, True])
 def test_eval_graph_replays(eng, persistent):
     n = 200
@@ -29,9 +46,10 @@ def test_eval_graph_replays(eng, persistent):
     try:
         eng.set_tours(S, [O.perm_tour(q + 1, n) for q in range(40)])
         eng.sync()
-        eng.graph_begin()
-        eng.eval(S, 0x1F, 0, persistent=persistent)
-        g = eng.graph_end()
+        gs = []
+        with capture(eng, gs):
+            eng.eval(S, 0x1F, 0, persistent=persistent)
+        g = gs[0]
         for rep in range(4):
             tours = [O.perm_tour(100 * rep + q + 7, n) for q in range(40)]
             eng.set_tours(S, tours)
@@ -61,11 +79,12 @@ def test_step_graph_replays(eng):
         tours = [O.random_tour(O.MT(60 + q), W, 0.5) for q in range(3)]
         eng.set_tours(S, tours)
         eng.sync()
-        eng.graph_begin()
-        for nb in (0, 2):
-            eng.eval(S, 1 << nb, 0)
-            eng.merge_apply(S, nb)
-        g = eng.graph_end()
+        gs = []
+        with capture(eng, gs):
+            for nb in (0, 2):
+                eng.eval(S, 1 << nb, 0)
+                eng.merge_apply(S, nb)
+        g = gs[0]
         refs = [t.copy() for t in tours]
         for rep in range(3):
             eng.graph_launch(g)
@@ -87,11 +106,12 @@ def test_capture_refuses_sync_calls(eng):
     s = eng.slot(inst)
     try:
         eng.set_tour(s, O.perm_tour(1, 50))
-        eng.graph_begin()
-        eng.eval([s], 1, 0)
-        with pytest.raises(wm.MLPError):
-            eng.rows(s, 0)
-        g = eng.graph_end()
+        gs = []
+        with capture(eng, gs):
+            eng.eval([s], 1, 0)
+            with pytest.raises(wm.MLPError):
+                eng.rows(s, 0)
+        g = gs[0]
         eng.graph_launch(g)
         eng.sync()
         W = O.weights(xs, ys, 1)

@@ -130,6 +130,7 @@ SlotLayout slot_layout(int N) {
 struct mlp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // non-blocking, for uploads while `stream` is captured
   int cu_count = 0;
   int64_t hbm = 0;
   char name[256] = {0};
@@ -208,7 +209,9 @@ bool slot_ok(mlp_ctx* ctx, int s) {
   return s >= 0 && s < (int)ctx->slots.size() && ctx->slots[s].alive;
 }
 
-// Device copy of an int32 list, uploaded once per distinct content.
+// Device copy of an int32 list, uploaded once per distinct content.  Inside a
+// graph capture a missing list is uploaded on a side stream (the captured
+// stream is not touched), so it exists before any replay.
 int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) {
   std::string key((const char*)v, (size_t)count * 4);
   auto it = ctx->lists.find(key);
@@ -216,20 +219,28 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     *out = (const int32_t*)it->second.p;
     return MLP_OK;
   }
-  if (ctx->capturing) return fail(ctx, MLP_ESTATE, "device list missing during graph capture");
-  if (ctx->lists.size() >= 8192) {
+  if (ctx->lists.size() >= 8192 && !ctx->capturing) {
     HCK(hipStreamSynchronize(ctx->stream));
     if (ctx->graph_exec) {  // its launches point into the lists
       HCK(hipGraphExecDestroy(ctx->graph_exec));
       ctx->graph_exec = nullptr;
       ctx->graph_key.clear();
     }
-    for (auto& kv : ctx->lists) hipFree(kv.second.p);
-    ctx->lists.clear();
+    bool user_graphs = false;
+    for (hipGraphExec_t g : ctx->user_graphs) user_graphs = user_graphs || g;
+    if (!user_graphs) {  // recorded graphs may point into the lists: keep them
+      for (auto& kv : ctx->lists) hipFree(kv.second.p);
+      ctx->lists.clear();
+    }
   }
   void* p = nullptr;
   HCK(hipMalloc(&p, std::max<size_t>(4, (size_t)count * 4)));
-  HCK(hipMemcpy(p, v, (size_t)count * 4, hipMemcpyHostToDevice));
+  if (ctx->capturing) {
+    HCK(hipMemcpyAsync(p, v, (size_t)count * 4, hipMemcpyHostToDevice, ctx->side));
+    HCK(hipStreamSynchronize(ctx->side));
+  } else {
+    HCK(hipMemcpy(p, v, (size_t)count * 4, hipMemcpyHostToDevice));
+  }
   ctx->lists.emplace(std::move(key), DevList{p});
   *out = (const int32_t*)p;
   return MLP_OK;
@@ -352,7 +363,9 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
-      hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+      hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess ||
       hipEventCreate(&ctx->ev1) != hipSuccess || hipMalloc(&ctx->d_counter, 64) != hipSuccess) {
     delete ctx;
     return MLP_EDEVICE;
@@ -396,6 +409,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->tev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
+  if (ctx->side) hipStreamDestroy(ctx->side);
   delete ctx;
   return MLP_OK;
 }
@@ -901,7 +915,7 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
 int mlp_graph_begin(mlp_ctx* ctx) {
   if (!ctx) return MLP_EINVAL;
   if (ctx->capturing) return fail(ctx, MLP_ESTATE, "a capture is already open");
-  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   ctx->user_capture = true;
   ctx->capture_slots.clear();
@@ -1332,7 +1346,7 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
     return MLP_OK;
   }
   hipGraph_t g = nullptr;
-  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   int st = MLP_OK;
   for (int r = 0; r < rounds && st == MLP_OK; ++r) st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
