@@ -166,8 +166,14 @@ struct mlp_ctx {
   bool capturing = false;     // the stream is being captured into a graph
   bool user_capture = false;  // ... by mlp_graph_begin
   std::vector<hipGraphExec_t> user_graphs;
-  std::vector<std::vector<int32_t>> user_graph_slots;  // slots each graph evaluates
-  std::vector<int32_t> capture_slots;
+  // host bookkeeping a graph's replay repeats: (slot, merge?, mask or nb, row map)
+  struct GraphOp {
+    int32_t slot, merge;
+    uint32_t mask;
+    int32_t rowmap;
+  };
+  std::vector<std::vector<GraphOp>> user_graph_ops;
+  std::vector<GraphOp> capture_ops;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
   int graph_uses = 0;
@@ -781,7 +787,9 @@ hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
 // a DVND round passes its own counter, cleared by k_round_begin.
 static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                      int32_t rowmap, int32_t persistent, int skip, int* round_counter) {
-  if (ctx->user_capture) ctx->capture_slots.insert(ctx->capture_slots.end(), slots, slots + count);
+  if (ctx->user_capture)
+    for (int q = 0; q < count; ++q)
+      ctx->capture_ops.push_back({slots[q], 0, nb_mask, rowmap});
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
     if (skip) {
@@ -918,7 +926,7 @@ int mlp_graph_begin(mlp_ctx* ctx) {
   HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   ctx->user_capture = true;
-  ctx->capture_slots.clear();
+  ctx->capture_ops.clear();
   return MLP_OK;
 }
 
@@ -937,21 +945,18 @@ int mlp_graph_end(mlp_ctx* ctx, int32_t* graph) {
   const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (ei != hipSuccess) return fail(ctx, MLP_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
-  std::vector<int32_t> used = ctx->capture_slots;
-  std::sort(used.begin(), used.end());
-  used.erase(std::unique(used.begin(), used.end()), used.end());
   // the recorded evals changed the host's parity choice without running
-  for (int32_t id : used)
-    if (slot_ok(ctx, id)) ctx->slots[id].parity_stale = true;
+  for (const auto& op : ctx->capture_ops)
+    if (slot_ok(ctx, op.slot)) ctx->slots[op.slot].parity_stale = true;
   for (size_t q = 0; q < ctx->user_graphs.size(); ++q)
     if (!ctx->user_graphs[q]) {
       ctx->user_graphs[q] = ex;
-      ctx->user_graph_slots[q] = used;
+      ctx->user_graph_ops[q] = ctx->capture_ops;
       *graph = (int32_t)q;
       return MLP_OK;
     }
   ctx->user_graphs.push_back(ex);
-  ctx->user_graph_slots.push_back(used);
+  ctx->user_graph_ops.push_back(ctx->capture_ops);
   *graph = (int32_t)ctx->user_graphs.size() - 1;
   return MLP_OK;
 }
@@ -962,9 +967,22 @@ int mlp_graph_launch(mlp_ctx* ctx, int32_t graph) {
   if (graph < 0 || graph >= (int)ctx->user_graphs.size() || !ctx->user_graphs[graph])
     return fail(ctx, MLP_EINVAL, "graph %d does not exist", graph);
   HCK(hipGraphLaunch(ctx->user_graphs[graph], ctx->stream));
-  // replays record row-buffer parities the host did not choose
-  for (int32_t id : ctx->user_graph_slots[graph])
-    if (slot_ok(ctx, id)) ctx->slots[id].parity_stale = true;
+  // replays record row-buffer parities the host did not choose; repeat the
+  // evaluated / merged bookkeeping of the recorded calls
+  for (const auto& op : ctx->user_graph_ops[graph]) {
+    if (!slot_ok(ctx, op.slot)) continue;
+    Slot& s = ctx->slots[op.slot];
+    s.parity_stale = true;
+    if (op.merge) {
+      s.merged[op.mask] = true;
+    } else {
+      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+        if (op.mask & (1u << nb)) {
+          s.eval_rowmap[nb] = op.rowmap;
+          s.merged[nb] = false;
+        }
+    }
+  }
   return MLP_OK;
 }
 
@@ -1223,6 +1241,8 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
     in_lds.clear();
   }
   for (int q = 0; q < count; ++q) ctx->slots[slots[q]].merged[nb] = true;
+  if (ctx->user_capture)
+    for (int q = 0; q < count; ++q) ctx->capture_ops.push_back({slots[q], 1, (uint32_t)nb, rowmap});
   for (int g = 0; g < 2; ++g) {
     const std::vector<int32_t>& list = g ? in_gmem : in_lds;
     for (size_t q0 = 0; q0 < list.size(); q0 += 65535) {
