@@ -56,6 +56,56 @@ def cpu_baseline(xs, ys, tours, budget_s=12.0):
                       f"({moves} moves, {dt:.1f} s, oracle eval_rows, 1 thread)"}
 
 
+def batched_cfg2(eng):
+    """256 random-insertion tours (seeds 1..256) of one clustered n=784
+    instance (20 clusters, sigma 40, centres in [0,2000]^2, seed 2), all five
+    neighborhoods in one launch: the autotuner picks the tile shape, the faster
+    of the persistent / one-workgroup-per-tile grids is captured in a HIP graph
+    and replayed."""
+    from wamca_mi import inputs as I
+    n, T = 784, 256
+    xs, ys = I.clustered(n, 2)
+    inst = eng.instance(xs, ys, 1)
+    slots = eng.slots(inst, T)
+    eng.set_tours(slots, [I.random_tour(xs, ys, 1, seed) for seed in range(1, T + 1)])
+    eng.sync()
+    tile, tile_us = eng.autotune(slots, 0x1F, 3)
+    eng.set_tile(tile)
+    grid_us = {}
+    for persistent in (False, True):
+        eng.eval(slots, 0x1F, 0, persistent=persistent)
+        eng.region_begin()
+        for _ in range(3):
+            eng.eval(slots, 0x1F, 0, persistent=persistent)
+        grid_us[persistent] = eng.region_end() * 1e3 / 3
+    persistent = min(grid_us, key=grid_us.get)
+    eng.graph_begin()
+    eng.eval(slots, 0x1F, 0, persistent=persistent)
+    g = eng.graph_end()
+    eng.graph_launch(g)
+    eng.sync()
+    reps = 10
+    t0 = time.perf_counter()
+    eng.region_begin()
+    for _ in range(reps):
+        eng.graph_launch(g)
+    ms = eng.region_end() / reps
+    dt = (time.perf_counter() - t0) / reps
+    eng.graph_destroy(g)
+    eng.set_tile(-1)
+    per_tour = sum(moves_per_nb(n + 1).values())
+    for s in slots:
+        eng.destroy_slot(s)
+    eng.destroy_instance(inst)
+    return {"workload": "n=784 clustered, 256 random-insertion tours, 5 neighborhoods, one launch",
+            "moves_per_launch": per_tour * T, "tile": TILES[tile],
+            "tile_us": {TILES[k]: v for k, v in enumerate(tile_us)},
+            "persistent": persistent, "grid_us": {str(k): v for k, v in grid_us.items()},
+            "graph_replay_ms": ms, "moves_per_s": per_tour * T / dt,
+            "lds_frac": sum(moves_per_nb(n + 1)[nb] * LDS_BYTES[nb] for nb in range(5)) * T
+            / (ms / 1e3) / 1e9 / PEAK_LDS_GBS}
+
+
 def dvnd_legs(eng, args):
     """DVND solves: n=100 alone; 64 uneven instances batched vs one at a time;
     a Poisson job stream served (continuous batching) vs plain."""
@@ -246,6 +296,8 @@ def main():
             if best is None or rec["moves_per_s"] > best["moves_per_s"]:
                 best = rec
         out["batched"] = best
+    if not args.no_batched:
+        out["batched_cfg2"] = batched_cfg2(eng)
     if not args.no_dvnd:
         out["dvnd"] = dvnd_legs(eng, args)
     if not args.no_cpu:
