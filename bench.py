@@ -30,7 +30,8 @@ PEAK_LDS_GBS = CUS * LDS_BYTES_PER_CLK * CLOCK_GHZ  # 78,643 GB/s
 # lane-varying operand bytes per move of the reference's ADS formulation
 # (swap 7 words, 2-opt 5, or-opt 5 forward / 6 backward -> 22 B average)
 LDS_BYTES = {0: 28, 1: 20, 2: 22, 3: 22, 4: 22}
-TILES = {0: "k_sweep<4,256>", 1: "k_sweep<8,512>", 2: "k_sweep<16,1024>"}
+TILES = {0: "k_sweep<4,256>", 1: "k_sweep<8,512>", 2: "k_sweep<16,1024>", 3: "k_sweep<8,256>",
+         4: "k_sweep<4,512>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_n1000_traffic.json")
 
 
@@ -220,7 +221,7 @@ def main():
     tile = args.tile
     if tile < 0:
         probe = slots[:min(T, 128)]
-        for t in (0, 1, 2):
+        for t in TILES:
             eng.set_tile(t)
             eng.eval_each(probe, 0x1F, 0)
             eng.region_begin()

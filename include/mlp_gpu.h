@@ -172,8 +172,8 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
 
 /* Sweep tile shape for mlp_eval with the ADS row map: -1 automatic (chosen
  * from the launch's total move count, or by the table mlp_autotune fills),
- * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024.  Results do not
- * depend on the shape. */
+ * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024, 3 = 8 x 256,
+ * 4 = 4 x 512.  Results do not depend on the shape. */
 /* Capture of the context stream into a replayable HIP graph: between
  * mlp_graph_begin and mlp_graph_end, mlp_eval, mlp_eval_each, mlp_merge_apply
  * and mlp_dvnd_rounds are recorded instead of run (an eval also records the
@@ -195,7 +195,7 @@ int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
  * buffers are overwritten), records the fastest for (nb_mask, log2 of the
  * launch's move count) in the context's table, which the automatic shape
  * choice consults (nearest tuned size of the same mask).  us_per_launch, if
- * given, receives 3 timings (one per shape). */
+ * given, receives 5 timings (one per shape). */
 int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                  int32_t reps, int32_t* best_tile, float* us_per_launch);
 /* The table: 32 masks x 48 log2-size buckets of int8 (-1 = untuned). */

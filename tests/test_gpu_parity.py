@@ -298,7 +298,7 @@ def test_rows_int64_path(eng, n, span):
             _check_rows(eng, s, W, t, nb, rowmap)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("n,span", [(5, 2000), (9, 2000), (300, 2000), (1000, 2000),
                                     (2000, 2000), (700, 30000)])
 def test_tile_shapes_and_row_buffers(eng, tile, n, span):

@@ -92,13 +92,15 @@ struct SweepSeg {
   int32_t parity;  // bit nb: buffer written by a standalone eval
 };
 #define MLP_SWEEPSEG_INTS 6
-// Tile shapes (rows x columns) of the sweep: 4 x 256 (most workgroups, for
-// launches of one or a few tours), 8 x 512, 16 x 1024 (fewest distance
+// Tile shapes (rows x columns) of the sweep: from 4 x 256 (most workgroups,
+// for launches of one or a few tours) to 16 x 1024 (fewest distance
 // evaluations per move, for large batches).
 #define MLP_TILE_4x256 0
 #define MLP_TILE_8x512 1
 #define MLP_TILE_16x1024 2
-#define MLP_TILE_COUNT 3
+#define MLP_TILE_8x256 3
+#define MLP_TILE_4x512 4
+#define MLP_TILE_COUNT 5
 
 #define MLP_SEGS_INLINE 32
 struct EvalSegsInline {

@@ -224,7 +224,7 @@ __device__ __forceinline__ float round_dr(int round) { return round ? 0.5F : 0.0
 // One row: lanes stride over the row-local index, keep the smallest key
 // (delta << 20 | ctx) -> lowest delta, then lowest index; wave reduction.
 template <int KIND, bool P32>
-__device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane, int64_t* out) {
+__device__ __forceinline__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane, int64_t* out) {
   const int N = s.N;
   const float dr = round_dr(s.round);
   int lo, hi;
@@ -246,7 +246,7 @@ __device__ void eval_row(const SlotDev& s, int nb, int rowmap, int row, int lane
 }
 
 template <bool P32>
-__device__ void eval_row_dispatch(const SlotDev& s, int nb, int rowmap, int row, int lane,
+__device__ __forceinline__ void eval_row_dispatch(const SlotDev& s, int nb, int rowmap, int row, int lane,
                                   int64_t* out) {
   if (nb == 0) eval_row<0, P32>(s, nb, rowmap, row, lane, out);
   else if (nb == 1) eval_row<1, P32>(s, nb, rowmap, row, lane, out);
@@ -273,7 +273,7 @@ __device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, in
   return sg;
 }
 
-__device__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
+__device__ __forceinline__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
                           int rowmap, bool dvnd) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int row = (item - sg.blk0) * MLP_EVAL_ROWS_PER_WG + wave;
@@ -977,7 +977,7 @@ __device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int l) {
 
 // Reset of the buffer holding the previous result (par ^ 1) by the tiles of
 // column 0; the first tile also clears both tails beyond this row map.
-__device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
+__device__ __forceinline__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
   int64_t* o = row_buf(s, nb, par ^ 1);
   for (int r = r0 + (int)threadIdx.x; r < r1; r += kBlock) o[r] = MLP_KEY_NONE;
   if (first) {
@@ -993,7 +993,7 @@ __device__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, b
 // column.  Row r: i = b0+r+1, j in [i+2, N-2].  delta = Cs_r + Bs(j) +
 // (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j] + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
 template <int R, int C, int RW>
-__device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
+__device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
                           int parity, int wave, int lane) {
   const int N = s.N;
   const int r0 = wave * RW;
@@ -1053,7 +1053,7 @@ __device__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, T
 // forward i = by+1 for j >= by+2, backward i = by+2 for j <= by+1, j in
 // [1, N-K-1]; row-local index j - 1.
 template <int R, int C, int RW, int K>
-__device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
+__device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
                            int parity, int wave, int lane) {
   const int N = s.N;
   const float dr = round_dr(s.round);
@@ -1164,7 +1164,7 @@ __device__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, 
 // int64 fallback of a fixed-i tile (column tile 0 only): one wave per
 // (neighborhood, row) over the whole row.
 template <int R>
-__device__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int parity) {
+__device__ __forceinline__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int parity) {
   const int N = s.N, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int b0 = rb * R;
   if (ct != 0) return;
@@ -1183,7 +1183,7 @@ __device__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int
 }
 
 template <int R, int C>
-__device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
+__device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
                             TileLds<R, C>& L) {
   constexpr int RW = R / (kBlock / kWave);
   constexpr int FI = (C + 4 + kBlock - 1) / kBlock;  // fill iterations per thread
@@ -1248,7 +1248,7 @@ __device__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int pari
   if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane);
 }
 
-__device__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
+__device__ __forceinline__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int rows = (s.N + 1) >> 1;
   const int par = (parity >> 1) & 1;
@@ -1259,7 +1259,7 @@ __device__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
 }
 
 template <int C>
-__device__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
+__device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
   constexpr int FE = (C + 1 + kWave - 1) / kWave;  // e entries per lane and part
   constexpr int CU = 4;                            // compute iterations per load batch
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
@@ -1370,7 +1370,7 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
 }
 
 template <int R, int C>
-__device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
+__device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
                            SweepLds<R, C>& L, bool dvnd) {
   const SlotDev& s = table[sg.slot];
   if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
@@ -1395,8 +1395,12 @@ __device__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg
 
 }  // namespace
 
+// Resident workgroups per CU the register budget is held to: the small
+// shapes serve one-tour launches, whose latency depends on every tile being
+// resident at once.
 template <int R, int C>
-__global__ __launch_bounds__(kBlock) void k_sweep(const SlotDev* __restrict__ table,
+__global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 5 : 1)) void k_sweep(
+    const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter,
                                                   int skip_inactive) {
@@ -1426,6 +1430,10 @@ template __global__ void k_sweep<8, 512>(const SlotDev*, SweepSegsInline, const 
                                          int, int*, int);
 template __global__ void k_sweep<16, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
                                            int, int*, int);
+template __global__ void k_sweep<8, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                         int, int*, int);
+template __global__ void k_sweep<4, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
+                                         int, int*, int);
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -1472,21 +1480,25 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid, int skip_inactive) {
-  if (tile == MLP_TILE_4x256)
-    hipLaunchKernelGGL((k_sweep<4, 256>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter, skip_inactive);
-  else if (tile == MLP_TILE_8x512)
-    hipLaunchKernelGGL((k_sweep<8, 512>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter, skip_inactive);
-  else
-    hipLaunchKernelGGL((k_sweep<16, 1024>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,
-                       nsegs, total, counter, skip_inactive);
+#define MLP_SWEEP_LAUNCH(R, C)                                                                 \
+  hipLaunchKernelGGL((k_sweep<R, C>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,   \
+                     nsegs, total, counter, skip_inactive)
+  switch (tile) {
+    case MLP_TILE_4x256: MLP_SWEEP_LAUNCH(4, 256); break;
+    case MLP_TILE_8x512: MLP_SWEEP_LAUNCH(8, 512); break;
+    case MLP_TILE_8x256: MLP_SWEEP_LAUNCH(8, 256); break;
+    case MLP_TILE_4x512: MLP_SWEEP_LAUNCH(4, 512); break;
+    default: MLP_SWEEP_LAUNCH(16, 1024); break;
+  }
+#undef MLP_SWEEP_LAUNCH
   return hipGetLastError();
 }
 
 void mlp_tile_shape(int tile, int* rows, int* cols) {
-  *rows = tile == MLP_TILE_4x256 ? 4 : tile == MLP_TILE_8x512 ? 8 : 16;
-  *cols = tile == MLP_TILE_4x256 ? 256 : tile == MLP_TILE_8x512 ? 512 : 1024;
+  static const int kRows[MLP_TILE_COUNT] = {4, 8, 16, 8, 4};
+  static const int kCols[MLP_TILE_COUNT] = {256, 512, 1024, 256, 512};
+  *rows = kRows[tile];
+  *cols = kCols[tile];
 }
 
 // keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16

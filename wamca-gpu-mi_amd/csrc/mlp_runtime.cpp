@@ -731,7 +731,7 @@ hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
 }
 
 // Resident workgroups per CU of each sweep tile shape (persistent grids).
-const int kTileWgPerCu[MLP_TILE_COUNT] = {5, 4, 3};
+const int kTileWgPerCu[MLP_TILE_COUNT] = {5, 4, 3, 4, 5};
 
 // Moves one evaluation of `mask` over the slots covers (ADS row map).
 int64_t launch_moves(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
@@ -1064,7 +1064,7 @@ int mlp_tune_set(mlp_ctx* ctx, const int8_t* table) {
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
   if (!ctx) return MLP_EINVAL;
   if (tile < -1 || tile >= MLP_TILE_COUNT)
-    return fail(ctx, MLP_EINVAL, "tile must be -1 (auto), 0 (4x256), 1 (8x512) or 2 (16x1024)");
+    return fail(ctx, MLP_EINVAL, "tile must be -1 (auto) or a shape 0..%d", MLP_TILE_COUNT - 1);
   ctx->tile_override = tile;
   return MLP_OK;
 }

@@ -16,6 +16,7 @@ ROWMAP_ADS, ROWMAP_ODC = 0, 1
 
 
 MAX_N = 32768  # MLP_MAX_N: largest tour size N (positions)
+N_TILES = 5    # sweep tile shapes
 
 
 def rows_of(nb: int, rowmap: int, N: int) -> int:
@@ -239,7 +240,8 @@ class Engine:
         return out.value
 
     def set_tile(self, tile: int):
-        """Sweep tile shape: -1 automatic, 0 = 4x256, 1 = 8x512, 2 = 16x1024."""
+        """Sweep tile shape: -1 automatic, 0 = 4x256, 1 = 8x512, 2 = 16x1024,
+        3 = 8x256, 4 = 4x512."""
         self._ck(self._L.mlp_set_tile(self.h, tile))
 
     def autotune(self, slots, nb_mask=0x1F, reps=3):
@@ -247,7 +249,7 @@ class Engine:
         context's table.  Returns (best shape, [us per launch per shape])."""
         slots = np.ascontiguousarray(slots, np.int32)
         best = C.c_int32()
-        us = (C.c_float * 3)()
+        us = (C.c_float * N_TILES)()
         self._ck(self._L.mlp_autotune(self.h, slots, len(slots), nb_mask, reps, C.byref(best), us))
         return best.value, list(us)
 
