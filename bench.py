@@ -142,16 +142,19 @@ def dvnd_legs(eng, args):
         "ratio": sv["solves_per_s"] / pl["solves_per_s"],
         "served_makespan_s": sv["makespan_s"], "plain_makespan_s": pl["makespan_s"],
         "identical_results": sv["latency"] == pl["latency"]}
-    # Poisson streams: n log-uniform in [100, 1000]; and [50, 2000] with a
-    # fraction of clustered n = 18000 jobs
-    for name, lo, hi, big in (("stream_100_1000", 100, 1000, 0.0),
-                              ("stream_50_2000_big", 50, 2000, args.stream_big_frac)):
+    # n log-uniform in [100, 1000]: all jobs at t = 0 (capacity) and a Poisson
+    # stream; n in [50, 2000] with a fraction of clustered n = 18000 jobs
+    for name, lo, hi, big, rate in (("burst_100_1000", 100, 1000, 0.0, 0.0),
+                                    ("stream_100_1000", 100, 1000, 0.0, args.stream_rate),
+                                    ("stream_50_2000_big", 50, 2000, args.stream_big_frac,
+                                     args.stream_rate)):
         jobs, meta = WL.dvnd_jobs(args.stream_jobs, 4, lo=lo, hi=hi, big_frac=big)
-        arr = WL.poisson_arrivals(args.stream_rate, args.stream_jobs, 40)
+        arr = (WL.poisson_arrivals(rate, args.stream_jobs, 40) if rate > 0
+               else [0.0] * args.stream_jobs)
         sv = serve.run_served(eng, jobs, arr)
         pl = serve.run_plain(eng, jobs, arr)
         res[name] = {
-            "jobs": args.stream_jobs, "rate_per_s": args.stream_rate,
+            "jobs": args.stream_jobs, "rate_per_s": rate if rate > 0 else "all at t=0",
             "n": f"log-uniform [{lo}, {hi}], half uniform [0,1000]^2, half clustered; "
                  f"{sum(m['n'] == 18000 for m in meta)} jobs clustered n = 18000",
             "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},

@@ -74,6 +74,11 @@ int mlp_ctx_create(int device, mlp_ctx** out);
 int mlp_ctx_destroy(mlp_ctx* ctx);
 const char* mlp_ctx_error(const mlp_ctx* ctx);
 const char* mlp_strerror(int status);
+/* The HIP device of the context. */
+int mlp_ctx_device(mlp_ctx* ctx, int32_t* device);
+/* Without waiting: *idle = 1 when every call queued on the context stream has
+ * finished on the device. */
+int mlp_query(mlp_ctx* ctx, int32_t* idle);
 int mlp_sync(mlp_ctx* ctx);
 /* Device memory held by instances and slots (live) and reserved by the
  * context's pool (released blocks are reused, not returned to HIP). */

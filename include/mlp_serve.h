@@ -3,13 +3,17 @@
  * (libwamca_mi.so).
  *
  * Jobs (an instance, a start tour) are submitted at any time and queue on the
- * host.  Each mlp_server_step admits queued jobs in arrival order while their
- * device footprint (coordinates + one slot: tour, ADS prefix arrays, row
- * buffers) fits the memory budget and the active-job limit, enqueues a chunk
- * of DVND rounds over every resident job in one set of launches (jobs of any
- * n share the launches; a finished job is skipped on the device), then
- * retires the jobs that reached their local optimum and releases their
- * memory to the context's pool.  A job's result does not depend on the other
+ * host.  Each mlp_server_step retires the jobs that reached their local
+ * optimum (releasing their memory to the pool), admits queued jobs in arrival
+ * order while their device footprint (coordinates + one slot: tour, ADS
+ * prefix arrays, row buffers, merge scratch) fits the memory budget and the
+ * active-job limit, and enqueues a chunk of DVND rounds over every resident
+ * job in one set of launches (jobs of any n share the launches; a finished
+ * job is skipped on the device).  Jobs of n >= big_n run in a second lane
+ * with its own context and HIP stream, so the GPU runs both lanes' kernels
+ * side by side; the first lane is waited for at each step, the second only
+ * polled, and a queued job whose lane is busy lets later jobs of the other
+ * lane go first.  A job's result does not depend on the other
  * jobs it ran beside: every kernel works on one job's slot, and a job's
  * rounds are the same rounds mlp_dvnd runs for it alone.
  *
@@ -36,6 +40,8 @@ typedef struct {
   int32_t rowmap;          /* MLP_ROWMAP_ADS or MLP_ROWMAP_ODC */
   int32_t rounds_per_step; /* DVND rounds per step (<= 0: 2) */
   int32_t max_rounds;      /* per job, 0 = to the local optimum */
+  int32_t big_n;           /* jobs of n >= big_n run in a second lane (own context and
+                              stream, same GPU); 0: 4096, < 0: one lane */
 } mlp_server_config;
 
 typedef struct {

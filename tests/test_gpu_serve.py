@@ -69,3 +69,13 @@ def test_server_rejects_oversized_job(eng):
             srv.submit(xs, ys, t)
     finally:
         srv.close()
+
+
+def test_two_lanes_equal_alone(eng):
+    """Jobs at or above big_n run in the second lane (own context and stream);
+    every result still equals the oracle's DVND of the job alone."""
+    ns = [60, 700, 90, 1500, 140, 75, 900, 120]
+    jobs, refs = _jobs(ns, 8300)
+    arrivals = [0.0, 0.0, 0.001, 0.001, 0.002, 0.003, 0.003, 0.004]
+    out = serve.run_served(eng, jobs, arrivals, big_n=600)
+    _check(out, refs)

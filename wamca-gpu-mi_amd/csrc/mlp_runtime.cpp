@@ -420,6 +420,26 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   return MLP_OK;
 }
 
+int mlp_ctx_device(mlp_ctx* ctx, int32_t* device) {
+  if (!ctx || !device) return MLP_EINVAL;
+  *device = ctx->device;
+  return MLP_OK;
+}
+
+int mlp_query(mlp_ctx* ctx, int32_t* idle) {
+  if (!ctx || !idle) return MLP_EINVAL;
+  const hipError_t e = hipStreamQuery(ctx->stream);
+  if (e == hipSuccess) {
+    *idle = 1;
+  } else if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *idle = 0;
+  } else {
+    return fail(ctx, MLP_EDEVICE, "stream query: %s", hipGetErrorString(e));
+  }
+  return MLP_OK;
+}
+
 int mlp_sync(mlp_ctx* ctx) {
   NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;

@@ -1,11 +1,17 @@
 // mlp_serve.cpp -- continuous-batching DVND server (include/mlp_serve.h) over
 // the public C ABI of the engine.  Host-side bookkeeping only: admission by
-// device footprint, one chunk of DVND rounds per step over every resident
-// job, retirement and result hand-off.
+// device footprint, one chunk of DVND rounds per step over every resident job
+// of a lane, retirement and result hand-off.
+//
+// Two lanes: jobs below cfg.big_n share the caller's context; larger jobs run
+// on a second context (its own HIP stream) on the same GPU, so the short
+// rounds of small jobs do not queue behind the long kernels of large ones.
+// The small lane is waited for at each step; the large lane is only polled.
 #include "mlp_serve.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <new>
@@ -21,6 +27,7 @@ struct Job {
   int32_t inst = -1, slot = -1;
   int64_t bytes = 0, latency = 0;
   int32_t rounds = 0;
+  int lane = 0;
 };
 
 // The pool rounds every block up to a power of two of at least 4 KiB.
@@ -30,17 +37,22 @@ int64_t pow2_class(int64_t b) {
   return c;
 }
 
+struct Lane {
+  mlp_ctx* ctx = nullptr;
+  bool owned = false;
+  std::vector<Job*> active;
+  bool chunk_pending = false;
+};
+
 }  // namespace
 
 struct mlp_server {
-  mlp_ctx* ctx = nullptr;
   mlp_server_config cfg{};
+  Lane lane[2];
   std::deque<Job*> queued;
-  std::vector<Job*> active;
   std::vector<Job*> done;    // finished, not yet polled
   std::vector<Job*> handed;  // returned by the last poll
   int64_t used = 0, next_id = 1;
-  bool chunk_pending = false;
   std::chrono::steady_clock::time_point t0;
 };
 
@@ -62,32 +74,38 @@ int mlp_server_create(mlp_ctx* ctx, const mlp_server_config* cfg, mlp_server** o
   if (!ctx || !out) return MLP_EINVAL;
   mlp_server* s = new (std::nothrow) mlp_server();
   if (!s) return MLP_ENOMEM;
-  s->ctx = ctx;
   if (cfg) s->cfg = *cfg;
   if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
   if (s->cfg.rounds_per_step <= 0) s->cfg.rounds_per_step = 2;
+  if (s->cfg.big_n == 0) s->cfg.big_n = 4096;
   if (s->cfg.rowmap != MLP_ROWMAP_ADS && s->cfg.rowmap != MLP_ROWMAP_ODC) {
     delete s;
     return MLP_EINVAL;
   }
+  s->lane[0].ctx = ctx;
   s->t0 = std::chrono::steady_clock::now();
   *out = s;
   return MLP_OK;
 }
 
 static void release(mlp_server* s, Job* j) {
-  if (j->slot >= 0) mlp_slot_destroy(s->ctx, j->slot);
-  if (j->inst >= 0) mlp_instance_destroy(s->ctx, j->inst);
+  mlp_ctx* ctx = s->lane[j->lane].ctx;
+  if (j->slot >= 0) mlp_slot_destroy(ctx, j->slot);
+  if (j->inst >= 0) mlp_instance_destroy(ctx, j->inst);
   j->slot = j->inst = -1;
 }
 
 int mlp_server_destroy(mlp_server* s) {
   if (!s) return MLP_EINVAL;
-  mlp_sync(s->ctx);
-  for (Job* j : s->active) {
-    release(s, j);
-    delete j;
+  for (Lane& l : s->lane) {
+    if (!l.ctx) continue;
+    mlp_sync(l.ctx);
+    for (Job* j : l.active) {
+      release(s, j);
+      delete j;
+    }
   }
+  if (s->lane[1].owned) mlp_ctx_destroy(s->lane[1].ctx);
   for (Job* j : s->queued) delete j;
   for (Job* j : s->done) delete j;
   for (Job* j : s->handed) delete j;
@@ -111,84 +129,126 @@ int mlp_server_submit(mlp_server* s, const int32_t* xs, const int32_t* ys, int32
   j->ys.assign(ys, ys + n);
   j->tour.assign(tour, tour + n + 1);
   j->bytes = bytes;
+  j->lane = (s->cfg.big_n > 0 && n >= s->cfg.big_n) ? 1 : 0;
   j->t_submit = mlp_server_now(s);
   s->queued.push_back(j);
   if (job) *job = j->id;
   return MLP_OK;
 }
 
+// Retires the finished jobs of a lane whose last chunk is done.
+static int retire(mlp_server* s, Lane& l) {
+  if (!l.chunk_pending || l.active.empty()) {
+    l.chunk_pending = false;
+    return MLP_OK;
+  }
+  const int c = (int)l.active.size();
+  std::vector<int32_t> slots((size_t)c), act((size_t)c), rnd((size_t)c);
+  for (int q = 0; q < c; ++q) slots[(size_t)q] = l.active[(size_t)q]->slot;
+  int st = mlp_dvnd_status(l.ctx, slots.data(), c, act.data(), rnd.data());
+  if (st) return st;
+  const double now = mlp_server_now(s);
+  std::vector<Job*> still;
+  for (int q = 0; q < c; ++q) {
+    Job* j = l.active[(size_t)q];
+    if (act[(size_t)q]) {
+      still.push_back(j);
+      continue;
+    }
+    j->rounds = rnd[(size_t)q];
+    st = mlp_get_tours(l.ctx, &j->slot, 1, j->tour.data());
+    if (st == MLP_OK) st = mlp_get_latency(l.ctx, &j->slot, 1, &j->latency);
+    if (st) return st;
+    j->t_finish = now;
+    release(s, j);
+    s->used -= j->bytes;
+    s->done.push_back(j);
+  }
+  l.active.swap(still);
+  l.chunk_pending = false;
+  return MLP_OK;
+}
+
 int mlp_server_step(mlp_server* s, int32_t* active_out, int32_t* queued_out) {
   if (!s) return MLP_EINVAL;
   int st = MLP_OK;
-  // 1. retire the jobs the previous chunk finished
-  if (s->chunk_pending && !s->active.empty()) {
-    const int c = (int)s->active.size();
-    std::vector<int32_t> slots((size_t)c), act((size_t)c), rnd((size_t)c);
-    for (int q = 0; q < c; ++q) slots[(size_t)q] = s->active[(size_t)q]->slot;
-    st = mlp_dvnd_status(s->ctx, slots.data(), c, act.data(), rnd.data());
+  // 1. retire: the small lane is waited for, the large lane only polled
+  bool ready[2] = {true, true};
+  if (s->lane[1].ctx && s->lane[1].chunk_pending) {
+    int32_t idle = 0;
+    st = mlp_query(s->lane[1].ctx, &idle);
     if (st) return st;
-    const double now = mlp_server_now(s);
-    std::vector<Job*> still;
-    for (int q = 0; q < c; ++q) {
-      Job* j = s->active[(size_t)q];
-      if (act[(size_t)q]) {
-        still.push_back(j);
-        continue;
-      }
-      j->rounds = rnd[(size_t)q];
-      st = mlp_get_tours(s->ctx, &j->slot, 1, j->tour.data());
-      if (st == MLP_OK) st = mlp_get_latency(s->ctx, &j->slot, 1, &j->latency);
-      if (st) return st;
-      j->t_finish = now;
-      release(s, j);
-      s->used -= j->bytes;
-      s->done.push_back(j);
-    }
-    s->active.swap(still);
+    ready[1] = idle != 0;
   }
-  s->chunk_pending = false;
+  for (int L = 0; L < 2; ++L)
+    if (s->lane[L].ctx && ready[L]) {
+      st = retire(s, s->lane[L]);
+      if (st) return st;
+    }
   // 2. admission in arrival order while the footprint fits
-  std::vector<Job*> admitted;
-  while (!s->queued.empty()) {
-    Job* j = s->queued.front();
-    if (s->cfg.max_active > 0 && (int)(s->active.size() + admitted.size()) >= s->cfg.max_active)
-      break;
-    if (s->used + j->bytes > s->cfg.hbm_budget) break;
-    st = mlp_instance_create_async(s->ctx, j->xs.data(), j->ys.data(), j->n, j->round, &j->inst);
-    if (st == MLP_OK) st = mlp_slot_create(s->ctx, j->inst, &j->slot);
+  std::vector<Job*> admitted[2];
+  const int total_active = (int)(s->lane[0].active.size() + s->lane[1].active.size());
+  int added = 0;
+  for (auto it = s->queued.begin(); it != s->queued.end();) {
+    Job* j = *it;
+    if (s->cfg.max_active > 0 && total_active + added >= s->cfg.max_active) break;
+    if (s->used + j->bytes > s->cfg.hbm_budget) break;  // memory is granted in arrival order
+    if (!ready[j->lane]) {  // its lane is still running a chunk: later jobs may go first
+      ++it;
+      continue;
+    }
+    Lane& l = s->lane[j->lane];
+    if (!l.ctx) {  // the large lane is created on first use, on the same device
+      int32_t dev = 0;
+      st = mlp_ctx_device(s->lane[0].ctx, &dev);
+      if (st == MLP_OK) st = mlp_ctx_create(dev, &l.ctx);
+      if (st) return st;
+      l.owned = true;
+      int8_t tune[MLP_TUNE_TABLE_BYTES];  // same tile choices as the first lane
+      st = mlp_tune_get(s->lane[0].ctx, tune);
+      if (st == MLP_OK) st = mlp_tune_set(l.ctx, tune);
+      if (st) return st;
+    }
+    st = mlp_instance_create_async(l.ctx, j->xs.data(), j->ys.data(), j->n, j->round, &j->inst);
+    if (st == MLP_OK) st = mlp_slot_create(l.ctx, j->inst, &j->slot);
     if (st) {
       release(s, j);
       return st;
     }
-    s->queued.pop_front();
+    it = s->queued.erase(it);
     s->used += j->bytes;
-    admitted.push_back(j);
+    admitted[j->lane].push_back(j);
+    ++added;
   }
-  if (!admitted.empty()) {
+  for (int L = 0; L < 2; ++L) {
+    if (admitted[L].empty()) continue;
+    Lane& l = s->lane[L];
     std::vector<int32_t> slots, tours;
-    for (Job* j : admitted) {
+    for (Job* j : admitted[L]) {
       slots.push_back(j->slot);
       tours.insert(tours.end(), j->tour.begin(), j->tour.end());
     }
-    st = mlp_set_tours(s->ctx, slots.data(), (int32_t)slots.size(), tours.data());
-    if (st == MLP_OK) st = mlp_dvnd_start(s->ctx, slots.data(), (int32_t)slots.size());
+    st = mlp_set_tours(l.ctx, slots.data(), (int32_t)slots.size(), tours.data());
+    if (st == MLP_OK) st = mlp_dvnd_start(l.ctx, slots.data(), (int32_t)slots.size());
     if (st) return st;
     const double now = mlp_server_now(s);
-    for (Job* j : admitted) {
+    for (Job* j : admitted[L]) {
       j->t_admit = now;
-      s->active.push_back(j);
+      l.active.push_back(j);
     }
   }
-  // 3. one chunk of rounds over every resident job
-  if (!s->active.empty()) {
+  // 3. one chunk of rounds over every resident job of each ready lane
+  for (int L = 0; L < 2; ++L) {
+    Lane& l = s->lane[L];
+    if (!ready[L] || l.active.empty()) continue;
     std::vector<int32_t> slots;
-    for (Job* j : s->active) slots.push_back(j->slot);
-    st = mlp_dvnd_rounds(s->ctx, slots.data(), (int32_t)slots.size(), s->cfg.rowmap,
+    for (Job* j : l.active) slots.push_back(j->slot);
+    st = mlp_dvnd_rounds(l.ctx, slots.data(), (int32_t)slots.size(), s->cfg.rowmap,
                          s->cfg.rounds_per_step, s->cfg.max_rounds);
     if (st) return st;
-    s->chunk_pending = true;
+    l.chunk_pending = true;
   }
-  if (active_out) *active_out = (int32_t)s->active.size();
+  if (active_out) *active_out = (int32_t)(s->lane[0].active.size() + s->lane[1].active.size());
   if (queued_out) *queued_out = (int32_t)s->queued.size();
   return MLP_OK;
 }

@@ -33,7 +33,7 @@ class MoveRec(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("rowmap", C.c_int32),
-                ("rounds_per_step", C.c_int32), ("max_rounds", C.c_int32)]
+                ("rounds_per_step", C.c_int32), ("max_rounds", C.c_int32), ("big_n", C.c_int32)]
 
 
 class JobResult(C.Structure):
@@ -52,6 +52,8 @@ _SIGS = {
     "mlp_ctx_error": (C.c_char_p, [C.c_void_p]),
     "mlp_strerror": (C.c_char_p, [C.c_int]),
     "mlp_sync": (C.c_int, [C.c_void_p]),
+    "mlp_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "mlp_ctx_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "mlp_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                   C.c_char_p, C.c_int32]),
     "mlp_instance_create": (C.c_int, [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32,

@@ -21,11 +21,12 @@ from .engine import ROWMAP_ADS
 
 class Server:
     def __init__(self, eng, hbm_budget=0, max_active=0, rowmap=ROWMAP_ADS, rounds_per_step=2,
-                 max_rounds=0):
+                 max_rounds=0, big_n=0):
+        """big_n: jobs of n >= big_n run in a second lane (0: 4096, < 0: one lane)."""
         self.eng = eng
         self._L = eng._L
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(rowmap), int(rounds_per_step),
-                           int(max_rounds))
+                           int(max_rounds), int(big_n))
         h = C.c_void_p()
         eng._ck(self._L.mlp_server_create(eng.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -84,9 +85,9 @@ def _summary(lat, span, count):
 
 
 def run_served(eng, jobs, arrivals, rnd=1, rowmap=ROWMAP_ADS, hbm_budget=0, max_active=0,
-               rounds_per_step=2):
+               rounds_per_step=2, big_n=0):
     """jobs: list of (xs, ys, tour); arrivals: seconds from the start (sorted)."""
-    srv = Server(eng, hbm_budget, max_active, rowmap, rounds_per_step)
+    srv = Server(eng, hbm_budget, max_active, rowmap, rounds_per_step, big_n=big_n)
     try:
         t0 = srv.now()
         nxt, results, active, queued = 0, {}, 0, 0
