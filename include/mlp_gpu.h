@@ -179,6 +179,15 @@ int mlp_dist_list(mlp_ctx* ctx, const int32_t* s, int32_t count, int32_t round, 
  * from the launch's total move count, or by the table mlp_autotune fills),
  * 0 = 4 rows x 256 columns, 1 = 8 x 512, 2 = 16 x 1024, 3 = 8 x 256,
  * 4 = 4 x 512.  Results do not depend on the shape. */
+/* Or-opt-k beyond k = 3: neighborhood id k + 1, up to id 14 (k = 13, the
+ * largest id MLMove64 holds).  Slots created after mlp_set_neighborhoods(ctx,
+ * count) (5 <= count <= 15) have row buffers for ids 0 .. count-1; their
+ * evaluation of ids >= 5 runs one wave per row.  mlp_dvnd_neighborhoods sets
+ * the ids a DVND round runs (0 .. count-1, default 5).  mlp_slot_bytes gives
+ * the device bytes of a slot created now (mlp_slot_footprint assumes 5). */
+int mlp_set_neighborhoods(mlp_ctx* ctx, int32_t count);
+int mlp_dvnd_neighborhoods(mlp_ctx* ctx, int32_t count);
+int mlp_slot_bytes(mlp_ctx* ctx, int32_t n, int64_t* bytes);
 /* Capture of the context stream into a replayable HIP graph: between
  * mlp_graph_begin and mlp_graph_end, mlp_eval, mlp_eval_each, mlp_merge_apply
  * and mlp_dvnd_rounds are recorded instead of run (an eval also records the

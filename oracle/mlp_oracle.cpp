@@ -661,16 +661,18 @@ int or_step(int nb, int rowmap, const int32_t* W, int n, int32_t* tour, int N, i
   return c;
 }
 
-// DVND fixed point: rounds over {swap, 2-opt, or-opt-1,2,3}, each neighborhood
-// chained on the tour left by the previous one (WamcaExperiment.hpp:414-416,
-// 360-362), until a round applies no move.  Returns the final latency.
-int64_t or_dvnd(int rowmap, const int32_t* W, int n, int32_t* tour, int N, int max_rounds,
-                int32_t* rounds_out, int64_t* moves_out) {
+// DVND fixed point: rounds over neighborhood ids 0 .. nbs-1 ({swap, 2-opt,
+// or-opt-1, 2, 3, ...}), each chained on the tour left by the previous one
+// (WamcaExperiment.hpp:414-416, 360-362; kMax neighborhoods,
+// WamcaExperiment.hpp:126-129), until a round applies no move.  Returns the
+// final latency.
+int64_t or_dvnd_nbs(int rowmap, const int32_t* W, int n, int32_t* tour, int N, int max_rounds,
+                    int nbs, int32_t* rounds_out, int64_t* moves_out) {
   int rounds = 0;
   int64_t moves = 0;
   while (max_rounds <= 0 || rounds < max_rounds) {
     int applied = 0;
-    for (int nb = 0; nb < 5; ++nb) {
+    for (int nb = 0; nb < nbs; ++nb) {
       if (rows_of(nb, rowmap, N) <= 0) continue;
       int64_t s;
       applied += or_step(nb, rowmap, W, n, tour, N, &s);
@@ -682,6 +684,11 @@ int64_t or_dvnd(int rowmap, const int32_t* W, int n, int32_t* tour, int N, int m
   if (rounds_out) *rounds_out = rounds;
   if (moves_out) *moves_out = moves;
   return latency(W, n, tour, N);
+}
+
+int64_t or_dvnd(int rowmap, const int32_t* W, int n, int32_t* tour, int N, int max_rounds,
+                int32_t* rounds_out, int64_t* moves_out) {
+  return or_dvnd_nbs(rowmap, W, n, tour, N, max_rounds, 5, rounds_out, moves_out);
 }
 
 }  // extern "C"

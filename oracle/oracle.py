@@ -72,6 +72,8 @@ def lib():
             "or_can_merge": (C.c_int, [C.c_int] * 5),
             "or_step": (C.c_int, [C.c_int, C.c_int, I32P, C.c_int, I32P, C.c_int, I64P]),
             "or_dvnd": (C.c_int64, [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int, I32P, I64P]),
+            "or_dvnd_nbs": (C.c_int64, [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int, C.c_int,
+                                        I32P, I64P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -241,11 +243,12 @@ def step(nb, rowmap, W, tour):
     return t, c, int(s[0])
 
 
-def dvnd(W, tour, rowmap=0, max_rounds=0):
-    """DVND to a local optimum: (final tour, latency, rounds, moves)."""
+def dvnd(W, tour, rowmap=0, max_rounds=0, nbs=5):
+    """DVND over neighborhood ids 0..nbs-1 to a local optimum:
+    (final tour, latency, rounds, moves)."""
     Wf, n, t = _wt(W, tour)
     t = t.copy()
     r = np.zeros(1, np.int32)
     m = np.zeros(1, np.int64)
-    L = lib().or_dvnd(rowmap, Wf, n, t, len(t), max_rounds, r, m)
+    L = lib().or_dvnd_nbs(rowmap, Wf, n, t, len(t), max_rounds, nbs, r, m)
     return t, int(L), int(r[0]), int(m[0])

@@ -185,3 +185,28 @@ def test_int32_exactness_bounds(n, seed, extreme):
             # the or-opt bound is not loose by orders of magnitude
             if nb >= 2 and extreme:
                 assert top * 20 >= bound, (nb, top, bound)
+
+
+@pytest.mark.parametrize("n,seed", [(9, 11), (16, 12), (30, 13), (41, 14)])
+@pytest.mark.parametrize("rowmap", [0, 1])
+def test_oropt_k_beyond_3(n, seed, rowmap):
+    """Or-opt-k for k = 4..8 (ids 5..9): the ADS and ODC closed forms equal
+    brute force, the row maps cover the move set, and a DVND over ids 0..9
+    ends with no improving move in any of them."""
+    W, t = _case(n, seed)
+    N = n + 1
+    for nb in range(5, 10):
+        if O.rows(nb, rowmap, N) <= 0:
+            continue
+        m = O.enumerate_moves(nb, rowmap, W, t, ads=True, odc=True, brute=True)
+        assert np.array_equal(m["ads"], m["brute"]), nb
+        assert np.array_equal(m["odc"], m["brute"]), nb
+        if rowmap == 0:
+            pairs = set(zip(m["i"].tolist(), m["j"].tolist()))
+            assert pairs == _valid_pairs(nb, N) and len(m["i"]) == len(pairs), nb
+    tf, L, rounds, _ = O.dvnd(W, O.random_tour(O.MT(seed), W, 0.5), rowmap, 0, 10)
+    assert L == O.latency(W, tf)
+    for nb in range(10):
+        if O.rows(nb, rowmap, N) > 0:
+            r = O.eval_rows(nb, rowmap, W, tf)
+            assert (r["cost"] >= 0).all(), nb

@@ -14,7 +14,8 @@
 
 #include <stdint.h>
 
-#define MLP_NB_COUNT 5
+#define MLP_NB_COUNT 5   // swap, 2-opt, or-opt-1/2/3: the tiled sweep's neighborhoods
+#define MLP_NB_MAX 15    // ids up to 14: or-opt-k for k <= 13 (MLMove64 id is 4 bits)
 #define MLP_KEY_SHIFT 20                       // row-local index bits in a key
 #define MLP_KEY_MASK ((1LL << MLP_KEY_SHIFT) - 1)
 #define MLP_KEY_NONE INT64_MAX                 // row without a move
@@ -29,8 +30,8 @@ enum {
   MLP_INFO_ACTIVE = 5,    // 1 while a DVND run on this slot continues
   MLP_INFO_ROUNDS = 6,    // DVND rounds run since the tour was set
   MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
-  MLP_INFO_PARITY0 = 16,  // row buffer (0/1) holding the last eval of nb (16 + nb)
-  MLP_INFO_WORDS = 24
+  MLP_INFO_PARITY0 = 24,  // row buffer (0/1) holding the last eval of nb (24 + nb)
+  MLP_INFO_WORDS = 40
 };
 
 enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2 };
@@ -59,9 +60,10 @@ struct SlotDev {
   // fold with atomicMin) and resets buffer p.  A standalone eval gets p ^ 1
   // from the host and its first tile records it; inside a DVND round the
   // eval reads p from info and the merge that follows flips it.
-  int64_t* rows[MLP_NB_COUNT];
-  int64_t* kept_cost[MLP_NB_COUNT]; // merged moves, sorted
-  int32_t* kept_ij[MLP_NB_COUNT];   // (i, j, row) triplets
+  int64_t* rows[MLP_NB_MAX];
+  int64_t* kept_cost[MLP_NB_MAX]; // merged moves, sorted
+  int32_t* kept_ij[MLP_NB_MAX];   // (i, j, row) triplets
+  int32_t nbs;                    // neighborhoods with buffers (ids 0 .. nbs-1)
   void* mscratch;  // the merge's working arrays when they exceed LDS (layout of its LDS)
 };
 

@@ -291,8 +291,14 @@ __device__ __forceinline__ void eval_item(const SlotDev* __restrict__ table, con
   }
   if (row >= sg.rows) return;
   if (lane == 0) oth[row] = MLP_KEY_NONE;
-  const int64_t f = s.info[MLP_INFO_PATH32];
-  if (f & (sg.nb == 1 ? MLP_P32_2OPT : MLP_P32_SWAP_OROPT))
+  bool p32;
+  if (sg.nb <= 4) {
+    p32 = (s.info[MLP_INFO_PATH32] & (sg.nb == 1 ? MLP_P32_2OPT : MLP_P32_SWAP_OROPT)) != 0;
+  } else {  // or-opt-k, k > 3: |delta| <= k Ttot + (k + 4) N maxd (one more N maxd margin)
+    const int64_t k = nb_k(sg.nb);
+    p32 = k * s.info[MLP_INFO_TTOT] + (k + 5) * (int64_t)s.N * s.maxd < 2147483647LL;
+  }
+  if (p32)
     eval_row_dispatch<true>(s, sg.nb, rowmap, row, lane, cur + row);
   else eval_row_dispatch<false>(s, sg.nb, rowmap, row, lane, cur + row);
 }
@@ -742,13 +748,15 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   // 3. greedy filter on wave 0
   if (tid < kWave) {
     int kept = 0;
+#define MLP_GREEDY(NB) \
+  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, keys, ctxL, bits, tid); break
     switch (nb) {
-      case 0: kept = greedy_wave<0>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
-      case 1: kept = greedy_wave<1>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
-      case 2: kept = greedy_wave<2>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
-      case 3: kept = greedy_wave<3>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
-      default: kept = greedy_wave<4>(s, N, rowmap, M, keys, ctxL, bits, tid); break;
+      MLP_GREEDY(0); MLP_GREEDY(1); MLP_GREEDY(2); MLP_GREEDY(3); MLP_GREEDY(4);
+      MLP_GREEDY(5); MLP_GREEDY(6); MLP_GREEDY(7); MLP_GREEDY(8); MLP_GREEDY(9);
+      MLP_GREEDY(10); MLP_GREEDY(11); MLP_GREEDY(12); MLP_GREEDY(13); MLP_GREEDY(14);
+      default: break;
     }
+#undef MLP_GREEDY
     if (tid == 0) s_kept = kept;
   }
   __syncthreads();

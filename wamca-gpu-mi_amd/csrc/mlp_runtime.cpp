@@ -38,10 +38,11 @@ struct Slot {
   void* mem = nullptr;
   size_t cls = 0;  // pool class of mem
   SlotDev dev{};
-  int eval_rowmap[MLP_NB_COUNT] = {-1, -1, -1, -1, -1};
-  int parity[MLP_NB_COUNT] = {0, 0, 0, 0, 0};  // row buffer of the last eval
-  bool parity_stale = false;  // DVND rounds flipped parities on the device
-  bool merged[MLP_NB_COUNT] = {false, false, false, false, false};
+  int nbs = MLP_NB_COUNT;  // neighborhoods with buffers
+  int eval_rowmap[MLP_NB_MAX] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int parity[MLP_NB_MAX] = {};  // row buffer of the last eval
+  bool parity_stale = false;    // DVND rounds flipped parities on the device
+  bool merged[MLP_NB_MAX] = {};
 };
 
 #define MLP_TUNE_BUCKETS 48
@@ -77,7 +78,7 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 // Entries of one row buffer: the most rows any neighborhood has in any row map.
 int row_cap(int N) {
   int r = 1;
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) r = std::max(r, std::max(rows_of(nb, 0, N), rows_of(nb, 1, N)));
+  for (int nb = 0; nb < 3; ++nb) r = std::max(r, std::max(rows_of(nb, 0, N), rows_of(nb, 1, N)));
   return r;
 }
 
@@ -92,11 +93,11 @@ size_t merge_bytes(int N) { return mlp_merge_lds_bytes(merge_pcap(N), N); }
 
 // Byte layout of one slot's device block.
 struct SlotLayout {
-  size_t tour, xy, w, t32, q32, g32, h32, t64, q64, info, rows[MLP_NB_COUNT], kc[MLP_NB_COUNT],
-      kij[MLP_NB_COUNT], ms, total;
+  size_t tour, xy, w, t32, q32, g32, h32, t64, q64, info, rows[MLP_NB_MAX], kc[MLP_NB_MAX],
+      kij[MLP_NB_MAX], ms, total;
 };
 
-SlotLayout slot_layout(int N) {
+SlotLayout slot_layout(int N, int nbs) {
   SlotLayout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -114,7 +115,7 @@ SlotLayout slot_layout(int N) {
   L.t64 = take(8 * (size_t)N);
   L.q64 = take(8 * (size_t)N);
   L.info = take(8 * MLP_INFO_WORDS);
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+  for (int nb = 0; nb < nbs; ++nb) {
     const size_t r = (size_t)row_cap(N);
     L.rows[nb] = take(16 * r);
     L.kc[nb] = take(8 * r);
@@ -160,6 +161,8 @@ struct mlp_ctx {
   // DVND rounds: launches are enqueued `dvnd_chunk` rounds at a time between
   // convergence checks; with dvnd_graph the chunk is captured once per active
   // set and replayed as a HIP graph; dvnd_persistent drains work counters.
+  int nbs = MLP_NB_COUNT;       // neighborhoods of slots created from now on
+  int dvnd_nbs = MLP_NB_COUNT;  // neighborhoods of a DVND round (ids 0 .. dvnd_nbs-1)
   int dvnd_chunk = 4;
   int dvnd_graph = 1;
   int dvnd_persistent = 0;
@@ -554,7 +557,7 @@ int mlp_instance_weights(mlp_ctx* ctx, int32_t inst, int32_t* w_out) {
 
 int64_t mlp_slot_footprint(int32_t n) {
   if (n < 2 || n + 1 > MLP_MAX_N) return -1;
-  return (int64_t)slot_layout(n + 1).total;
+  return (int64_t)slot_layout(n + 1, MLP_NB_COUNT).total;
 }
 
 int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
@@ -564,7 +567,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
     return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
   const Instance& in = ctx->insts[inst];
   const int N = in.n + 1;
-  const SlotLayout L = slot_layout(N);
+  const SlotLayout L = slot_layout(N, ctx->nbs);
   int id = -1;
   for (size_t q = 0; q < ctx->slots.size(); ++q)
     if (!ctx->slots[q].alive) {
@@ -580,6 +583,7 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   Slot s;
   s.inst = inst;
   s.N = N;
+  s.nbs = ctx->nbs;
   st = pool_alloc(ctx, L.total, &s.mem, &s.cls);
   if (st) return st;
   HCK(hipMemsetAsync(s.mem, 0, L.total, ctx->stream));
@@ -602,13 +606,14 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   d.t64 = (int64_t*)(b + L.t64);
   d.q64 = (int64_t*)(b + L.q64);
   d.info = (int64_t*)(b + L.info);
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+  d.nbs = ctx->nbs;
+  for (int nb = 0; nb < d.nbs; ++nb) {
     d.rows[nb] = (int64_t*)(b + L.rows[nb]);
     d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
     d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
   }
   d.mscratch = b + L.ms;
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+  for (int nb = 0; nb < d.nbs; ++nb)
     HCK(mlp_launch_fill_i64(ctx->stream, d.rows[nb], 2 * (int64_t)d.rowcap, MLP_KEY_NONE));
   HCK(mlp_launch_set_slot(ctx->stream, ctx->d_table, id, d));
   s.alive = true;
@@ -663,7 +668,7 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
     HCK(hipMemcpyAsync(s.dev.tour, ctx->h_stage + off, 4 * (size_t)s.N, hipMemcpyHostToDevice,
                        ctx->stream));
     off += (size_t)s.N;
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    for (int nb = 0; nb < MLP_NB_MAX; ++nb) {
       s.eval_rowmap[nb] = -1;
       s.merged[nb] = false;
     }
@@ -805,6 +810,16 @@ hipError_t launch_sweep_cb(mlp_ctx* ctx, void* p) {
 // One evaluation launch.  counter == nullptr: one workgroup per work item
 // (persistent == 0) or the context's standalone counter cleared by a memset;
 // a DVND round passes its own counter, cleared by k_round_begin.
+// Every neighborhood of the mask has buffers in every slot.
+static int check_mask(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask) {
+  if (nb_mask >> MLP_NB_MAX) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  for (int q = 0; q < count; ++q)
+    if (nb_mask >> ctx->slots[slots[q]].nbs)
+      return fail(ctx, MLP_EINVAL, "slot %d has buffers for neighborhoods 0..%d only", slots[q],
+                  ctx->slots[slots[q]].nbs - 1);
+  return MLP_OK;
+}
+
 static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
                      int32_t rowmap, int32_t persistent, int skip, int* round_counter) {
   if (ctx->user_capture)
@@ -815,13 +830,13 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     if (skip) {
       s.parity_stale = true;  // the DVND merges commit parities on the device
     } else if (s.parity_stale) {
-      int64_t par[MLP_NB_COUNT];
+      int64_t par[MLP_NB_MAX];
       HCK(hipStreamSynchronize(ctx->stream));
       HCK(hipMemcpy(par, s.dev.info + MLP_INFO_PARITY0, sizeof par, hipMemcpyDeviceToHost));
-      for (int nb = 0; nb < MLP_NB_COUNT; ++nb) s.parity[nb] = (int)(par[nb] & 1);
+      for (int nb = 0; nb < MLP_NB_MAX; ++nb) s.parity[nb] = (int)(par[nb] & 1);
       s.parity_stale = false;
     }
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    for (int nb = 0; nb < MLP_NB_MAX; ++nb)
       if (nb_mask & (1u << nb)) {
         s.eval_rowmap[nb] = rowmap;
         s.merged[nb] = false;
@@ -830,7 +845,9 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
   EvalArgs a{};
   a.skip = skip;
   int st = MLP_OK;
-  if (rowmap == 0) {
+  // ADS row map: the five tiled neighborhoods go to k_sweep; or-opt-k for
+  // k > 3 (ids 5..14) to the one-wave-per-row k_eval below
+  if (rowmap == 0 && (nb_mask & 0x1fu)) {
     // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
     int tile = ctx->tile_override;
     if (tile < 0) tile = auto_tile(ctx, slots, count, nb_mask);
@@ -868,7 +885,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
         }
       }
     }
-    if (segs.empty()) return MLP_OK;
+    if (!segs.empty()) {
     a.nsegs = (int)segs.size();
     a.total = total;
     a.rowmap = tile;
@@ -891,14 +908,20 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
         HCK(zero_counter(ctx, a.counter));
       }
     }
-    return timed_launch(ctx, launch_sweep_cb, &a);
+    st = timed_launch(ctx, launch_sweep_cb, &a);
+    if (st) return st;
+    }
+    nb_mask &= ~0x1fu;
+    if (!nb_mask) return MLP_OK;
+    a = EvalArgs{};
+    a.skip = skip;
   }
   std::vector<EvalSeg> segs;
   segs.reserve((size_t)count * 5);
   int total = 0;
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    for (int nb = 0; nb < MLP_NB_MAX; ++nb) {
       if (!(nb_mask & (1u << nb))) continue;
       const int R = rows_of(nb, rowmap, s.N);
       if (R <= 0) continue;
@@ -936,7 +959,8 @@ int mlp_eval(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
-  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  st = check_mask(ctx, slots, count, nb_mask);
+  if (st) return st;
   return eval_impl(ctx, slots, count, nb_mask, rowmap, persistent, 0, nullptr);
 }
 
@@ -996,7 +1020,7 @@ int mlp_graph_launch(mlp_ctx* ctx, int32_t graph) {
     if (op.merge) {
       s.merged[op.mask] = true;
     } else {
-      for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+      for (int nb = 0; nb < MLP_NB_MAX; ++nb)
         if (op.mask & (1u << nb)) {
           s.eval_rowmap[nb] = op.rowmap;
           s.merged[nb] = false;
@@ -1013,6 +1037,29 @@ int mlp_graph_destroy(mlp_ctx* ctx, int32_t graph) {
   HCK(hipStreamSynchronize(ctx->stream));
   HCK(hipGraphExecDestroy(ctx->user_graphs[graph]));
   ctx->user_graphs[graph] = nullptr;
+  return MLP_OK;
+}
+
+int mlp_set_neighborhoods(mlp_ctx* ctx, int32_t count) {
+  if (!ctx) return MLP_EINVAL;
+  if (count < MLP_NB_COUNT || count > MLP_NB_MAX)
+    return fail(ctx, MLP_EINVAL, "neighborhood count must be in [%d, %d]", MLP_NB_COUNT, MLP_NB_MAX);
+  ctx->nbs = count;
+  return MLP_OK;
+}
+
+int mlp_dvnd_neighborhoods(mlp_ctx* ctx, int32_t count) {
+  if (!ctx) return MLP_EINVAL;
+  if (count < 1 || count > MLP_NB_MAX)
+    return fail(ctx, MLP_EINVAL, "neighborhood count must be in [1, %d]", MLP_NB_MAX);
+  ctx->dvnd_nbs = count;
+  return MLP_OK;
+}
+
+int mlp_slot_bytes(mlp_ctx* ctx, int32_t n, int64_t* bytes) {
+  if (!ctx || !bytes) return MLP_EINVAL;
+  if (n < 2 || n + 1 > MLP_MAX_N) return fail(ctx, MLP_ERANGE, "n = %d out of range", n);
+  *bytes = (int64_t)slot_layout(n + 1, ctx->nbs).total;
   return MLP_OK;
 }
 
@@ -1143,7 +1190,8 @@ int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
-  if (nb_mask & ~0x1fu) return fail(ctx, MLP_EINVAL, "bad neighborhood mask");
+  st = check_mask(ctx, slots, count, nb_mask);
+  if (st) return st;
   for (int q = 0; q < count; ++q) {
     st = eval_impl(ctx, slots + q, 1, nb_mask, rowmap, 0, 0, nullptr);
     if (st) return st;
@@ -1175,7 +1223,7 @@ int mlp_get_rows(mlp_ctx* ctx, int32_t slot, int32_t nb, int64_t* cost, int32_t*
   NO_CAPTURE(ctx);
   if (!ctx) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
-  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  if (nb < 0 || nb >= MLP_NB_MAX) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
   const Slot& s = ctx->slots[slot];
   const int rm = s.eval_rowmap[nb];
   if (rm < 0) return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not evaluated", nb, slot);
@@ -1281,7 +1329,7 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
 int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb) {
   int st = check_slots(ctx, slots, count);
   if (st) return st;
-  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  if (nb < 0 || nb >= MLP_NB_MAX) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
   return merge_impl(ctx, slots, count, nb, 0);
 }
 
@@ -1290,7 +1338,7 @@ int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int
   NO_CAPTURE(ctx);
   if (!ctx || !count) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
-  if (nb < 0 || nb >= MLP_NB_COUNT) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  if (nb < 0 || nb >= MLP_NB_MAX) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
   const Slot& s = ctx->slots[slot];
   if (!s.merged[nb]) return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not merged", nb, slot);
   HCK(hipStreamSynchronize(ctx->stream));
@@ -1321,8 +1369,8 @@ int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int
   int st = slot_list(ctx, slots, count, inl, &dev);
   if (st) return st;
   HCK(mlp_launch_round_begin(ctx->stream, ctx->d_table, inl, dev, count, ctx->d_counter + 1,
-                             MLP_NB_COUNT));
-  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+                             ctx->dvnd_nbs));
+  for (int nb = 0; nb < ctx->dvnd_nbs; ++nb) {
     bool any = false;
     for (int q = 0; q < count && !any; ++q)
       if (rows_of(nb, rowmap, ctx->slots[slots[q]].N) > 0) any = true;
@@ -1366,7 +1414,7 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
     for (int q = 0; q < count; ++q) {
       Slot& s = ctx->slots[slots[q]];
       s.parity_stale = true;
-      for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      for (int nb = 0; nb < ctx->dvnd_nbs; ++nb) {
         s.eval_rowmap[nb] = rowmap;
         s.merged[nb] = true;
       }
@@ -1431,6 +1479,8 @@ int mlp_dvnd_rounds(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t r
                     int32_t rounds, int32_t max_rounds) {
   int st = check_slots(ctx, slots, count);
   if (st) return st;
+  st = check_mask(ctx, slots, count, (1u << ctx->dvnd_nbs) - 1u);
+  if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
   if (rounds < 0) return fail(ctx, MLP_EINVAL, "rounds must be >= 0");
   if (count == 0 || rounds == 0) return MLP_OK;
@@ -1473,6 +1523,8 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
   if (st) return st;
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
   if (count == 0) return MLP_OK;
+  st = check_mask(ctx, slots, count, (1u << ctx->dvnd_nbs) - 1u);
+  if (st) return st;
   st = mlp_dvnd_start(ctx, slots, count);
   if (st) return st;
   std::vector<int32_t> active(slots, slots + count), flag((size_t)count);

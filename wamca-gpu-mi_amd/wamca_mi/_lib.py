@@ -101,6 +101,9 @@ _SIGS = {
     "mlp_sqrt_selfcheck": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mlp_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_merge_force_global": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_set_neighborhoods": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_dvnd_neighborhoods": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_slot_bytes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
     "mlp_graph_begin": (C.c_int, [C.c_void_p]),
     "mlp_graph_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "mlp_graph_launch": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -185,6 +185,20 @@ class Engine:
     def graph_destroy(self, g: int):
         self._ck(self._L.mlp_graph_destroy(self.h, g))
 
+    def set_neighborhoods(self, count: int):
+        """Slots created from now on hold neighborhood ids 0..count-1 (ids
+        5..14 are or-opt-4..13)."""
+        self._ck(self._L.mlp_set_neighborhoods(self.h, count))
+
+    def dvnd_neighborhoods(self, count: int):
+        """A DVND round runs neighborhood ids 0..count-1 (default 5)."""
+        self._ck(self._L.mlp_dvnd_neighborhoods(self.h, count))
+
+    def slot_bytes(self, n: int) -> int:
+        b = C.c_int64()
+        self._ck(self._L.mlp_slot_bytes(self.h, n, C.byref(b)))
+        return b.value
+
     def merge_force_global(self, on: bool):
         """Merge working arrays in device memory for every tour size."""
         self._ck(self._L.mlp_merge_force_global(self.h, int(on)))
