@@ -21,6 +21,9 @@
 //                        brute force L(apply(tour)) - L(tour), every move
 //   * per-row / merge  : results/*.txt best move, improving-row count and
 //                        merged (sum, count); berlin52 ODC swap pass 367403
+//   * TOTAL entries    : results/*.txt TOTAL sections: best entry, improving
+//                        entry count and GPU-GPU merged (sum, count) for the
+//                        logged block size
 //   * DVND loop        : no reference implementation; pinned by brute force
 //                        local optimality of the final tour (parity unpinned
 //                        against reference outputs -- none exist)
@@ -633,6 +636,42 @@ int or_eval_rows(int nb, int rowmap, const int32_t* W, int n, const int32_t* tou
     ctxs[r] = rb[r].ctx;
   }
   return (int)rb.size();
+}
+
+// TOTAL mode (kernelSwapTotal mlkswap.cu:334-526, kernel2OptTotal
+// mlk2opt.cu:270-447, kernelOrOptTotal mlkoropt.cu:368-612): one launch row per
+// ADS row, B threads; thread t of row `row` scans the row-local indices
+// ctx = t, t + B, ... and keeps the first strictly smallest cost; its record
+// lands at entry row * B + t (mlkswap.cu:526).  Entries whose indices hold no
+// move are (INT64_MAX, 0, 0).  Returns rows * B (entries written when the
+// arrays are non-NULL).
+int64_t or_eval_total(int nb, const int32_t* W, int n, const int32_t* tour, int N, int B,
+                      int64_t* cost, int32_t* is, int32_t* js) {
+  const int R = rows_of(nb, 0, N), X = ctx_of(nb, 0, N);
+  if (R <= 0 || B <= 0) return 0;
+  const int64_t M = (int64_t)R * B;
+  if (!cost) return M;
+  std::vector<int64_t> T(N), C((int64_t)N * N);
+  prefix_time(W, n, tour, N, T.data());
+  ads_cost(T.data(), N, C.data());
+  const int64_t L = latency(W, n, tour, N);
+  for (int64_t e = 0; e < M; ++e) {
+    cost[e] = kInf;
+    is[e] = js[e] = 0;
+  }
+  for (int by = 0; by < R; ++by)
+    for (int ctx = 0; ctx < X; ++ctx) {
+      int i, j;
+      if (!move_of(nb, 0, N, by, ctx, &i, &j)) continue;
+      const int64_t e = (int64_t)by * B + ctx % B;
+      const int64_t c = delta_ads(nb, W, n, tour, N, T.data(), C.data(), L, i, j);
+      if (c < cost[e]) {
+        cost[e] = c;
+        is[e] = i;
+        js[e] = j;
+      }
+    }
+  return M;
 }
 
 // Merge of a per-row list (any source): surviving row ids in sorted order.

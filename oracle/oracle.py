@@ -67,6 +67,8 @@ def lib():
                                          vp, vp, vp, vp, vp, vp, vp]),
             "or_eval_rows": (C.c_int, [C.c_int, C.c_int, I32P, C.c_int, I32P, C.c_int,
                                        I64P, I32P, I32P, I32P]),
+            "or_eval_total": (C.c_int64, [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
             "or_merge": (C.c_int, [C.c_int, C.c_int, I64P, I32P, I32P, I32P, I64P]),
             "or_apply": (None, [C.c_int, I32P, C.c_int, C.c_int]),
             "or_can_merge": (C.c_int, [C.c_int] * 5),
@@ -211,6 +213,22 @@ def eval_rows(nb, rowmap, W, tour):
     ctx = np.empty(R, np.int32)
     lib().or_eval_rows(nb, rowmap, Wf, n, t, N, cost, i, j, ctx)
     return {"cost": cost, "i": i, "j": j, "ctx": ctx}
+
+
+def eval_total(nb, W, tour, block):
+    """TOTAL-mode entries (ADS row map): row * block + t holds the first
+    strict minimum over the row-local indices t, t + block, ...; dict(cost
+    int64 (INT64_MAX when empty), i, j) of length rows * block."""
+    Wf, n, t = _wt(W, tour)
+    N = len(t)
+    M = lib().or_eval_total(nb, Wf, n, t, N, block, None, None, None)
+    cost = np.empty(M, np.int64)
+    i = np.empty(M, np.int32)
+    j = np.empty(M, np.int32)
+    if M:
+        lib().or_eval_total(nb, Wf, n, t, N, block, cost.ctypes.data, i.ctypes.data,
+                            j.ctypes.data)
+    return {"cost": cost, "i": i, "j": j}
 
 
 def merge(nb, cost, i, j):

@@ -122,3 +122,43 @@ def test_logged_partial_kernels(logname, idx, kname):
     assert [int((r["cost"] < 0).sum()), len(r["cost"])] == p["graph"]
     kept, s = O.merge(k["nb"], r["cost"], r["i"], r["j"])
     assert [s, len(kept)] == p["merge_gpu"]
+
+
+def _total_block(k, N):
+    t = k["total"]
+    if "block" in t:
+        return t["block"]
+    R = O.rows(k["nb"], 0, N)
+    assert t["move_elems"] % R == 0
+    return t["move_elems"] // R
+
+
+TOTAL_CASES = [(log["log"], sol["index"], name)
+               for log, sol in _logged() for name, k in sol["kernels"].items()
+               if "total" in k and "best" in k["total"]]
+
+
+@pytest.mark.parametrize("logname,idx,kname", TOTAL_CASES)
+def test_logged_total_kernels(logname, idx, kname):
+    """TOTAL mode: per-(row, thread) entries for the logged block size give
+    the logged best entry, improving-entry count and GPU-GPU merged (sum,
+    count).  The CPU graph of the later build caps the count at 120000
+    (mlkernel.cu:778-787); results-inst1000-bug-or1.txt predates the cap."""
+    log = next(l for l in results_logs() if l["log"] == logname)
+    sol = next(s for s in log["solutions"] if s["index"] == idx)
+    k = sol["kernels"][kname]
+    p = k["total"]
+    W = _W(log["instance"])
+    t = np.array(sol["tour"], np.int32)
+    B = _total_block(k, len(t))
+    r = O.eval_total(k["nb"], W, t, B)
+    assert len(r["cost"]) == p["move_elems"]
+    b = int(np.argsort(r["cost"], kind="stable")[0])
+    assert [int(r["i"][b]), int(r["j"][b]), int(r["cost"][b])] == p["best"]
+    v = int((r["cost"] < 0).sum())
+    g = p["graph"]
+    assert g[0] == g[1]
+    assert v == g[0] or (g[0] == 120000 and v > 120000), (v, g)
+    if "merge_gpu" in p:
+        kept, s = O.merge(k["nb"], r["cost"], r["i"], r["j"])
+        assert [s, len(kept)] == p["merge_gpu"]
