@@ -138,6 +138,33 @@ int mlp_merge_apply(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t n
 int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int32_t cap,
                     int32_t* count);
 
+/* TOTAL mode (the reference's isTotal kernels: kernelSwapTotal
+ * mlkswap.cu:334-526, kernel2OptTotal mlk2opt.cu:270-447, kernelOrOptTotal
+ * mlkoropt.cu:368-612, launched with `block` threads per ADS row): entry
+ * row * block + t holds the best move of nb among the row-local indices t,
+ * t + block, t + 2 block, ... of that row (ties: lowest index), so a row
+ * yields `block` entries instead of one.  block is 1..MLP_TOTAL_MAX_BLOCK;
+ * a block at least the row length gives every move an entry of its own.
+ * The entries stay with the slot until its next mlp_eval_total or tour
+ * upload.  ADS row map only (the reference has no ODC TOTAL kernels). */
+#define MLP_TOTAL_MAX_BLOCK 16384
+int mlp_eval_total(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb,
+                   int32_t block);
+
+/* The entries of the slot's last mlp_eval_total: cost (INT64_MAX for an
+ * entry without a move), i, j for the first min(cap, rows * block) entries;
+ * *entries receives rows * block.  Any array may be NULL. */
+int mlp_get_total(mlp_ctx* ctx, int32_t slot, int64_t* cost, int32_t* i, int32_t* j,
+                  int64_t cap, int64_t* entries);
+
+/* Device greedy merge over all entries of each slot's last mlp_eval_total
+ * (stable sort by cost, keep an improving entry iff no earlier kept move
+ * conflicts: mergeGPU mlkernel.cu:254-384 with testSwapTotal /
+ * test2OptTotal / testOrOptTotal :183-250) and application of the kept
+ * moves; the ADS is rebuilt.  mlp_get_applied(slot, nb) then lists the kept
+ * moves, with the entry index in place of the row. */
+int mlp_merge_total(mlp_ctx* ctx, const int32_t* slots, int32_t count);
+
 /* DVND to a local optimum: rounds of {eval, merge, apply} over swap, 2-opt,
  * or-opt-1/2/3 until a round applies no move (or max_rounds, 0 = unbounded).
  * rounds_out (per slot, may be NULL) receives the rounds each slot ran. */
@@ -199,9 +226,10 @@ int mlp_graph_begin(mlp_ctx* ctx);
 int mlp_graph_end(mlp_ctx* ctx, int32_t* graph);
 int mlp_graph_launch(mlp_ctx* ctx, int32_t graph);
 int mlp_graph_destroy(mlp_ctx* ctx, int32_t graph);
-/* The merge keeps its working arrays in LDS when they fit (about N <= 8190)
- * and in the slot's device memory otherwise; on = 1 uses device memory for
- * every size (same results; for checking the large-N path on small tours). */
+/* The merge keeps its working arrays in LDS when they fit (about N <= 8190;
+ * TOTAL merge: about N <= 16100) and in the slot's device memory otherwise;
+ * on = 1 uses device memory for every size (same results; for checking the
+ * large-N path on small tours). */
 int mlp_merge_force_global(mlp_ctx* ctx, int32_t on);
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile);
 /* Autotuner of the sweep tile shape: times every shape on an evaluation of

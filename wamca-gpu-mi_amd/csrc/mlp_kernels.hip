@@ -651,6 +651,41 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
 }
 
 
+// Applies K kept moves (disjoint regions) to s.tour through a position ->
+// source map: a move with a short region is applied by one thread, the few
+// long ones (regions are disjoint, so at most N / kShortMove) by all threads.
+// ij(q) gives the q-th kept move as (i << 32 | j); tnew ends as the new tour.
+template <typename IJ>
+__device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ ij, int32_t* told,
+                                           int32_t* tnew, int& s_long, int* s_longq) {
+  const int N = s.N, tid = threadIdx.x, k = nb_k(nb);
+  for (int p = tid; p < N; p += kBlock) {
+    const int c = s.tour[p];
+    told[p] = c;
+    tnew[p] = c;
+  }
+  if (tid == 0) s_long = 0;
+  __syncthreads();
+  for (int q = tid; q < K; q += kBlock) {
+    const int64_t v = ij(q);
+    const int i = (int)(v >> 32), j = (int)(uint32_t)v;
+    if (move_span(nb, k, i, j) <= kShortMove) {
+      apply_span(nb, k, i, j, told, tnew, 0, 1);
+    } else {
+      const int at = atomicAdd(&s_long, 1);
+      if (at < kLongMoves) s_longq[at] = q;
+    }
+  }
+  __syncthreads();
+  const int nl = min(s_long, kLongMoves);
+  for (int t = 0; t < nl; ++t) {
+    const int64_t v = ij(s_longq[t]);
+    apply_span(nb, k, (int)(v >> 32), (int)(uint32_t)v, told, tnew, tid, kBlock);
+  }
+  __syncthreads();
+  for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
+}
+
 }  // namespace
 
 // One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
@@ -669,7 +704,6 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
   const int N = s.N, tid = threadIdx.x;
   const int R = rows_of(nb, rowmap, N);
-  const int k = nb_k(nb);
   // working arrays in LDS (capacities of the launch), or in the slot's global
   // scratch laid out for its own size
   int pcap = pcap_l, ncap = ncap_l;
@@ -761,34 +795,263 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   }
   __syncthreads();
   const int K = s_kept;
-  // 4. apply the kept moves (disjoint regions) through a position -> source
-  // map: a move with a short region is applied by one thread, the few long
-  // ones (regions are disjoint, so at most N / kShortMove) by all threads.
-  for (int p = tid; p < N; p += kBlock) {
-    const int c = s.tour[p];
-    told[p] = c;
-    tnew[p] = c;
+  // 4. apply the kept moves, rebuild the prefix arrays
+  apply_kept(s, nb, K, [&](int q) { return keys[q]; }, told, tnew, s_long, s_longq);
+  prep_wg(s, tnew, PL);
+  if (tid == 0) {
+    s.info[MLP_INFO_KEPT0 + nb] = K;
+    s.info[MLP_INFO_ROUND_MOVES] += K;
+    s.info[MLP_INFO_TOTAL_MOVES] += K;
   }
-  if (tid == 0) s_long = 0;
+}
+
+// ---------------------------------------------------------------------------
+// TOTAL mode (kernelSwapTotal mlkswap.cu:334-526, kernel2OptTotal
+// mlk2opt.cu:270-447, kernelOrOptTotal mlkoropt.cu:368-612, merged by
+// mergeGPU mlkernel.cu:254-384 with testSwapTotal etc. :183-250).  The
+// reference launches B threads per ADS row; thread t scans the row-local
+// indices ctx = t, t + B, ... and keeps the first strictly smallest cost, so
+// a row yields B entries (row * B + t).  Here one workgroup takes a row, its
+// threads stride over the row's indices and fold each move's key
+// (delta << 20 | ctx) into entry ctx mod B with an LDS atomic minimum: the
+// smallest key of an entry is the reference's first strict minimum.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int KIND, bool P32>
+__device__ __forceinline__ void total_row(const SlotDev& s, int nb, int row, int B, int64_t* best) {
+  const int N = s.N;
+  const float dr = round_dr(s.round);
+  const int k = nb_k(nb);
+  int lo, hi;
+  ctx_range(nb, 0, N, row, lo, hi);
+  int slot = (lo + (int)threadIdx.x) % B;
+  const int step = kBlock % B;
+  for (int ctx = lo + (int)threadIdx.x; ctx < hi; ctx += kBlock) {
+    int i, j;
+    move_ij(nb, 0, N, row, ctx, i, j);
+    int64_t d;
+    if (KIND == 0) d = delta_swap<P32>(s, i, j, dr);
+    else if (KIND == 1) d = delta_2opt<P32>(s, i, j, dr);
+    else d = delta_oropt<P32>(s, k, i, j, dr);
+    atomicMin((long long*)&best[slot], (long long)(d * (int64_t)(1 << MLP_KEY_SHIFT) + ctx));
+    slot += step;
+    if (slot >= B) slot -= B;
+  }
+}
+
+template <bool P32>
+__device__ __forceinline__ void total_row_dispatch(const SlotDev& s, int nb, int row, int B,
+                                                   int64_t* best) {
+  if (nb == 0) total_row<0, P32>(s, nb, row, B, best);
+  else if (nb == 1) total_row<1, P32>(s, nb, row, B, best);
+  else total_row<2, P32>(s, nb, row, B, best);
+}
+
+__device__ __forceinline__ bool int32_path(const SlotDev& s, int nb) {
+  if (nb <= 4) return (s.info[MLP_INFO_PATH32] & (nb == 1 ? MLP_P32_2OPT : MLP_P32_SWAP_OROPT)) != 0;
+  const int64_t k = nb_k(nb);  // or-opt-k, k > 3: |delta| <= k Ttot + (k + 4) N maxd
+  return k * s.info[MLP_INFO_TTOT] + (k + 5) * (int64_t)s.N * s.maxd < 2147483647LL;
+}
+
+}  // namespace
+
+// grid = rows of nb (ADS row map); dynamic LDS: B int64.
+__global__ __launch_bounds__(kBlock) void k_total_eval(const SlotDev* __restrict__ table, int id,
+                                                       int nb, int B, int64_t* __restrict__ out) {
+  extern __shared__ int64_t best[];
+  const SlotDev& s = table[id];
+  const int row = blockIdx.x;
+  for (int t = threadIdx.x; t < B; t += kBlock) best[t] = MLP_KEY_NONE;
   __syncthreads();
-  for (int q = tid; q < K; q += kBlock) {
-    const int64_t v = keys[q];
-    const int i = (int)(v >> 32), j = (int)(uint32_t)v;
-    if (move_span(nb, k, i, j) <= kShortMove) {
-      apply_span(nb, k, i, j, told, tnew, 0, 1);
-    } else {
-      const int at = atomicAdd(&s_long, 1);
-      if (at < kLongMoves) s_longq[at] = q;
+  if (int32_path(s, nb)) total_row_dispatch<true>(s, nb, row, B, best);
+  else total_row_dispatch<false>(s, nb, row, B, best);
+  __syncthreads();
+  int64_t* o = out + (int64_t)row * B;
+  for (int t = threadIdx.x; t < B; t += kBlock) o[t] = best[t];
+}
+
+// Sort keys of the entries: an improving entry's delta + bias (bias >= every
+// |delta|), bias for the rest; values = entry index.  A stable sort by these
+// keys orders the entries by (cost, entry) like the reference's stable sort.
+__global__ void k_total_keys(const int64_t* __restrict__ tk, int64_t M, uint64_t bias,
+                             uint64_t* __restrict__ ku, int32_t* __restrict__ ve) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < M;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int64_t c = tk[e];
+    uint64_t u = bias;
+    if (c != MLP_KEY_NONE) {
+      const int64_t d = c >> MLP_KEY_SHIFT;
+      if (d < 0) u = d + (int64_t)bias > 0 ? (uint64_t)(d + (int64_t)bias) : 0;
     }
+    ku[e] = u;
+    ve[e] = (int32_t)e;
+  }
+}
+
+namespace {
+
+constexpr int kTotalPer = 4;                       // candidates per thread per chunk
+constexpr int kTotalChunk = kTotalPer * kBlock;    // candidates decoded per chunk
+
+struct TotalLds {
+  int32_t ci[kTotalChunk], cj[kTotalChunk], ce[kTotalChunk];
+  int64_t cc[kTotalChunk];
+  int wc[kTotalPer][kBlock / kWave];
+  int end, kept, nlong;
+  int longq[kLongMoves];
+};
+
+// Greedy over S decoded candidates in order (one wave), continuing a kept
+// list of `base` moves: same batch logic as greedy_wave.
+template <int NB>
+__device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t* bits, int lane,
+                           int base_kept) {
+  constexpr int K = NB >= 2 ? NB - 1 : 0;
+  int kept_total = base_kept;
+  for (int base = 0; base < S; base += kWave) {
+    const int t = base + lane;
+    const bool valid = t < S;
+    const int i = valid ? T.ci[t] : 0, j = valid ? T.cj[t] : 0;
+    const int a = NB == 1 ? i : min(i, j);
+    const int b = NB == 1 ? j + 1 : max(i, j) + K;
+    bool pre = valid;
+    if (valid) {
+      if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+      else pre = !bits_any(bits, a, b);
+    }
+    uint64_t m = __ballot(pre), kept = 0;
+    while (m) {
+      const int q = __builtin_ctzll(m);
+      m &= m - 1;
+      const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
+      const bool c = conflicts_nb<NB>(iq, jq, i, j);
+      if ((__ballot(c) & kept) == 0) kept |= 1ull << q;
+    }
+    if ((kept >> lane) & 1ull) {
+      if (NB == 0) {
+        bits_set(bits, i - 1, i + 1);
+        bits_set(bits, j - 1, j + 1);
+      } else {
+        bits_set(bits, a, b);
+      }
+      const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
+      s.kept_cost[NB][q] = T.cc[t];
+      s.kept_ij[NB][3 * q + 0] = i;
+      s.kept_ij[NB][3 * q + 1] = j;
+      s.kept_ij[NB][3 * q + 2] = T.ce[t];
+    }
+    kept_total += __popcll(kept);
+  }
+  return kept_total;
+}
+
+}  // namespace
+
+// Greedy merge of the sorted entries and apply, one workgroup.  The sorted
+// list is consumed in chunks: all threads decode a chunk and drop the
+// candidates that already conflict with a kept move (the kept set only
+// grows, so they would be dropped anyway), keeping the survivors in order;
+// wave 0 then runs the greedy over the survivors.  The first non-improving
+// key ends the list.  Dynamic LDS: bits | told[N] | tnew[N] (told/tnew in the
+// slot's merge scratch when GMEM).
+template <bool GMEM>
+__global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restrict__ table, int id,
+                                                        int nb, int B, int64_t M, uint64_t bias,
+                                                        const int64_t* __restrict__ tk,
+                                                        const uint64_t* __restrict__ ku,
+                                                        const int32_t* __restrict__ ve) {
+  extern __shared__ int64_t lds[];
+  __shared__ TotalLds T;
+  __shared__ PrepLds PL;
+  const SlotDev& s = table[id];
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const int k = nb_k(nb);
+  const int nwords = (N + 32) / 32 + 1;
+  uint32_t* bits = (uint32_t*)lds;
+  int32_t* told = GMEM ? (int32_t*)s.mscratch : (int32_t*)(bits + nwords);
+  int32_t* tnew = told + N;
+  for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
+  if (tid == 0) {
+    T.end = 0;
+    T.kept = 0;
   }
   __syncthreads();
-  const int nl = min(s_long, kLongMoves);
-  for (int t = 0; t < nl; ++t) {
-    const int64_t v = keys[s_longq[t]];
-    apply_span(nb, k, (int)(v >> 32), (int)(uint32_t)v, told, tnew, tid, kBlock);
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int64_t base = 0; base < M; base += kTotalChunk) {
+    int ci[kTotalPer], cj[kTotalPer], ce[kTotalPer];
+    int64_t cc[kTotalPer];
+    uint64_t bal[kTotalPer];
+#pragma unroll
+    for (int u = 0; u < kTotalPer; ++u) {
+      const int64_t q = base + u * kBlock + tid;
+      bool pass = false;
+      ci[u] = cj[u] = ce[u] = 0;
+      cc[u] = 0;
+      if (q < M) {
+        const uint64_t key = ku[q];
+        if (key < bias) {
+          const int e = ve[q];
+          const int ctx = (int)(tk[e] & MLP_KEY_MASK);
+          const int row = e / B;
+          int i, j;
+          move_ij(nb, 0, N, row, ctx, i, j);
+          ci[u] = i;
+          cj[u] = j;
+          ce[u] = e;
+          cc[u] = (int64_t)key - (int64_t)bias;
+          if (nb == 0) pass = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+          else if (nb == 1) pass = !bits_any(bits, i, j + 1);
+          else pass = !bits_any(bits, min(i, j), max(i, j) + k);
+        } else {
+          T.end = 1;
+        }
+      }
+      bal[u] = __ballot(pass);
+      if (lane == 0) T.wc[u][wave] = __popcll(bal[u]);
+    }
+    __syncthreads();
+    int S = 0;
+#pragma unroll
+    for (int u = 0; u < kTotalPer; ++u) {
+      int off = S;
+      for (int w = 0; w < kBlock / kWave; ++w) {
+        if (w < wave) off += T.wc[u][w];
+        S += T.wc[u][w];
+      }
+      if ((bal[u] >> lane) & 1ull) {
+        const int at = off + __popcll(bal[u] & lt);
+        T.ci[at] = ci[u];
+        T.cj[at] = cj[u];
+        T.ce[at] = ce[u];
+        T.cc[at] = cc[u];
+      }
+    }
+    __syncthreads();
+    // read before the barrier below: the next chunk's decode may set it
+    const bool end = T.end != 0;
+    if (wave == 0 && S > 0) {
+      int kept = T.kept;
+#define MLP_TGREEDY(NB) \
+  case NB: kept = greedy_list<NB>(s, S, T, bits, lane, kept); break
+      switch (nb) {
+        MLP_TGREEDY(0); MLP_TGREEDY(1); MLP_TGREEDY(2); MLP_TGREEDY(3); MLP_TGREEDY(4);
+        MLP_TGREEDY(5); MLP_TGREEDY(6); MLP_TGREEDY(7); MLP_TGREEDY(8); MLP_TGREEDY(9);
+        MLP_TGREEDY(10); MLP_TGREEDY(11); MLP_TGREEDY(12); MLP_TGREEDY(13); MLP_TGREEDY(14);
+        default: break;
+      }
+#undef MLP_TGREEDY
+      if (lane == 0) T.kept = kept;
+    }
+    __syncthreads();
+    if (end) break;
   }
-  __syncthreads();
-  for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
+  const int K = T.kept;
+  const int32_t* kij = s.kept_ij[nb];
+  apply_kept(
+      s, nb, K,
+      [&](int q) { return ((int64_t)kij[3 * q] << 32) | (uint32_t)kij[3 * q + 1]; }, told, tnew,
+      T.nlong, T.longq);
   prep_wg(s, tnew, PL);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
@@ -1516,8 +1779,11 @@ size_t mlp_merge_lds_bytes(int pcap, int ncap) {
 }
 
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
-  return hipFuncSetAttribute((const void*)k_merge_apply<false>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+  const hipError_t e = hipFuncSetAttribute((const void*)k_merge_apply<false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)max_dyn_lds);
+  if (e != hipSuccess) (void)hipGetLastError();  // do not leave it for the next launch check
+  return e;
 }
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
@@ -1530,6 +1796,56 @@ hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const Sl
     hipLaunchKernelGGL(k_merge_apply<false>, dim3(count), dim3(kBlock),
                        mlp_merge_lds_bytes(pcap, ncap), st, table, inl, list_dev, nb, rowmap, pcap,
                        ncap, skip_inactive);
+  return hipGetLastError();
+}
+
+size_t mlp_total_merge_lds_bytes(int N) {
+  return (size_t)((N + 32) / 32 + 1) * 4 + (size_t)N * 8;
+}
+
+// The merge kernel's own static LDS (chunk arrays, prefix scratch) is below
+// kTotalStaticLds; its dynamic part may use the rest.
+constexpr size_t kTotalStaticLds = 24 * 1024;
+static_assert(sizeof(TotalLds) + sizeof(PrepLds) <= kTotalStaticLds, "static LDS estimate");
+
+size_t mlp_total_static_lds() { return kTotalStaticLds; }
+
+hipError_t mlp_total_setup(size_t max_dyn_lds) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_total_eval,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_total_merge<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(max_dyn_lds - kTotalStaticLds));
+  if (e != hipSuccess) (void)hipGetLastError();  // do not leave it for the next launch check
+  return e;
+}
+
+hipError_t mlp_launch_total_eval(hipStream_t st, const SlotDev* table, int id, int nb, int rows,
+                                 int B, int64_t* out) {
+  hipLaunchKernelGGL(k_total_eval, dim3(rows), dim3(kBlock), (size_t)B * 8, st, table, id, nb, B,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, uint64_t bias,
+                                 uint64_t* ku, int32_t* ve) {
+  const int64_t blocks = std::min<int64_t>((M + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(k_total_keys, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(kBlock), 0,
+                     st, tk, M, bias, ku, ve);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_total_merge(hipStream_t st, const SlotDev* table, int id, int nb, int N,
+                                  int B, int64_t M, uint64_t bias, const int64_t* tk,
+                                  const uint64_t* ku, const int32_t* ve, int gmem) {
+  if (gmem)
+    hipLaunchKernelGGL(k_total_merge<true>, dim3(1), dim3(kBlock),
+                       (size_t)((N + 32) / 32 + 1) * 4, st, table, id, nb, B, M, bias, tk, ku,
+                       ve);
+  else
+    hipLaunchKernelGGL(k_total_merge<false>, dim3(1), dim3(kBlock), mlp_total_merge_lds_bytes(N),
+                       st, table, id, nb, B, M, bias, tk, ku, ve);
   return hipGetLastError();
 }
 

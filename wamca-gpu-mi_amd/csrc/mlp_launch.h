@@ -29,6 +29,22 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
                                   int pcap, int ncap, int skip_inactive, int gmem);
+// TOTAL mode: entries rows * B of nb, their sort keys, the merge + apply.
+size_t mlp_total_merge_lds_bytes(int N);
+size_t mlp_total_static_lds();
+hipError_t mlp_total_setup(size_t max_dyn_lds);
+hipError_t mlp_launch_total_eval(hipStream_t st, const SlotDev* table, int id, int nb, int rows,
+                                 int B, int64_t* out);
+hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, uint64_t bias,
+                                 uint64_t* ku, int32_t* ve);
+hipError_t mlp_launch_total_merge(hipStream_t st, const SlotDev* table, int id, int nb, int N,
+                                  int B, int64_t M, uint64_t bias, const int64_t* tk,
+                                  const uint64_t* ku, const int32_t* ve, int gmem);
+// Stable LSD radix sort of (key, value) pairs on bits [0, end_bit) of the
+// keys (mlp_sort.hip).  tmp == nullptr: *tmp_bytes receives the scratch size.
+hipError_t mlp_sort_pairs_u64(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
+                              const int32_t* vin, int32_t* vout, int n, int end_bit,
+                              hipStream_t st);
 hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, uint32_t* S,
                           uint32_t* out);
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);

@@ -43,6 +43,12 @@ struct Slot {
   int parity[MLP_NB_MAX] = {};  // row buffer of the last eval
   bool parity_stale = false;    // DVND rounds flipped parities on the device
   bool merged[MLP_NB_MAX] = {};
+  // TOTAL mode: entries (delta << 20 | ctx) of the last mlp_eval_total
+  int64_t* total = nullptr;
+  size_t total_cls = 0;
+  int64_t total_cap = 0;  // entries the block holds
+  int total_nb = -1, total_block = 0, total_rows = 0;
+  bool total_merged = false;
 };
 
 #define MLP_TUNE_BUCKETS 48
@@ -150,6 +156,7 @@ struct mlp_ctx {
   size_t i64_cap = 0;
   std::unordered_map<std::string, DevList> lists;
   size_t max_merge_lds = 0;
+  size_t max_total_lds = 0;  // dynamic LDS limit of the TOTAL-mode kernels
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
   bool merge_force_global = false;  // merge working arrays in global memory for every size
   int8_t tune[32][MLP_TUNE_BUCKETS];  // best shape per (mask, log2 moves), -1 untuned
@@ -391,6 +398,9 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
   if (lds > 160 * 1024) lds = 160 * 1024;
   ctx->max_merge_lds = lds > 8192 ? lds - 8192 : lds;
   if (mlp_merge_setup(ctx->max_merge_lds) != hipSuccess) ctx->max_merge_lds = 64 * 1024 - 8192;
+  ctx->max_total_lds = ctx->max_merge_lds;
+  if (mlp_total_setup(ctx->max_total_lds) != hipSuccess) ctx->max_total_lds = 64 * 1024 - 8192;
+  (void)hipGetLastError();
   *out = ctx;
   return MLP_OK;
 }
@@ -627,6 +637,7 @@ int mlp_slot_destroy(mlp_ctx* ctx, int32_t slot) {
   if (!ctx) return MLP_EINVAL;
   if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
   pool_release(ctx, ctx->slots[slot].mem, ctx->slots[slot].cls);
+  pool_release(ctx, ctx->slots[slot].total, ctx->slots[slot].total_cls);
   ctx->slots[slot] = Slot();
   return MLP_OK;
 }
@@ -672,6 +683,8 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
       s.eval_rowmap[nb] = -1;
       s.merged[nb] = false;
     }
+    s.total_nb = -1;
+    s.total_merged = false;
   }
   for (int q0 = 0; q0 < count; q0 += 65535) {
     const int c = std::min(65535, count - q0);
@@ -1354,6 +1367,137 @@ int mlp_get_applied(mlp_ctx* ctx, int32_t slot, int32_t nb, mlp_move* moves, int
     for (int q = 0; q < take; ++q)
       moves[q] = mlp_move{c[(size_t)q], nb, ij[3 * (size_t)q], ij[3 * (size_t)q + 1],
                           ij[3 * (size_t)q + 2]};
+  }
+  return MLP_OK;
+}
+
+// ---- TOTAL mode ------------------------------------------------------------
+
+int mlp_eval_total(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb,
+                   int32_t block) {
+  NO_CAPTURE(ctx);
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  if (nb < 0 || nb >= MLP_NB_MAX) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  if (block < 1 || block > MLP_TOTAL_MAX_BLOCK)
+    return fail(ctx, MLP_EINVAL, "block %d outside 1..%d", block, MLP_TOTAL_MAX_BLOCK);
+  if ((size_t)block * 8 > ctx->max_total_lds)
+    return fail(ctx, MLP_ERANGE, "block %d needs %zu bytes of LDS (limit %zu)", block,
+                (size_t)block * 8, ctx->max_total_lds);
+  for (int q = 0; q < count; ++q)
+    if (nb >= ctx->slots[slots[q]].nbs)
+      return fail(ctx, MLP_EINVAL, "slot %d has no buffers for neighborhood %d", slots[q], nb);
+  for (int q = 0; q < count; ++q) {
+    const int id = slots[q];
+    Slot& s = ctx->slots[id];
+    const int R = std::max(0, rows_of(nb, 0, s.N));
+    const int64_t M = (int64_t)R * block;
+    if (M > s.total_cap) {
+      pool_release(ctx, s.total, s.total_cls);
+      s.total = nullptr;
+      s.total_cap = 0;
+      void* p = nullptr;
+      st = pool_alloc(ctx, 8 * (size_t)M, &p, &s.total_cls);
+      if (st) return st;
+      s.total = (int64_t*)p;
+      s.total_cap = (int64_t)(s.total_cls / 8);
+    }
+    s.total_nb = nb;
+    s.total_block = block;
+    s.total_rows = R;
+    s.total_merged = false;
+    if (R > 0) HCK(mlp_launch_total_eval(ctx->stream, ctx->d_table, id, nb, R, block, s.total));
+  }
+  return MLP_OK;
+}
+
+int mlp_get_total(mlp_ctx* ctx, int32_t slot, int64_t* cost, int32_t* i, int32_t* j, int64_t cap,
+                  int64_t* entries) {
+  NO_CAPTURE(ctx);
+  if (!ctx) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  const Slot& s = ctx->slots[slot];
+  if (s.total_nb < 0) return fail(ctx, MLP_ESTATE, "slot %d has no TOTAL evaluation", slot);
+  const int64_t M = (int64_t)s.total_rows * s.total_block;
+  if (entries) *entries = M;
+  const int64_t take = std::min<int64_t>(M, std::max<int64_t>(cap, 0));
+  if (take == 0) return MLP_OK;
+  std::vector<int64_t> keys((size_t)take);
+  HCK(hipStreamSynchronize(ctx->stream));
+  HCK(hipMemcpy(keys.data(), s.total, 8 * (size_t)take, hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < take; ++e) {
+    const int64_t k = keys[(size_t)e];
+    int ii = 0, jj = 0;
+    int64_t c = INT64_MAX;
+    if (k != MLP_KEY_NONE) {
+      c = k >> MLP_KEY_SHIFT;
+      move_ij(s.total_nb, 0, s.N, (int)(e / s.total_block), (int)(k & MLP_KEY_MASK), ii, jj);
+    }
+    if (cost) cost[e] = c;
+    if (i) i[e] = ii;
+    if (j) j[e] = jj;
+  }
+  return MLP_OK;
+}
+
+int mlp_merge_total(mlp_ctx* ctx, const int32_t* slots, int32_t count) {
+  NO_CAPTURE(ctx);
+  int st = check_slots(ctx, slots, count);
+  if (st) return st;
+  for (int q = 0; q < count; ++q) {
+    const Slot& s = ctx->slots[slots[q]];
+    if (s.total_nb < 0)
+      return fail(ctx, MLP_ESTATE, "slot %d has no TOTAL evaluation", slots[q]);
+    if (s.total_merged)
+      return fail(ctx, MLP_ESTATE, "TOTAL evaluation of slot %d already merged", slots[q]);
+  }
+  for (int q = 0; q < count; ++q) {
+    const int id = slots[q];
+    Slot& s = ctx->slots[id];
+    const int N = s.N, nb = s.total_nb, B = s.total_block;
+    const int64_t M = (int64_t)s.total_rows * B;
+    // every |delta| <= N (N + 32) maxd (2-opt: N (Ttot + 6 maxd) + 3 Ttot,
+    // or-opt-k, k <= 13: k Ttot + (k + 5) N maxd, Ttot <= (N - 1) maxd)
+    const uint64_t bias = (uint64_t)N * (uint64_t)(N + 32) * (uint64_t)s.dev.maxd;
+    int end_bit = 1;
+    while (end_bit < 64 && (bias >> end_bit) != 0) ++end_bit;
+    // tour copies in LDS unless they (with the kernel's static LDS) exceed
+    // it, or global arrays are forced
+    const int gmem = ctx->merge_force_global ||
+                     mlp_total_merge_lds_bytes(N) + mlp_total_static_lds() > ctx->max_total_lds;
+    if (M == 0) {
+      HCK(mlp_launch_total_merge(ctx->stream, ctx->d_table, id, nb, N, B, 0, bias, nullptr,
+                                 nullptr, nullptr, gmem));
+    } else {
+      void *ku = nullptr, *ko = nullptr, *vi = nullptr, *vo = nullptr, *tmp = nullptr;
+      size_t cku = 0, cko = 0, cvi = 0, cvo = 0, ctmp = 0, tbytes = 0;
+      HCK(mlp_sort_pairs_u64(nullptr, &tbytes, nullptr, nullptr, nullptr, nullptr, (int)M, end_bit,
+                             ctx->stream));
+      st = pool_alloc(ctx, 8 * (size_t)M, &ku, &cku);
+      if (!st) st = pool_alloc(ctx, 8 * (size_t)M, &ko, &cko);
+      if (!st) st = pool_alloc(ctx, 4 * (size_t)M, &vi, &cvi);
+      if (!st) st = pool_alloc(ctx, 4 * (size_t)M, &vo, &cvo);
+      if (!st) st = pool_alloc(ctx, std::max<size_t>(tbytes, 1), &tmp, &ctmp);
+      hipError_t e = hipSuccess;
+      if (!st) {
+        e = mlp_launch_total_keys(ctx->stream, s.total, M, bias, (uint64_t*)ku, (int32_t*)vi);
+        if (e == hipSuccess)
+          e = mlp_sort_pairs_u64(tmp, &tbytes, (const uint64_t*)ku, (uint64_t*)ko,
+                                 (const int32_t*)vi, (int32_t*)vo, (int)M, end_bit, ctx->stream);
+        if (e == hipSuccess)
+          e = mlp_launch_total_merge(ctx->stream, ctx->d_table, id, nb, N, B, M, bias, s.total,
+                                     (const uint64_t*)ko, (const int32_t*)vo, gmem);
+      }
+      pool_release(ctx, ku, cku);  // stream-ordered: reused only by later work
+      pool_release(ctx, ko, cko);
+      pool_release(ctx, vi, cvi);
+      pool_release(ctx, vo, cvo);
+      pool_release(ctx, tmp, ctmp);
+      if (st) return st;
+      HCK(e);
+    }
+    s.total_merged = true;
+    s.merged[nb] = true;  // mlp_get_applied reads the kept moves
   }
   return MLP_OK;
 }

@@ -84,6 +84,10 @@ _SIGS = {
     "mlp_get_moves64": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, U64P,
                                   C.POINTER(C.c_int32)]),
     "mlp_merge_apply": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32]),
+    "mlp_eval_total": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, C.c_int32]),
+    "mlp_get_total": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int64, C.POINTER(C.c_int64)]),
+    "mlp_merge_total": (C.c_int, [C.c_void_p, I32P, C.c_int32]),
     "mlp_get_applied": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(MoveRec),
                                   C.c_int32, C.POINTER(C.c_int32)]),
     "mlp_dvnd": (C.c_int, [C.c_void_p, I32P, C.c_int32, C.c_int32, C.c_int32, I32P]),

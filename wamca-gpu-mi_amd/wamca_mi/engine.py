@@ -162,6 +162,30 @@ class Engine:
         slots = np.ascontiguousarray(slots, np.int32)
         self._ck(self._L.mlp_merge_apply(self.h, slots, len(slots), nb))
 
+    # -- TOTAL mode (per-(row, thread) entries) ---------------------------------
+    def eval_total(self, slots, nb, block):
+        """Entries row * block + t: best move of nb among the row-local
+        indices t, t + block, ... (the reference's TOTAL kernels)."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_eval_total(self.h, slots, len(slots), nb, block))
+
+    def total(self, s):
+        M = C.c_int64()
+        self._ck(self._L.mlp_get_total(self.h, s, None, None, None, 0, C.byref(M)))
+        m = M.value
+        cost = np.empty(m, np.int64)
+        i = np.empty(m, np.int32)
+        j = np.empty(m, np.int32)
+        if m:
+            self._ck(self._L.mlp_get_total(self.h, s, cost.ctypes.data, i.ctypes.data,
+                                           j.ctypes.data, m, C.byref(M)))
+        return {"cost": cost, "i": i, "j": j}
+
+    def merge_total(self, slots):
+        """Greedy merge over all TOTAL entries + apply (kept moves: applied())."""
+        slots = np.ascontiguousarray(slots, np.int32)
+        self._ck(self._L.mlp_merge_total(self.h, slots, len(slots)))
+
     def applied(self, s, nb):
         cap = MAX_N
         buf = (MoveRec * cap)()
