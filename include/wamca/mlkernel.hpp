@@ -18,8 +18,13 @@
 // Differences: device work goes through the C ABI (the ADS is built on the
 // device from the tour; sendSolution uploads the tour only); latencies are
 // kept as uint32 like the reference (cost64 holds the exact value); errors
-// throw wamca::Error instead of exiting; isTotal kernels are not provided
-// (constructing one throws).
+// throw wamca::Error instead of exiting.  TOTAL kernels (isTotal = true)
+// use `totalBlock` threads per row: the reference takes the block size from
+// the CUDA occupancy calculator (mlkswap.cu:1009-1014), which has no MI355X
+// counterpart, so it is a field here (0: the largest multiple of 64 not above
+// min(size, 1024)); set it to a logged blck(...) value to reproduce that run.
+// Move ids go up to MLMI_OROPT13 (the 4-bit id of MLMove64 holds 0..15 and
+// the device keeps buffers for ids 0..14).
 #ifndef WAMCA_MLKERNEL_HPP_
 #define WAMCA_MLKERNEL_HPP_
 
@@ -47,6 +52,16 @@ enum MLMoveId {
   MLMI_OROPT1,
   MLMI_OROPT2,
   MLMI_OROPT3,
+  MLMI_OROPT4,
+  MLMI_OROPT5,
+  MLMI_OROPT6,
+  MLMI_OROPT7,
+  MLMI_OROPT8,
+  MLMI_OROPT9,
+  MLMI_OROPT10,
+  MLMI_OROPT11,
+  MLMI_OROPT12,
+  MLMI_OROPT13,
 };
 
 // mlads.h:190-216: 64-bit packed move (id:4 | i:14 | j:14, int32 cost)
@@ -68,7 +83,7 @@ struct MLMove {
 };
 static_assert(sizeof(MLMove64) == 8, "MLMove64 is 8 bytes");
 
-extern const char* nameMove[5];
+extern const char* nameMove[15];  // mlads.cpp:41-72, ids 0..14
 
 class MTRandom {
  public:
@@ -190,8 +205,11 @@ class MLKernel {
   const char* name;
   bool isTotal;
   int rowmap = MLP_ROWMAP_ODC;  // the shipped build's kernels; MLP_ROWMAP_ADS for the logged ones
+                                // (TOTAL kernels always use the ADS row map)
+  uint totalBlock = 0;          // TOTAL: threads (entries) per row; 0 = default, set before init
   uint moveElems = 0;
-  std::vector<MLMove64> moves;      // per-row records (recvResult) or merged list (mergeGPU)
+  std::vector<MLMove64> moves;      // per-row (TOTAL: per-entry) records (recvResult) or
+                                    // merged list (mergeGPU)
   MLMove64* transBuffer = nullptr;  // = moves.data()
   MLSolution* solution = nullptr;
   bool solDestroy = false;

@@ -7,6 +7,10 @@
 //   test_mlkernel gpu <instances-dir>         GPU cases (need a device)
 //   test_mlkernel chain <tsp> <seed> <rowmap> one runWAMCA2016 pass over the
 //                                             five kernels; prints the result
+//   test_mlkernel total <tsp> <seed> <sols> <kid>:<block> ...
+//                                             the logged TOTAL section of each of
+//                                             the first <sols> random solutions
+//                                             (WamcaExperiment.hpp:368-581)
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
@@ -201,6 +205,51 @@ static int run_chain(const char* tsp, int seed, int rowmap) {
   return 0;
 }
 
+// For each solution and kernel: best entry, improving entries, GPU merge and
+// CPU merge (sum, count), as the reference's TOTAL section logs them.
+static int run_total(const char* tsp, int seed, int sols, int argc, char** argv) {
+  MLProblem p(true, false, false);
+  p.load(tsp);
+  MTRandom rng;
+  rng.seed((uint)seed);
+  MLSolution sol(p);
+  for (int q = 1; q <= sols; ++q) {
+    sol.random(rng, 0.50F);
+    for (int a = 0; a < argc; ++a) {
+      int kid = 0, block = 0;
+      if (std::sscanf(argv[a], "%d:%d", &kid, &block) != 2) return 2;
+      MLKernel* k = kid == 0   ? (MLKernel*)new MLKernelSwap(p, true)
+                    : kid == 1 ? (MLKernel*)new MLKernel2Opt(p, true)
+                               : (MLKernel*)new MLKernelOrOpt(p, (uint)kid - 1, true);
+      k->totalBlock = (uint)block;
+      k->init(true);
+      k->setSolution(&sol);
+      k->sendSolution();
+      k->launchKernel();
+      k->recvResult();
+      MLMove best;
+      k->bestMove(best);
+      int imp = 0;
+      for (uint e = 0; e < k->moveElems; ++e) imp += k->transBuffer[e].cost < 0;
+      const uint elems = k->moveElems;
+      std::vector<MLMove64> merged(elems);
+      int ccount = 0;
+      const int csum = k->mergeGreedy(merged.data(), ccount);
+      k->mergeGPU();
+      long long gsum = 0;
+      int gcount = 0;
+      for (uint e = 0; e < k->moveElems && k->transBuffer[e].cost < 0; ++e) {
+        gsum += k->transBuffer[e].cost;
+        ++gcount;
+      }
+      std::printf("%d %s elems %u best %d %d %d improving %d gpu %lld %d cpu %d %d\n", q, k->name,
+                  elems, (int)best.i, (int)best.j, best.cost, imp, gsum, gcount, csum, ccount);
+      delete k;
+    }
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 3) {
     std::fprintf(stderr, "usage: %s cpu|gpu <instances-dir> | chain <tsp> <seed> <rowmap>\n", argv[0]);
@@ -208,6 +257,8 @@ int main(int argc, char** argv) {
   }
   const std::string mode = argv[1];
   try {
+    if (mode == "total" && argc > 5)
+      return run_total(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argc - 5, argv + 5);
     if (mode == "chain") return run_chain(argv[2], argc > 3 ? std::atoi(argv[3]) : 1,
                                           argc > 4 ? std::atoi(argv[4]) : MLP_ROWMAP_ODC);
     g_dir = argv[2];

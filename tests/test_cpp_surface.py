@@ -109,3 +109,55 @@ def test_cpp_surface_chain_matches_oracle(name, seed, rowmap):
     assert moves == ref_moves
     assert np.array_equal(final, t)
     assert cost == O.latency_u32(W, t)
+
+
+def _total_lines(out):
+    res = {}
+    for ln in out.splitlines():
+        f = ln.split()
+        if len(f) == 16 and f[2] == "elems":
+            res[(int(f[0]), f[1])] = {
+                "elems": int(f[3]), "best": [int(f[5]), int(f[6]), int(f[7])],
+                "improving": int(f[9]), "gpu": [int(f[11]), int(f[12])],
+                "cpu": [int(f[14]), int(f[15])]}
+    return res
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("logname", ["results-inst100.txt", "results-inst226.txt",
+                                     "results-inst1000.txt"])
+def test_cpp_surface_total_kernels_match_logs(logname):
+    """MLKernel{Swap,2Opt,OrOpt}(problem, isTotal = true) with the logged block
+    sizes reproduce each logged TOTAL record: best entry, improving entries,
+    GPU-GPU and GPU-CPU merged (sum, count) -- including the runs where the
+    reference's CPU graph stopped at 120000 moves and the two merges differ."""
+    from helpers import results_logs
+    log = next(l for l in results_logs() if l["log"] == logname)
+    sols = log["solutions"]
+    xs, _ = load_tsplib_coords(log["instance"])
+    N = len(xs) + 1
+    specs = []
+    for name, k in sols[0]["kernels"].items():
+        t = k.get("total", {})
+        if "best" in t:
+            B = t.get("block") or t["move_elems"] // O.rows(k["nb"], 0, N)
+            specs.append(f"{k['nb']}:{B}")
+    r = _run("total", os.path.join(INST, log["instance"]), str(log["seed"]), str(len(sols)),
+             *specs, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _total_lines(r.stdout)
+    checked = 0
+    for sol in sols:
+        for name, k in sol["kernels"].items():
+            p = k.get("total", {})
+            if "best" not in p:
+                continue
+            g = got[(sol["index"], name)]
+            assert g["elems"] == p["move_elems"]
+            assert g["best"] == p["best"]
+            assert g["improving"] == p["graph"][0] or (p["graph"][0] == 120000 and
+                                                       g["improving"] > 120000)
+            assert g["gpu"] == p["merge_gpu"], (sol["index"], name)
+            assert g["cpu"] == p["merge_cpu"], (sol["index"], name)
+            checked += 1
+    assert checked == 3 * len(sols)

@@ -16,7 +16,17 @@
 
 namespace wamca {
 
-const char* nameMove[5] = {"SWAP", "2OPT", "OROPT1", "OROPT2", "OROPT3"};
+// mlads.cpp:41-72 (ids this build supports)
+const char* nameMove[15] = {"SWAP",   "2OPT",   "OROPT1", "OROPT2",  "OROPT3",
+                            "OROPT4", "OROPT5", "OROPT6", "OROPT7",  "OROPT8",
+                            "OROPT9", "OROPT10", "OROPT11", "OROPT12", "OROPT13"};
+
+// Default TOTAL block: the largest multiple of 64 (one wavefront) not above
+// min(size, 1024).
+static uint default_total_block(uint size) {
+  const uint b = std::min<uint>(size, 1024u) / 64u * 64u;
+  return b ? b : std::max<uint>(size, 1u);
+}
 
 namespace {
 
@@ -248,10 +258,9 @@ std::string MLSolution::str(bool showCost) const {
 
 // ---- MLKernel -----------------------------------------------------------------
 MLKernel::MLKernel(MLProblem& p, bool total, int kid, uint ktag)
-    : problem(p), id(kid), tag(ktag), name(kid >= 0 && kid < 5 ? nameMove[kid] : "OROPT"),
+    : problem(p), id(kid), tag(ktag), name(kid >= 0 && kid < 15 ? nameMove[kid] : "OROPT"),
       isTotal(total) {
-  if (kid < 0 || kid > 4) throw Error("neighborhood id must be 0..4");
-  if (isTotal) throw Error("the TOTAL (per-thread) kernels are not provided by this surface");
+  if (kid < 0 || kid > (int)MLMI_OROPT13) throw Error("neighborhood id must be 0..14");
 }
 
 MLKernel::~MLKernel() {
@@ -264,6 +273,8 @@ MLKernel::~MLKernel() {
 void MLKernel::init(bool solCreate) {
   solDestroy = solCreate;
   if (solCreate) solution = new MLSolution(problem);
+  // or-opt-k for k > 3 needs device buffers for ids 5..14 (slots created from now on)
+  if (id >= 5) check(problem.ctx(), mlp_set_neighborhoods(problem.ctx(), 15), "mlp_set_neighborhoods");
   check(problem.ctx(), mlp_slot_create(problem.ctx(), problem.instance(), &slot_), "mlp_slot_create");
   defineKernelGrid();
 }
@@ -283,7 +294,13 @@ void MLKernel::sendSolution() {
 }
 
 void MLKernel::defineKernelGrid() {
-  moveElems = (uint)std::max(0, rows_of(id, rowmap, (int)problem.size));
+  if (isTotal) {  // rows * block entries (mlkswap.cu:1022-1024)
+    rowmap = MLP_ROWMAP_ADS;
+    if (!totalBlock) totalBlock = default_total_block(problem.size);
+    moveElems = (uint)std::max(0, rows_of(id, rowmap, (int)problem.size)) * totalBlock;
+  } else {
+    moveElems = (uint)std::max(0, rows_of(id, rowmap, (int)problem.size));
+  }
   moves.assign(std::max<uint>(moveElems, 1), MLMove64{0, 0, 0, 0});
   transBuffer = moves.data();
 }
@@ -291,13 +308,38 @@ void MLKernel::defineKernelGrid() {
 void MLKernel::launchKernel() {
   ++callCount;
   merged_ = false;
-  check(problem.ctx(), mlp_eval(problem.ctx(), &slot_, 1, 1u << id, rowmap, 0), "launchKernel");
+  if (isTotal)
+    check(problem.ctx(), mlp_eval_total(problem.ctx(), &slot_, 1, id, (int32_t)totalBlock),
+          "launchKernel");
+  else
+    check(problem.ctx(), mlp_eval(problem.ctx(), &slot_, 1, 1u << id, rowmap, 0), "launchKernel");
 }
 
 void MLKernel::sync() { check(problem.ctx(), mlp_sync(problem.ctx()), "sync"); }
 
 void MLKernel::recvResult() {
   if (merged_) return;  // mergeGPU already left the merged list in transBuffer
+  if (isTotal) {  // entries in the 64-bit wire format; COST_INFTY where empty (mlads.h:24)
+    if (problem.size > 16384) throw Error("recvResult: N > 16384 (14-bit i/j of MLMove64)");
+    int64_t M = 0;
+    check(problem.ctx(), mlp_get_total(problem.ctx(), slot_, nullptr, nullptr, nullptr, 0, &M),
+          "recvResult");
+    std::vector<int64_t> c((size_t)M);
+    std::vector<int32_t> ii((size_t)M), jj((size_t)M);
+    check(problem.ctx(),
+          mlp_get_total(problem.ctx(), slot_, c.data(), ii.data(), jj.data(), M, &M), "recvResult");
+    moveElems = (uint)M;
+    moves.assign(std::max<uint>(moveElems, 1), MLMove64{(unsigned)id, 0, 0, 0x7fffffff});
+    for (int64_t e = 0; e < M; ++e) {
+      const int64_t v = c[(size_t)e];
+      if (v == INT64_MAX) continue;
+      if (v < INT32_MIN || v > INT32_MAX) throw Error("recvResult: cost outside int32");
+      moves[(size_t)e] = MLMove64{(unsigned)id, (unsigned)ii[(size_t)e] & 0x3fffu,
+                                  (unsigned)jj[(size_t)e] & 0x3fffu, (int)v};
+    }
+    transBuffer = moves.data();
+    return;
+  }
   std::vector<uint64_t> w(MLP_MAX_N);
   int32_t rows = 0;
   check(problem.ctx(), mlp_get_moves64(problem.ctx(), slot_, id, w.data(), &rows), "recvResult");
@@ -316,7 +358,10 @@ void MLKernel::recvResult() {
 // a caller that applies the entries with cost < 0 in order applies exactly the
 // kept moves (WamcaExperiment.hpp:229-290).
 void MLKernel::mergeGPU() {
-  check(problem.ctx(), mlp_merge_apply(problem.ctx(), &slot_, 1, id), "mergeGPU");
+  if (isTotal)
+    check(problem.ctx(), mlp_merge_total(problem.ctx(), &slot_, 1), "mergeGPU");
+  else
+    check(problem.ctx(), mlp_merge_apply(problem.ctx(), &slot_, 1, id), "mergeGPU");
   std::vector<mlp_move> mv(std::max<uint>(moveElems, 1));
   int32_t count = 0;
   check(problem.ctx(), mlp_get_applied(problem.ctx(), slot_, id, mv.data(), (int32_t)mv.size(), &count),
@@ -361,10 +406,14 @@ bool can_merge(const MLMove64& m1, const MLMove64& m2) {
 // Host greedy merge (mlkernel.cu:732-864): std::sort by cost, keep a move iff
 // no earlier kept move conflicts.  Sorts transBuffer in place like the
 // reference.
+// In TOTAL mode the reference's conflict graph takes at most the first
+// 120000 improving moves (mlkernel.cu:778-787); the same limit applies here,
+// so this merge can differ from mergeGPU exactly where the reference's does.
 int MLKernel::mergeGreedy(MLMove64* merge, int& count) {
   MLMove64* mv = transBuffer;
-  const int n = (int)moveElems;
+  int n = (int)moveElems;
   std::sort(mv, mv + n, [](const MLMove64& a, const MLMove64& b) { return a.cost < b.cost; });
+  if (isTotal) n = std::min(n, 120000);
   count = 0;
   int cost = 0;
   std::vector<const MLMove64*> kept;
