@@ -107,6 +107,45 @@ def batched_cfg2(eng):
             / (ms / 1e3) / 1e9 / PEAK_LDS_GBS}
 
 
+def total_legs(eng):
+    """TOTAL mode at n = 1001 (N = 1002, uniform [0,1000]^2 seed 5, truncated
+    distances, random-insertion start tour seed 5) with the reference's
+    logged block sizes at that size (576 swap / 2-opt, 896 or-opt-1): device
+    evaluation of all rows * block entries, then the greedy merge over every
+    entry (key transform + radix sort + merge + apply).  Median of 5."""
+    from wamca_mi import inputs as I
+    n = 1001
+    xs, ys = I.uniform(n, 5, 1000)
+    t0 = I.random_tour(xs, ys, 0, 5)
+    inst = eng.instance(xs, ys, 0)
+    s = eng.slot(inst)
+    res = {"workload": "n=1001 uniform, one random-insertion tour, TOTAL entries + merge"}
+    for nb, name, B in ((0, "swap", 576), (1, "2opt", 576), (2, "oropt1", 896)):
+        ev, mg = [], []
+        for rep in range(6):
+            eng.set_tour(s, t0)
+            eng.sync()
+            eng.region_begin()
+            eng.eval_total([s], nb, B)
+            e_ms = eng.region_end()
+            eng.region_begin()
+            eng.merge_total([s])
+            m_ms = eng.region_end()
+            if rep:  # the first pass warms the allocator and the sort's setup
+                ev.append(e_ms)
+                mg.append(m_ms)
+        kept = eng.applied(s, nb)
+        eng.set_tour(s, t0)
+        eng.eval_total([s], nb, B)
+        r = eng.total(s)
+        res[name] = {"block": B, "entries": len(r["cost"]),
+                     "improving": int((r["cost"] < 0).sum()), "kept": len(kept),
+                     "eval_ms": sorted(ev)[len(ev) // 2], "merge_ms": sorted(mg)[len(mg) // 2]}
+    eng.destroy_slot(s)
+    eng.destroy_instance(inst)
+    return res
+
+
 def dvnd_legs(eng, args):
     """DVND solves: n=100 alone; 64 uneven instances batched vs one at a time;
     a Poisson job stream served (continuous batching) vs plain."""
@@ -186,6 +225,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dvnd", action="store_true")
+    ap.add_argument("--no-total", action="store_true")
     ap.add_argument("--stream-jobs", type=int, default=300)
     ap.add_argument("--stream-rate", type=float, default=2000.0)
     ap.add_argument("--stream-big-frac", type=float, default=0.01,
@@ -304,6 +344,8 @@ def main():
         out["batched_cfg2"] = batched_cfg2(eng)
     if not args.no_dvnd:
         out["dvnd"] = dvnd_legs(eng, args)
+    if not args.no_total:
+        out["total_mode"] = total_legs(eng)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(xs, ys, tours)
     eng.close()

@@ -227,7 +227,7 @@ int mlp_graph_end(mlp_ctx* ctx, int32_t* graph);
 int mlp_graph_launch(mlp_ctx* ctx, int32_t graph);
 int mlp_graph_destroy(mlp_ctx* ctx, int32_t graph);
 /* The merge keeps its working arrays in LDS when they fit (about N <= 8190;
- * TOTAL merge: about N <= 16100) and in the slot's device memory otherwise;
+ * TOTAL merge: about N <= 13100) and in the slot's device memory otherwise;
  * on = 1 uses device memory for every size (same results; for checking the
  * large-N path on small tours). */
 int mlp_merge_force_global(mlp_ctx* ctx, int32_t on);

@@ -872,26 +872,32 @@ __global__ __launch_bounds__(kBlock) void k_total_eval(const SlotDev* __restrict
 }
 
 // Sort keys of the entries: an improving entry's delta + bias (bias >= every
-// |delta|), bias for the rest; values = entry index.  A stable sort by these
-// keys orders the entries by (cost, entry) like the reference's stable sort.
-__global__ void k_total_keys(const int64_t* __restrict__ tk, int64_t M, uint64_t bias,
+// |delta|), bias for the rest.  Values: row << 15 | ctx (N <= 32768), from
+// which the merge recovers the move and the entry index row * B + ctx mod B
+// without reading the entry again.  Keys are written in entry order, so a
+// stable sort orders the entries by (cost, entry) like the reference's.
+__global__ void k_total_keys(const int64_t* __restrict__ tk, int64_t M, int B, uint64_t bias,
                              uint64_t* __restrict__ ku, int32_t* __restrict__ ve) {
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < M;
        e += (int64_t)gridDim.x * kBlock) {
     const int64_t c = tk[e];
     uint64_t u = bias;
+    int32_t v = 0;
     if (c != MLP_KEY_NONE) {
       const int64_t d = c >> MLP_KEY_SHIFT;
-      if (d < 0) u = d + (int64_t)bias > 0 ? (uint64_t)(d + (int64_t)bias) : 0;
+      if (d < 0) {
+        u = d + (int64_t)bias > 0 ? (uint64_t)(d + (int64_t)bias) : 0;
+        v = (int32_t)((e / B) << 15) | (int32_t)(c & MLP_KEY_MASK);
+      }
     }
     ku[e] = u;
-    ve[e] = (int32_t)e;
+    ve[e] = v;
   }
 }
 
 namespace {
 
-constexpr int kTotalPer = 4;                       // candidates per thread per chunk
+constexpr int kTotalPer = 8;                       // candidates per thread per chunk
 constexpr int kTotalChunk = kTotalPer * kBlock;    // candidates decoded per chunk
 
 struct TotalLds {
@@ -952,13 +958,13 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
 // list is consumed in chunks: all threads decode a chunk and drop the
 // candidates that already conflict with a kept move (the kept set only
 // grows, so they would be dropped anyway), keeping the survivors in order;
-// wave 0 then runs the greedy over the survivors.  The first non-improving
-// key ends the list.  Dynamic LDS: bits | told[N] | tnew[N] (told/tnew in the
-// slot's merge scratch when GMEM).
+// wave 0 then runs the greedy over the survivors while the next chunk's keys
+// are already being loaded.  The first non-improving key ends the list.
+// Dynamic LDS: bits | told[N] | tnew[N] (told/tnew in the slot's merge
+// scratch when GMEM).
 template <bool GMEM>
 __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restrict__ table, int id,
                                                         int nb, int B, int64_t M, uint64_t bias,
-                                                        const int64_t* __restrict__ tk,
                                                         const uint64_t* __restrict__ ku,
                                                         const int32_t* __restrict__ ve) {
   extern __shared__ int64_t lds[];
@@ -976,40 +982,51 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
     T.end = 0;
     T.kept = 0;
   }
-  __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
+  // raw keys / values of the next chunk (bias = past the end or not improving)
+  uint64_t nk[kTotalPer];
+  int32_t nv[kTotalPer];
+#pragma unroll
+  for (int u = 0; u < kTotalPer; ++u) {
+    const int64_t q = (int64_t)u * kBlock + tid;
+    nk[u] = q < M ? ku[q] : bias;
+    nv[u] = q < M ? ve[q] : 0;
+  }
+  __syncthreads();
   for (int64_t base = 0; base < M; base += kTotalChunk) {
-    int ci[kTotalPer], cj[kTotalPer], ce[kTotalPer];
-    int64_t cc[kTotalPer];
-    uint64_t bal[kTotalPer];
+    uint64_t ck[kTotalPer];
+    int32_t cv[kTotalPer];
 #pragma unroll
     for (int u = 0; u < kTotalPer; ++u) {
-      const int64_t q = base + u * kBlock + tid;
+      ck[u] = nk[u];
+      cv[u] = nv[u];
+      const int64_t q = base + kTotalChunk + (int64_t)u * kBlock + tid;
+      nk[u] = q < M ? ku[q] : bias;
+      nv[u] = q < M ? ve[q] : 0;
+    }
+    int ci[kTotalPer], cj[kTotalPer];
+    uint64_t bal[kTotalPer];
+    bool end = false;
+#pragma unroll
+    for (int u = 0; u < kTotalPer; ++u) {
       bool pass = false;
-      ci[u] = cj[u] = ce[u] = 0;
-      cc[u] = 0;
-      if (q < M) {
-        const uint64_t key = ku[q];
-        if (key < bias) {
-          const int e = ve[q];
-          const int ctx = (int)(tk[e] & MLP_KEY_MASK);
-          const int row = e / B;
-          int i, j;
-          move_ij(nb, 0, N, row, ctx, i, j);
-          ci[u] = i;
-          cj[u] = j;
-          ce[u] = e;
-          cc[u] = (int64_t)key - (int64_t)bias;
-          if (nb == 0) pass = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
-          else if (nb == 1) pass = !bits_any(bits, i, j + 1);
-          else pass = !bits_any(bits, min(i, j), max(i, j) + k);
-        } else {
-          T.end = 1;
-        }
+      ci[u] = cj[u] = 0;
+      if (ck[u] < bias) {
+        const int row = cv[u] >> 15, ctx = cv[u] & 0x7fff;
+        int i, j;
+        move_ij(nb, 0, N, row, ctx, i, j);
+        ci[u] = i;
+        cj[u] = j;
+        if (nb == 0) pass = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+        else if (nb == 1) pass = !bits_any(bits, i, j + 1);
+        else pass = !bits_any(bits, min(i, j), max(i, j) + k);
+      } else {
+        end = true;  // past the list or the first non-improving key
       }
       bal[u] = __ballot(pass);
       if (lane == 0) T.wc[u][wave] = __popcll(bal[u]);
     }
+    if (end) T.end = 1;
     __syncthreads();
     int S = 0;
 #pragma unroll
@@ -1021,15 +1038,16 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
       }
       if ((bal[u] >> lane) & 1ull) {
         const int at = off + __popcll(bal[u] & lt);
+        const int row = cv[u] >> 15, ctx = cv[u] & 0x7fff;
         T.ci[at] = ci[u];
         T.cj[at] = cj[u];
-        T.ce[at] = ce[u];
-        T.cc[at] = cc[u];
+        T.ce[at] = row * B + ctx % B;
+        T.cc[at] = (int64_t)ck[u] - (int64_t)bias;
       }
     }
     __syncthreads();
     // read before the barrier below: the next chunk's decode may set it
-    const bool end = T.end != 0;
+    const bool stop = T.end != 0;
     if (wave == 0 && S > 0) {
       int kept = T.kept;
 #define MLP_TGREEDY(NB) \
@@ -1044,7 +1062,7 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
       if (lane == 0) T.kept = kept;
     }
     __syncthreads();
-    if (end) break;
+    if (stop) break;
   }
   const int K = T.kept;
   const int32_t* kij = s.kept_ij[nb];
@@ -1805,7 +1823,7 @@ size_t mlp_total_merge_lds_bytes(int N) {
 
 // The merge kernel's own static LDS (chunk arrays, prefix scratch) is below
 // kTotalStaticLds; its dynamic part may use the rest.
-constexpr size_t kTotalStaticLds = 24 * 1024;
+constexpr size_t kTotalStaticLds = 48 * 1024;
 static_assert(sizeof(TotalLds) + sizeof(PrepLds) <= kTotalStaticLds, "static LDS estimate");
 
 size_t mlp_total_static_lds() { return kTotalStaticLds; }
@@ -1828,24 +1846,23 @@ hipError_t mlp_launch_total_eval(hipStream_t st, const SlotDev* table, int id, i
   return hipGetLastError();
 }
 
-hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, uint64_t bias,
-                                 uint64_t* ku, int32_t* ve) {
+hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, int B,
+                                 uint64_t bias, uint64_t* ku, int32_t* ve) {
   const int64_t blocks = std::min<int64_t>((M + kBlock - 1) / kBlock, 4096);
   hipLaunchKernelGGL(k_total_keys, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(kBlock), 0,
-                     st, tk, M, bias, ku, ve);
+                     st, tk, M, B, bias, ku, ve);
   return hipGetLastError();
 }
 
 hipError_t mlp_launch_total_merge(hipStream_t st, const SlotDev* table, int id, int nb, int N,
-                                  int B, int64_t M, uint64_t bias, const int64_t* tk,
-                                  const uint64_t* ku, const int32_t* ve, int gmem) {
+                                  int B, int64_t M, uint64_t bias, const uint64_t* ku,
+                                  const int32_t* ve, int gmem) {
   if (gmem)
     hipLaunchKernelGGL(k_total_merge<true>, dim3(1), dim3(kBlock),
-                       (size_t)((N + 32) / 32 + 1) * 4, st, table, id, nb, B, M, bias, tk, ku,
-                       ve);
+                       (size_t)((N + 32) / 32 + 1) * 4, st, table, id, nb, B, M, bias, ku, ve);
   else
     hipLaunchKernelGGL(k_total_merge<false>, dim3(1), dim3(kBlock), mlp_total_merge_lds_bytes(N),
-                       st, table, id, nb, B, M, bias, tk, ku, ve);
+                       st, table, id, nb, B, M, bias, ku, ve);
   return hipGetLastError();
 }
 

@@ -35,11 +35,11 @@ size_t mlp_total_static_lds();
 hipError_t mlp_total_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_total_eval(hipStream_t st, const SlotDev* table, int id, int nb, int rows,
                                  int B, int64_t* out);
-hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, uint64_t bias,
-                                 uint64_t* ku, int32_t* ve);
+hipError_t mlp_launch_total_keys(hipStream_t st, const int64_t* tk, int64_t M, int B,
+                                 uint64_t bias, uint64_t* ku, int32_t* ve);
 hipError_t mlp_launch_total_merge(hipStream_t st, const SlotDev* table, int id, int nb, int N,
-                                  int B, int64_t M, uint64_t bias, const int64_t* tk,
-                                  const uint64_t* ku, const int32_t* ve, int gmem);
+                                  int B, int64_t M, uint64_t bias, const uint64_t* ku,
+                                  const int32_t* ve, int gmem);
 // Stable LSD radix sort of (key, value) pairs on bits [0, end_bit) of the
 // keys (mlp_sort.hip).  tmp == nullptr: *tmp_bytes receives the scratch size.
 hipError_t mlp_sort_pairs_u64(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,

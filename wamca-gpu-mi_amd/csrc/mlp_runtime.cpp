@@ -1467,7 +1467,7 @@ int mlp_merge_total(mlp_ctx* ctx, const int32_t* slots, int32_t count) {
                      mlp_total_merge_lds_bytes(N) + mlp_total_static_lds() > ctx->max_total_lds;
     if (M == 0) {
       HCK(mlp_launch_total_merge(ctx->stream, ctx->d_table, id, nb, N, B, 0, bias, nullptr,
-                                 nullptr, nullptr, gmem));
+                                 nullptr, gmem));
     } else {
       void *ku = nullptr, *ko = nullptr, *vi = nullptr, *vo = nullptr, *tmp = nullptr;
       size_t cku = 0, cko = 0, cvi = 0, cvo = 0, ctmp = 0, tbytes = 0;
@@ -1480,12 +1480,13 @@ int mlp_merge_total(mlp_ctx* ctx, const int32_t* slots, int32_t count) {
       if (!st) st = pool_alloc(ctx, std::max<size_t>(tbytes, 1), &tmp, &ctmp);
       hipError_t e = hipSuccess;
       if (!st) {
-        e = mlp_launch_total_keys(ctx->stream, s.total, M, bias, (uint64_t*)ku, (int32_t*)vi);
+        e = mlp_launch_total_keys(ctx->stream, s.total, M, B, bias, (uint64_t*)ku,
+                                  (int32_t*)vi);
         if (e == hipSuccess)
           e = mlp_sort_pairs_u64(tmp, &tbytes, (const uint64_t*)ku, (uint64_t*)ko,
                                  (const int32_t*)vi, (int32_t*)vo, (int)M, end_bit, ctx->stream);
         if (e == hipSuccess)
-          e = mlp_launch_total_merge(ctx->stream, ctx->d_table, id, nb, N, B, M, bias, s.total,
+          e = mlp_launch_total_merge(ctx->stream, ctx->d_table, id, nb, N, B, M, bias,
                                      (const uint64_t*)ko, (const int32_t*)vo, gmem);
       }
       pool_release(ctx, ku, cku);  // stream-ordered: reused only by later work
