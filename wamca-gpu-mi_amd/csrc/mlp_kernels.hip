@@ -263,14 +263,25 @@ __device__ __forceinline__ EvalSeg find_seg(const EvalSeg* segs, int nsegs, int 
   return segs[lo];
 }
 
-// Inline (kernel-argument) segment table: constant indices only, so the
-// argument stays in the kernarg segment instead of being copied to scratch.
-__device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, int nsegs, int item) {
-  EvalSeg sg = inl.s[0];
+// Inline (kernel-argument) segment tables.  The argument stays in the
+// kernarg segment and a dynamic index is one load from it.  One or two
+// segments (the launches of a single slot) are read together and selected;
+// otherwise the blk0 column is read in one batch and counted, then the
+// segment is loaded.
+template <typename Seg, typename Inl>
+__device__ __forceinline__ Seg find_inline(const Inl& inl, int nsegs, int item) {
+  if (nsegs <= 2) {
+    const Seg a = inl.s[0], b = inl.s[1];
+    return (nsegs == 2 && item >= b.blk0) ? b : a;
+  }
+  int q = 0;
 #pragma unroll
-  for (int q = 1; q < MLP_SEGS_INLINE; ++q)
-    if (q < nsegs && inl.s[q].blk0 <= item) sg = inl.s[q];
-  return sg;
+  for (int t = 1; t < MLP_SEGS_INLINE; ++t) q += (t < nsegs && inl.s[t].blk0 <= item) ? 1 : 0;
+  return inl.s[q];
+}
+
+__device__ __forceinline__ EvalSeg find_seg_inline(const EvalSegsInline& inl, int nsegs, int item) {
+  return find_inline<EvalSeg>(inl, nsegs, item);
 }
 
 __device__ __forceinline__ void eval_item(const SlotDev* __restrict__ table, const EvalSeg& sg, int item,
@@ -304,12 +315,7 @@ __device__ __forceinline__ void eval_item(const SlotDev* __restrict__ table, con
 }
 
 __device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t* dev, int q) {
-  if (dev) return dev[q];
-  int v = inl.id[0];
-#pragma unroll
-  for (int t = 1; t < MLP_SLOTS_INLINE; ++t)
-    if (t == q) v = inl.id[t];
-  return v;
+  return dev ? dev[q] : inl.id[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,14 +1171,7 @@ __global__ __launch_bounds__(kBlock) void k_clear_rows(const SlotDev* __restrict
                                                        SweepSegsInline inl,
                                                        const SweepSeg* segs_dev, int nsegs) {
   if ((int)blockIdx.x >= nsegs) return;
-  SweepSeg sg = inl.s[0];
-  if (segs_dev) {
-    sg = segs_dev[blockIdx.x];
-  } else {
-#pragma unroll
-    for (int q = 1; q < MLP_SEGS_INLINE; ++q)
-      if (q == (int)blockIdx.x) sg = inl.s[q];
-  }
+  const SweepSeg sg = segs_dev ? segs_dev[blockIdx.x] : inl.s[blockIdx.x];
   const SlotDev& s = table[sg.slot];
   for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
     if ((sg.mask >> nb) & 1) {
@@ -1531,6 +1530,23 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     L.rxy[tid] = rxv;
   }
   __syncthreads();
+  if (R == kBlock / kWave && __popc(mask & 0x1d) >= 2) {
+    // Several neighborhoods on a four-row tile: wave w takes the w-th
+    // neighborhood of the mask over all rows, so the neighborhoods run side
+    // by side instead of one after another (the latency of a one-tour launch).
+    int nb = -1, q = wave;
+#pragma unroll
+    for (int b = 0; b < MLP_NB_COUNT; ++b)
+      if (b != 1 && ((mask >> b) & 1)) {
+        if (q == 0) nb = b;
+        --q;
+      }
+    if (nb == 0) tile_swap<R, C, R>(s, b0, q0, q1, cbase, L, parity, 0, lane);
+    else if (nb == 2) tile_oropt<R, C, R, 1>(s, b0, q0, q1, cbase, L, parity, 0, lane);
+    else if (nb == 3) tile_oropt<R, C, R, 2>(s, b0, q0, q1, cbase, L, parity, 0, lane);
+    else if (nb == 4) tile_oropt<R, C, R, 3>(s, b0, q0, q1, cbase, L, parity, 0, lane);
+    return;
+  }
   if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane);
   if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane);
   if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane);
@@ -1651,11 +1667,7 @@ __device__ __forceinline__ SweepSeg find_sweep(const SweepSeg* segs, int nsegs, 
 
 __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl, int nsegs,
                                                       int item) {
-  SweepSeg sg = inl.s[0];
-#pragma unroll
-  for (int q = 1; q < MLP_SEGS_INLINE; ++q)
-    if (q < nsegs && inl.s[q].blk0 <= item) sg = inl.s[q];
-  return sg;
+  return find_inline<SweepSeg>(inl, nsegs, item);
 }
 
 template <int R, int C>
