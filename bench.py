@@ -200,6 +200,22 @@ def dvnd_legs(eng, args):
             "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
             "ratio": sv["solves_per_s"] / pl["solves_per_s"],
             "identical_results": sv["latency"] == pl["latency"]}
+    # Poisson rate sweep, n log-uniform [100, 1000] (no n = 18000 jobs): about
+    # one second of arrivals per rate (60..300 jobs)
+    sweep = {}
+    for rate in (50.0, 200.0, 500.0, 1000.0, 2000.0, 5000.0):
+        cnt = int(min(300, max(60, rate)))
+        jobs, _ = WL.dvnd_jobs(cnt, 5, lo=100, hi=1000)
+        arr = WL.poisson_arrivals(rate, cnt, 41)
+        sv = serve.run_served(eng, jobs, arr)
+        pl = serve.run_plain(eng, jobs, arr)
+        sweep[str(int(rate))] = {
+            "jobs": cnt,
+            "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s")},
+            "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s")},
+            "ratio": sv["solves_per_s"] / pl["solves_per_s"],
+            "identical_results": sv["latency"] == pl["latency"]}
+    res["rate_sweep_100_1000"] = sweep
     return res
 
 
