@@ -603,8 +603,8 @@ __device__ __forceinline__ bool conflicts_nb(int i1, int j1, int i2, int j2) {
 // time.  The bitmap of positions claimed by kept moves is an exact conflict
 // test against every move kept so far (swap: positions i-1..i+1, j-1..j+1;
 // 2-opt: [i, j+1]; or-opt: [min(i,j), max(i,j)+k]); within a batch the
-// candidates that pass it are decided in order by a scalar loop, each
-// against the batch's kept set through one ballot.  Kept moves go to keys[q]
+// candidates that pass it are decided in order by a scalar loop over the
+// kept ones.  Kept moves go to keys[q]
 // as (i << 32 | j) and to the slot's kept arrays.  Returns their count.
 template <int NB>
 __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* keys,
@@ -629,13 +629,16 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
       if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
       else pre = !bits_any(bits, a, b);
     }
+    // the first remaining candidate is kept and every candidate it
+    // conflicts with leaves the batch (the predicates are symmetric), so the
+    // loop runs once per kept move
     uint64_t m = __ballot(pre), kept = 0;
     while (m) {
       const int q = __builtin_ctzll(m);
-      m &= m - 1;
+      kept |= 1ull << q;
       const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
       const bool c = conflicts_nb<NB>(iq, jq, i, j);
-      if ((__ballot(c) & kept) == 0) kept |= 1ull << q;
+      m &= ~(__ballot(c) | (1ull << q));
     }
     if ((kept >> lane) & 1ull) {
       if (NB == 0) {
@@ -750,38 +753,53 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
     if (tid == 0) s.info[MLP_INFO_KEPT0 + nb] = 0;
     return;
   }
-  int P = 1;
-  while (P < M) P <<= 1;
-  for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
-  __syncthreads();
-  // 2. bitonic sort by (cost, row) == stable sort by cost of the row list.
-  // Each phase has P/2 compare-swaps; a thread takes up to four pairs at a
-  // time, loading all of them before comparing.
-  {
-    const int half = P >> 1;
-    for (int kk = 2; kk <= P; kk <<= 1) {
-      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        const int lj = 31 - __clz(jj);
-        for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
-          int lo[4];
-          int64_t va[4], vb[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int pr = min(p0 + u * kBlock, half - 1);
-            lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
-            va[u] = keys[lo[u]];
-            vb[u] = keys[lo[u] + jj];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const bool up = (lo[u] & kk) == 0;
-            if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
-              keys[lo[u]] = vb[u];
-              keys[lo[u] + jj] = va[u];
+  // 2. sort by (cost, row) == stable sort by cost of the row list.  Up to
+  // one key per lane of a wave, keys in LDS: rank sort on wave 0, which also
+  // runs the greedy next (the keys are distinct; a key's rank is the number
+  // of smaller keys; every lane reads all keys before any lane writes).
+  // Otherwise a bitonic sort over the workgroup.
+  if (!GMEM && M <= kWave) {
+    if (tid < kWave) {
+      const int64_t my = tid < M ? keys[tid] : 0;
+      int rank = 0;
+      for (int u = 0; u < M; ++u) rank += keys[u] < my ? 1 : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (tid < M) keys[rank] = my;
+      __builtin_amdgcn_wave_barrier();
+    }
+  } else {
+    int P = 1;
+    while (P < M) P <<= 1;
+    for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
+    __syncthreads();
+    // Each phase has P/2 compare-swaps; a thread takes up to four pairs at a
+    // time, loading all of them before comparing.
+    {
+      const int half = P >> 1;
+      for (int kk = 2; kk <= P; kk <<= 1) {
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+          const int lj = 31 - __clz(jj);
+          for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
+            int lo[4];
+            int64_t va[4], vb[4];
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int pr = min(p0 + u * kBlock, half - 1);
+              lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
+              va[u] = keys[lo[u]];
+              vb[u] = keys[lo[u] + jj];
+            }
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const bool up = (lo[u] & kk) == 0;
+              if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
+                keys[lo[u]] = vb[u];
+                keys[lo[u] + jj] = va[u];
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
   }
@@ -932,13 +950,16 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
       if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
       else pre = !bits_any(bits, a, b);
     }
+    // the first remaining candidate is kept and every candidate it
+    // conflicts with leaves the batch (the predicates are symmetric), so the
+    // loop runs once per kept move
     uint64_t m = __ballot(pre), kept = 0;
     while (m) {
       const int q = __builtin_ctzll(m);
-      m &= m - 1;
+      kept |= 1ull << q;
       const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
       const bool c = conflicts_nb<NB>(iq, jq, i, j);
-      if ((__ballot(c) & kept) == 0) kept |= 1ull << q;
+      m &= ~(__ballot(c) | (1ull << q));
     }
     if ((kept >> lane) & 1ull) {
       if (NB == 0) {
