@@ -330,3 +330,15 @@ def test_sqrt_exhaustive_vs_library(eng):
     """The kernels' square root equals the library's correctly rounded sqrtf
     for float(s) of every s in [0, 2^31)."""
     assert eng.sqrt_selfcheck() == 0
+
+
+def test_instance_beyond_64bit_keys_is_refused(eng):
+    """n = 14000 over the full coordinate range: N (N + 32) maxd >= 2^43."""
+    rng = np.random.RandomState(3)
+    xs = rng.randint(0, 32768, 14000).astype(np.int32)
+    ys = rng.randint(0, 32768, 14000).astype(np.int32)
+    xs[0], ys[0], xs[1], ys[1] = 0, 0, 32767, 32767
+    with pytest.raises(wm.MLPError):
+        eng.instance(xs, ys, 1)
+    inst = eng.instance(xs[:12000], ys[:12000], 1)  # below the bound
+    eng.destroy_instance(inst)

@@ -210,3 +210,26 @@ def test_oropt_k_beyond_3(n, seed, rowmap):
         if O.rows(nb, rowmap, N) > 0:
             r = O.eval_rows(nb, rowmap, W, tf)
             assert (r["cost"] >= 0).all(), nb
+
+
+@pytest.mark.parametrize("n,seed,extreme", [(14, 1, True), (20, 2, True), (40, 3, False),
+                                            (90, 4, True)])
+def test_delta_bound_of_64bit_keys(n, seed, extreme):
+    """Every |delta| of every neighborhood id 0..14 is below N (N + 32) maxd:
+    the bound instance creation checks against 2^43 (row keys delta * 2^20 +
+    index) and the TOTAL merge uses as its sort bias."""
+    span = 3000
+    xs, ys = _extreme_instance(n, seed, span) if extreme else O.instance_uniform(n, seed, span)
+    W = O.weights(xs, ys, 1)
+    maxd = int(W.max())
+    N = n + 1
+    tours = [O.perm_tour(seed + q, n) for q in range(2)]
+    if extreme:
+        tours.append(np.array([0] + list(range(1, n)) + [0], np.int32))
+    for t in tours:
+        for nb in range(15):
+            if O.rows(nb, 0, N) <= 0:
+                continue
+            m = O.enumerate_moves(nb, 0, W, t)
+            if len(m["ads"]):
+                assert int(np.abs(m["ads"]).max()) < N * (N + 32) * maxd, nb

@@ -496,6 +496,12 @@ static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, i
   in.round = round;
   const double dx = xmax - xmin, dy = ymax - ymin;
   in.maxd = (int)std::ceil(std::sqrt(dx * dx + dy * dy)) + 1;
+  // move keys hold delta * 2^20 + index in 64 bits: every |delta| is below
+  // N (N + 32) maxd, which must stay below 2^43
+  const uint64_t N = (uint64_t)n + 1;
+  if (N * (N + 32) * (uint64_t)in.maxd >= (1ull << 43))
+    return fail(ctx, MLP_ERANGE, "n = %d with a coordinate span of %d exceeds the 64-bit move keys",
+                n, in.maxd);
   HCK(hipSetDevice(ctx->device));
   const size_t half = align_up(4 * (size_t)n, 256);
   void* mem = nullptr;
