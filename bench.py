@@ -299,6 +299,7 @@ def main():
         eng.eval_each(slots, 0x1F, 0)
     region_ms = eng.region_end()
     dt = time.perf_counter() - t0
+    eng.set_tile(-1)  # later legs use the autotuned table
     launches = args.steps * T
     value = launches * per_tour / dt
     avg_launch_s = region_ms / 1e3 / launches

@@ -474,8 +474,24 @@ __global__ __launch_bounds__(kBlock) void k_dvnd_start(const SlotDev* __restrict
     const SlotDev& s = table[list_at(list, list_dev, t)];
     s.info[MLP_INFO_ACTIVE] = 1;
     s.info[MLP_INFO_ROUNDS] = 0;
+    s.info[MLP_INFO_ROUND_MOVES] = 0;
   }
 }
+
+namespace {
+
+// End of a DVND round for one active slot (run by the round's last merge):
+// count the round; a round that applied no move (or reached max_rounds > 0)
+// stops the slot; the next round counts its moves from zero.
+__device__ __forceinline__ void round_tail(const SlotDev& s, int max_rounds) {
+  const int64_t r = s.info[MLP_INFO_ROUNDS] + 1;
+  s.info[MLP_INFO_ROUNDS] = r;
+  if (s.info[MLP_INFO_ROUND_MOVES] == 0 || (max_rounds > 0 && r >= max_rounds))
+    s.info[MLP_INFO_ACTIVE] = 0;
+  s.info[MLP_INFO_ROUND_MOVES] = 0;
+}
+
+}  // namespace
 
 // Start of a DVND round: active slots count their moves from zero; the
 // persistent-grid work counters of the round's launches are cleared (a
@@ -698,12 +714,14 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
 }  // namespace
 
 // One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
+// tail != 0: this is the last merge of a DVND round, and it also ends the
+// round for each active slot (round_tail, max_rounds = tail - 1).
 template <bool GMEM>
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
                                                         int rowmap, int pcap_l, int ncap_l,
-                                                        int skip_inactive) {
+                                                        int skip_inactive, int tail) {
   extern __shared__ int64_t lds[];
   __shared__ int s_m, s_kept, s_long;
   __shared__ int s_longq[kLongMoves];
@@ -750,7 +768,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   __syncthreads();
   const int M = s_m;
   if (M == 0) {
-    if (tid == 0) s.info[MLP_INFO_KEPT0 + nb] = 0;
+    if (tid == 0) {
+      s.info[MLP_INFO_KEPT0 + nb] = 0;
+      if (tail) round_tail(s, tail - 1);
+    }
     return;
   }
   // 2. sort by (cost, row) == stable sort by cost of the row list.  Up to
@@ -826,6 +847,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
     s.info[MLP_INFO_KEPT0 + nb] = K;
     s.info[MLP_INFO_ROUND_MOVES] += K;
     s.info[MLP_INFO_TOTAL_MOVES] += K;
+    if (tail) round_tail(s, tail - 1);
   }
 }
 
@@ -1839,14 +1861,14 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds) {
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive, int gmem) {
+                                  int pcap, int ncap, int skip_inactive, int gmem, int tail) {
   if (gmem)
     hipLaunchKernelGGL(k_merge_apply<true>, dim3(count), dim3(kBlock), 0, st, table, inl,
-                       list_dev, nb, rowmap, pcap, ncap, skip_inactive);
+                       list_dev, nb, rowmap, pcap, ncap, skip_inactive, tail);
   else
     hipLaunchKernelGGL(k_merge_apply<false>, dim3(count), dim3(kBlock),
                        mlp_merge_lds_bytes(pcap, ncap), st, table, inl, list_dev, nb, rowmap, pcap,
-                       ncap, skip_inactive);
+                       ncap, skip_inactive, tail);
   return hipGetLastError();
 }
 

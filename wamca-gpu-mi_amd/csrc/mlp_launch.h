@@ -26,9 +26,10 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
 void mlp_tile_shape(int tile, int* rows, int* cols);
 size_t mlp_merge_lds_bytes(int pcap, int ncap);
 hipError_t mlp_merge_setup(size_t max_dyn_lds);
+// tail > 0: the last merge of a DVND round also ends the round (max_rounds = tail - 1).
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive, int gmem);
+                                  int pcap, int ncap, int skip_inactive, int gmem, int tail);
 // TOTAL mode: entries rows * B of nb, their sort keys, the merge + apply.
 size_t mlp_total_merge_lds_bytes(int N);
 size_t mlp_total_static_lds();

@@ -1299,7 +1299,10 @@ int mlp_get_moves64(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* moves, int
   return MLP_OK;
 }
 
-static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb, int skip) {
+// tail > 0: the last merge of a DVND round, which also ends the round
+// (max_rounds = tail - 1).
+static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t nb, int skip,
+                      int tail = 0) {
   if (count == 0) return MLP_OK;
   int rowmap = -1, pcap = 1, ncap = 1;
   std::vector<int32_t> in_lds, in_gmem;
@@ -1339,7 +1342,7 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
       int st = slot_list(ctx, list.data() + q0, c, inl, &dev);
       if (st) return st;
       HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
-                                 skip, g));
+                                 skip, g, tail));
     }
   }
   return MLP_OK;
@@ -1519,9 +1522,16 @@ int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int
   const int32_t* dev = nullptr;
   int st = slot_list(ctx, slots, count, inl, &dev);
   if (st) return st;
-  HCK(mlp_launch_round_begin(ctx->stream, ctx->d_table, inl, dev, count, ctx->d_counter + 1,
-                             ctx->dvnd_nbs));
-  for (int nb = 0; nb < ctx->dvnd_nbs; ++nb) {
+  // the persistent grids' work counters are cleared at the round's start;
+  // the round's moves are counted from zero since the previous round's end
+  if (ctx->dvnd_persistent)
+    HCK(mlp_launch_round_begin(ctx->stream, ctx->d_table, inl, dev, count, ctx->d_counter + 1,
+                               ctx->dvnd_nbs));
+  int last = -1;
+  for (int nb = 0; nb < ctx->dvnd_nbs; ++nb)
+    for (int q = 0; q < count && last != nb; ++q)
+      if (rows_of(nb, rowmap, ctx->slots[slots[q]].N) > 0) last = nb;
+  for (int nb = 0; nb <= last; ++nb) {
     bool any = false;
     for (int q = 0; q < count && !any; ++q)
       if (rows_of(nb, rowmap, ctx->slots[slots[q]].N) > 0) any = true;
@@ -1529,10 +1539,11 @@ int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int
     st = eval_impl(ctx, slots, count, 1u << nb, rowmap, ctx->dvnd_persistent, 1,
                    ctx->d_counter + 1 + nb);
     if (st) return st;
-    st = merge_impl(ctx, slots, count, nb, 1);
+    // the round's last merge also ends the round for each slot
+    st = merge_impl(ctx, slots, count, nb, 1, nb == last ? max_rounds + 1 : 0);
     if (st) return st;
   }
-  HCK(mlp_launch_round_end(ctx->stream, ctx->d_table, inl, dev, count, max_rounds));
+  if (last < 0) HCK(mlp_launch_round_end(ctx->stream, ctx->d_table, inl, dev, count, max_rounds));
   return MLP_OK;
 }
 
