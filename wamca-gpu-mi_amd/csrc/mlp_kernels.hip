@@ -1290,6 +1290,16 @@ __device__ __forceinline__ uint32_t umul(uint32_t a, uint32_t b) { return __umul
 // signed 24-bit multiply, wrapped to 32 bits (|a|, |b| < 2^23)
 __device__ __forceinline__ uint32_t smul(int a, int b) { return (uint32_t)__mul24(a, b); }
 
+// Word 3 of a raw buffer descriptor on gfx9-family parts (untyped dword
+// access, no swizzle).
+constexpr int kRsrcWord3 = 0x00020000;
+
+// Wave index as a wave-uniform value (kept in a scalar register, so that
+// branches on it are uniform branches).
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 __device__ __forceinline__ int64_t pair_key(int32_t d, int32_t c) {
   return c < 0 ? MLP_KEY_NONE : (int64_t)d * (int64_t)(1 << MLP_KEY_SHIFT) + c;
 }
@@ -1496,7 +1506,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
 // (neighborhood, row) over the whole row.
 template <int R>
 __device__ __forceinline__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int parity) {
-  const int N = s.N, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int N = s.N, wave = wave_id(), lane = threadIdx.x % kWave;
   const int b0 = rb * R;
   if (ct != 0) return;
 #pragma unroll
@@ -1518,7 +1528,7 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
                             TileLds<R, C>& L) {
   constexpr int RW = R / (kBlock / kWave);
   constexpr int FI = (C + 4 + kBlock - 1) / kBlock;  // fill iterations per thread
-  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const int N = s.N, tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int b0 = rb * R;
   const int q0 = 1 + ct * C;                 // move j in [q0, q1)
@@ -1610,7 +1620,7 @@ template <int C>
 __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int parity, DiagLds<C>& L) {
   constexpr int FE = (C + 1 + kWave - 1) / kWave;  // e entries per lane and part
   constexpr int CU = 4;                            // compute iterations per load batch
-  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const int N = s.N, tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int rows = (N + 1) >> 1;
   const int par = (parity >> 1) & 1;
@@ -1662,37 +1672,70 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
     }
   }
   __syncthreads();
+  // The prefix arrays are read from global memory in batches of CU x 64
+  // positions; the batches of both parts form one sequence and batch k + 1
+  // is loaded while batch k is evaluated, so each batch's memory latency
+  // hides behind the previous batch's arithmetic.
   int32_t bd = INT32_MAX, bc = -1;
+  const int nb0 = ihi[0] >= ilo[0] ? (ihi[0] - ilo[0]) / (CU * kWave) + 1 : 0;
+  const int nb1 = ihi[1] >= ilo[1] ? (ihi[1] - ilo[1]) / (CU * kWave) + 1 : 0;
+  uint32_t gi[CU], hj[CU], ti[CU], tj[CU];
+  // raw buffer loads: one scalar descriptor per array and a 32-bit byte
+  // offset per lane (no 64-bit address arithmetic); reads beyond 4N bytes
+  // would return 0, none is issued (indices are clamped to the row)
+  const int nbytes = 4 * N;
+  const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc((void*)s.g32, 0, nbytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t rH = __builtin_amdgcn_make_buffer_rsrc((void*)s.h32, 0, nbytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t rT = __builtin_amdgcn_make_buffer_rsrc((void*)s.t32, 0, nbytes, kRsrcWord3);
+  auto load_batch = [&](int k, uint32_t* g, uint32_t* h, uint32_t* a, uint32_t* b) {
+    const int part = k < nb0 ? 0 : 1;
+    const int lo = ilo[part], hi = ihi[part], m = mm[part];
+    const int i0 = lo + (k - (part ? nb0 : 0)) * CU * kWave;
 #pragma unroll
-  for (int part = 0; part < 2; ++part) {
+    for (int u = 0; u < CU; ++u) {
+      const uint32_t i = (uint32_t)min(i0 + lane + u * kWave, hi);
+      const uint32_t jm = i + (uint32_t)m;
+      g[u] = __builtin_amdgcn_raw_buffer_load_b32(rG, 4u * i, 0, 0);
+      h[u] = __builtin_amdgcn_raw_buffer_load_b32(rH, 4u * jm, 0, 0);
+      a[u] = __builtin_amdgcn_raw_buffer_load_b32(rT, 4u * i, 0, 0);
+      b[u] = __builtin_amdgcn_raw_buffer_load_b32(rT, 4u * jm, 0, 0);
+    }
+  };
+  // A lane meets its indices in increasing i: ctx = by - i falls in part
+  // 0 and ctx = i + by - 1 rises in part 1 (above every part-0 ctx), so
+  // the lowest ctx among equal costs is the last equal one in part 0
+  // (<=) and the first in part 1 (<).
+  auto eval_batch = [&](int k, const uint32_t* g, const uint32_t* h, const uint32_t* a,
+                        const uint32_t* b) {
+    const int part = k < nb0 ? 0 : 1;
     const uint16_t* e = L.e[wave][part];
     const int m = mm[part], lo = ilo[part], hi = ihi[part];
-    for (int i0 = lo; i0 <= hi; i0 += CU * kWave) {
-      uint32_t gi[CU], hj[CU], ti[CU], tj[CU];
+    const int i0 = lo + (k - (part ? nb0 : 0)) * CU * kWave;
 #pragma unroll
-      for (int u = 0; u < CU; ++u) {
-        const int i = min(i0 + lane + u * kWave, hi);
-        gi[u] = (uint32_t)ldg(s.g32 + i);
-        hj[u] = (uint32_t)ldg(s.h32 + i + m);
-        ti[u] = (uint32_t)ldg(s.t32 + i);
-        tj[u] = (uint32_t)ldg(s.t32 + i + m);
-      }
-#pragma unroll
-      for (int u = 0; u < CU; ++u) {
-        const int i = i0 + lane + u * kWave;
-        const int ic = min(i, hi);
-        const int j = ic + m;
-        const int x = ic - lo;
-        uint32_t d = gi[u] + hj[u];
-        d += umul((uint32_t)(N - ic), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
-        d += (uint32_t)m * (ti[u] + tj[u]);
-        const int ctx = part == 0 ? by - ic : ic + by - 1;
-        if (i <= hi && ((int32_t)d < bd || ((int32_t)d == bd && ctx < bc))) {
-          bd = (int32_t)d;
-          bc = ctx;
-        }
-      }
+    for (int u = 0; u < CU; ++u) {
+      const int i = i0 + lane + u * kWave;
+      const int ic = min(i, hi);
+      const int j = ic + m;
+      const int x = ic - lo;
+      uint32_t d = g[u] + h[u];
+      d += umul((uint32_t)(N - ic), e[x]) + umul((uint32_t)(N - j - 1), e[x + 1]);
+      d += (uint32_t)m * (a[u] + b[u]);
+      const int ctx = part == 0 ? by - ic : ic + by - 1;
+      const bool take = i <= hi && (part == 0 ? (int32_t)d <= bd : (int32_t)d < bd);
+      bd = take ? (int32_t)d : bd;
+      bc = take ? ctx : bc;
     }
+  };
+  // two register sets, alternating: batch k + 1 loads while batch k runs
+  const int nbat = nb0 + nb1;
+  uint32_t gj[CU], hk[CU], tk[CU], tl[CU];
+  if (nbat > 0) load_batch(0, gi, hj, ti, tj);
+  for (int k = 0; k < nbat; k += 2) {
+    if (k + 1 < nbat) load_batch(k + 1, gj, hk, tk, tl);
+    eval_batch(k, gi, hj, ti, tj);
+    if (k + 1 >= nbat) break;
+    if (k + 2 < nbat) load_batch(k + 2, gi, hj, ti, tj);
+    eval_batch(k + 1, gj, hk, tk, tl);
   }
   const int64_t key = wave_argmin(bd, bc);
   if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, key);
