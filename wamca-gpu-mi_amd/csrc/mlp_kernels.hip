@@ -327,19 +327,35 @@ __device__ __forceinline__ int list_at(const SlotListInline& inl, const int32_t*
 // ---------------------------------------------------------------------------
 constexpr int kPrepPer = 8;
 
+// 64-bit value of lane (lane - shift within a row of 16, or a row
+// broadcast) through DPP on both halves; lanes without a source get 0.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ int64_t dpp64(int64_t v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(uint64_t)v, CTRL, RMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((uint64_t)v >> 32), CTRL, RMASK,
+                                             0xf, false);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Inclusive wave scan (Hillis-Steele over rows of 16 lanes: shifts 1, 2, 4,
+// 8; then row 0's total into row 1 and row 2's into row 3, then rows 0-1's
+// total into rows 2 and 3), in registers.
 __device__ __forceinline__ int64_t wave_scan_incl(int64_t v, int lane) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const int64_t o = __shfl_up((long long)v, off, kWave);
-    if (lane >= off) v += o;
-  }
+  (void)lane;
+  v += dpp64<0x111, 0xf>(v);
+  v += dpp64<0x112, 0xf>(v);
+  v += dpp64<0x114, 0xf>(v);
+  v += dpp64<0x118, 0xf>(v);
+  v += dpp64<0x142, 0xa>(v);
+  v += dpp64<0x143, 0xc>(v);
   return v;
 }
 
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor((long long)v, off, kWave);
-  return v;
+  v = wave_scan_incl(v, 0);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 struct PrepLds {
@@ -680,15 +696,19 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
 // source map: a move with a short region is applied by one thread, the few
 // long ones (regions are disjoint, so at most N / kShortMove) by all threads.
 // ij(q) gives the q-th kept move as (i << 32 | j); tnew ends as the new tour.
+// loaded: told and tnew already hold the tour (copied at the kernel's start,
+// its load overlapping the row-key loads).
 template <typename IJ>
 __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ ij, int32_t* told,
-                                           int32_t* tnew, int& s_long, int* s_longq) {
+                                           int32_t* tnew, int& s_long, int* s_longq,
+                                           bool loaded = false) {
   const int N = s.N, tid = threadIdx.x, k = nb_k(nb);
-  for (int p = tid; p < N; p += kBlock) {
-    const int c = s.tour[p];
-    told[p] = c;
-    tnew[p] = c;
-  }
+  if (!loaded)
+    for (int p = tid; p < N; p += kBlock) {
+      const int c = s.tour[p];
+      told[p] = c;
+      tnew[p] = c;
+    }
   if (tid == 0) s_long = 0;
   __syncthreads();
   for (int q = tid; q < K; q += kBlock) {
@@ -751,18 +771,25 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   __syncthreads();
   // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
   // wrote the buffer opposite info[PARITY0 + nb]; this merge commits it.
+  // The tour (for the apply step) is copied in the same loop, so its loads
+  // go out with the row keys'.
   const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
   const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
   if (skip_inactive) {
     __syncthreads();  // every thread has read the old parity
     if (tid == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
   }
-  for (int r = tid; r < R; r += kBlock) {
-    const int64_t key = rk[r];
+  for (int q = tid; q < max(R, N); q += kBlock) {
+    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
+    const int c = q < N ? s.tour[q] : 0;
+    if (q < N) {
+      told[q] = c;
+      tnew[q] = c;
+    }
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
       const int at = atomicAdd(&s_m, 1);
-      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + r;
-      ctxL[r] = (uint16_t)(key & MLP_KEY_MASK);
+      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
+      ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
     }
   }
   __syncthreads();
@@ -788,6 +815,17 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
       if (tid < M) keys[rank] = my;
       __builtin_amdgcn_wave_barrier();
     }
+  } else if (!GMEM && M <= kBlock) {
+    // one key per thread: the same rank sort over the workgroup (M reads of
+    // LDS broadcasts per thread against the 36 barrier phases of a bitonic
+    // sort of 256)
+    const int64_t my = tid < M ? keys[tid] : 0;
+    int rank = 0;
+    if (tid < M)
+      for (int u = 0; u < M; ++u) rank += keys[u] < my ? 1 : 0;
+    __syncthreads();
+    if (tid < M) keys[rank] = my;
+    __syncthreads();
   } else {
     int P = 1;
     while (P < M) P <<= 1;
@@ -841,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   __syncthreads();
   const int K = s_kept;
   // 4. apply the kept moves, rebuild the prefix arrays
-  apply_kept(s, nb, K, [&](int q) { return keys[q]; }, told, tnew, s_long, s_longq);
+  apply_kept(s, nb, K, [&](int q) { return keys[q]; }, told, tnew, s_long, s_longq, true);
   prep_wg(s, tnew, PL);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
