@@ -19,6 +19,8 @@
 // square root (mlproblem.cpp:159-162).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mlp_gpu.h"
 #include "mlp_internal.h"
 
@@ -1306,7 +1308,7 @@ namespace {
 template <int R, int C>
 struct TileLds {
   uint16_t v[R + 4][C + 4];  // v[a - b0][c - cbase] = d(tour[a], tour[c])
-  int32_t w[C + 4];          // w[c] staged at c - cbase
+  uint16_t w[C + 4];         // w[c] staged at c - cbase (a distance: below 2^16)
   int32_t t[C + 4];          // T[c] (mod 2^32) staged at c - cbase
   int32_t rw[R + 8];         // row side: w, T and xy at positions b0 .. b0 + R + 7
   int32_t rt[R + 8];
@@ -1367,6 +1369,32 @@ __device__ __forceinline__ void reset_other(const SlotDev& s, int nb, int par, i
   }
 }
 
+// Value barrier: the compiler sees an opaque value, so a per-column term
+// stays one shared value instead of being distributed into every row's
+// products (which turns two 24-bit multiplies into one quarter-rate 32-bit
+// multiply per row).
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
+// Per-lane running minimum over increasing indices: strict < keeps the
+// first (lowest) index on ties.  Selects, not branches.
+__device__ __forceinline__ void take_min(int32_t& bd, int32_t& bc, uint32_t d, int c) {
+  const bool lt = (int32_t)d < bd;
+  bc = lt ? c : bc;
+  bd = lt ? (int32_t)d : bd;
+}
+
+__device__ __forceinline__ void take_min_if(int32_t& bd, int32_t& bc, uint32_t d, int c, bool ok) {
+  const bool lt = ok && (int32_t)d < bd;
+  bc = lt ? c : bc;
+  bd = lt ? (int32_t)d : bd;
+}
+
+template <bool B>
+using Flag = std::integral_constant<bool, B>;
+
 // Swap rows of a tile: each wave owns RW consecutive rows (r0 = wave * RW),
 // lanes stride over the tile's columns j and evaluate all RW rows for their
 // column.  Row r: i = b0+r+1, j in [i+2, N-2].  delta = Cs_r + Bs(j) +
@@ -1396,30 +1424,33 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - 1);
   const int jlo = max(q0, b0 + r0 + 3);
-  for (int jb = jlo - ((jlo - q0) & (kWave - 1)); jb < jhi; jb += kWave) {
-    const int j = jb + lane;
-    const int jj = min(max(j, jlo), jhi - 1);
-    const int x = jj - cbase;
-    const uint32_t nj = (uint32_t)(N - jj);
-    const uint32_t cs = 0u - umul(nj, (uint32_t)L.w[x]) - umul(nj - 1u, (uint32_t)L.w[x + 1]);
-    uint32_t c[RW + 2];
+  // FR: every row of the wave is in range (no per-row guard in the loop)
+  auto cols = [&](auto fr) {
+    constexpr bool FR = decltype(fr)::value;
+    for (int jb = jlo - ((jlo - q0) & (kWave - 1)); jb < jhi; jb += kWave) {
+      const int j = jb + lane;
+      const int jj = min(max(j, jlo), jhi - 1);
+      const int x = jj - cbase;
+      const uint32_t nj = (uint32_t)(N - jj);
+      const uint32_t cs = opaque(0u - umul(nj, L.w[x]) - umul(nj - 1u, L.w[x + 1]));
+      uint32_t c[RW + 2];
 #pragma unroll
-    for (int a = 0; a < RW + 2; ++a) c[a] = V[a][x];
-    const bool full = jb >= b0 + r0 + RW + 2 && jb + kWave <= jhi;
+      for (int a = 0; a < RW + 2; ++a) c[a] = V[a][x];
+      const bool full = jb >= b0 + r0 + RW + 2 && jb + kWave <= jhi;
 #pragma unroll
-    for (int r = 0; r < RW; ++r) {
-      if (r < rv) {
-        const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
-        const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) +
-                           umul(nj, V[r + 1][x - 1]) + umul(nj - 1u, V[r + 1][x + 1]);
-        const bool ok = full || (j < jhi && j >= b0 + r0 + r + 3);
-        if (ok && (int32_t)d < bd[r]) {
-          bd[r] = (int32_t)d;
-          bc[r] = j;
+      for (int r = 0; r < RW; ++r) {
+        if (FR || r < rv) {
+          const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
+          const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) +
+                             umul(nj, V[r + 1][x - 1]) + umul(nj - 1u, V[r + 1][x + 1]);
+          if (full) take_min(bd[r], bc[r], d, j);
+          else take_min_if(bd[r], bc[r], d, j, j < jhi && j >= b0 + r0 + r + 3);
         }
       }
     }
-  }
+  };
+  if (rv == RW) cols(Flag<true>{});
+  else cols(Flag<false>{});
   int64_t* out = row_buf(s, 0, parity & 1) + b0 + r0;
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -1475,63 +1506,57 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
   }
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - K);
-  for (int jb = q0; jb < jhi; jb += kWave) {
-    const int j = jb + lane;
-    const int jj = min(j, jhi - 1);
-    const int x = jj - cbase;
-    const uint32_t nj = (uint32_t)(N - jj);
-    const bool tail = jb + kWave > jhi;
-    const bool allF = !tail && jb >= bb + RW + 1;
-    const bool allB = !tail && jb + kWave - 1 <= bb + 1;
-    if (allF) {
-      const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
+  auto cols = [&](auto fr) {
+    constexpr bool FR = decltype(fr)::value;
+    for (int jb = q0; jb < jhi; jb += kWave) {
+      const int j = jb + lane;
+      const int jj = min(j, jhi - 1);
+      const int x = jj - cbase;
+      const uint32_t nj = (uint32_t)(N - jj);
+      const bool tail = jb + kWave > jhi;
+      const bool allF = !tail && jb >= bb + RW + 1;
+      const bool allB = !tail && jb + kWave - 1 <= bb + 1;
+      if (allF) {
+        const uint32_t CF = opaque(0u - umul(nj - K, L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1]);
 #pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        if (r < rv) {
-          uint32_t d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
-          if (K > 1) d += smul(jj, nSF[r]);
-          if ((int32_t)d < bd[r]) {
-            bd[r] = (int32_t)d;
-            bc[r] = j - 1;
-          }
-        }
-      }
-    } else if (allB) {
-      const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
-#pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        if (r < rv) {
-          uint32_t d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
-          if (K > 1) d += smul(jj, nSB[r]);
-          if ((int32_t)d < bd[r]) {
-            bd[r] = (int32_t)d;
-            bc[r] = j - 1;
-          }
-        }
-      }
-    } else {
-      const uint32_t CF = 0u - umul(nj - K, (uint32_t)L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1];
-      const uint32_t CB = 0u - umul(nj, (uint32_t)L.w[x]) + (uint32_t)K * (uint32_t)L.t[x];
-#pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        if (r < rv) {
-          const bool fwd = jj >= bb + r + 2;
-          uint32_t d;
-          if (fwd) {
-            d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
+        for (int r = 0; r < RW; ++r) {
+          if (FR || r < rv) {
+            uint32_t d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
             if (K > 1) d += smul(jj, nSF[r]);
-          } else {
-            d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
-            if (K > 1) d += smul(jj, nSB[r]);
+            take_min(bd[r], bc[r], d, j - 1);
           }
-          if (j < jhi && (int32_t)d < bd[r]) {
-            bd[r] = (int32_t)d;
-            bc[r] = j - 1;
+        }
+      } else if (allB) {
+        const uint32_t CB = opaque(0u - umul(nj, L.w[x]) + (uint32_t)K * (uint32_t)L.t[x]);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          if (FR || r < rv) {
+            uint32_t d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
+            if (K > 1) d += smul(jj, nSB[r]);
+            take_min(bd[r], bc[r], d, j - 1);
+          }
+        }
+      } else {
+        const uint32_t CF = opaque(0u - umul(nj - K, L.w[x + K]) + (uint32_t)K * (uint32_t)L.t[x + K - 1]);
+        const uint32_t CB = opaque(0u - umul(nj, L.w[x]) + (uint32_t)K * (uint32_t)L.t[x]);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          if (FR || r < rv) {
+            const bool fwd = jj >= bb + r + 2;
+            uint32_t dF = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
+            uint32_t dB = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
+            if (K > 1) {
+              dF += smul(jj, nSF[r]);
+              dB += smul(jj, nSB[r]);
+            }
+            take_min_if(bd[r], bc[r], fwd ? dF : dB, j - 1, j < jhi);
           }
         }
       }
     }
-  }
+  };
+  if (rv == RW) cols(Flag<true>{});
+  else cols(Flag<false>{});
   int64_t* out = row_buf(s, K + 1, (parity >> (K + 1)) & 1) + bb;
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -1611,7 +1636,7 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     if (c < cend) {
 #pragma unroll
       for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc[it], dr);
-      L.w[c - cbase] = wv[it];
+      L.w[c - cbase] = (uint16_t)wv[it];
       L.t[c - cbase] = tv[it];
     }
   }

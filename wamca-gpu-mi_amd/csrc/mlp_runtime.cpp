@@ -155,6 +155,9 @@ struct mlp_ctx {
   int64_t* h_i64 = nullptr;  // pinned
   size_t i64_cap = 0;
   std::unordered_map<std::string, DevList> lists;
+  // pinned staging of list uploads: a bump area reused after the stream drains
+  char* h_lists = nullptr;
+  size_t h_lists_cap = 0, h_lists_off = 0;
   size_t max_merge_lds = 0;
   size_t max_total_lds = 0;  // dynamic LDS limit of the TOTAL-mode kernels
   int tile_override = -1;  // -1 auto, else an MLP_TILE_* shape
@@ -225,9 +228,12 @@ bool slot_ok(mlp_ctx* ctx, int s) {
   return s >= 0 && s < (int)ctx->slots.size() && ctx->slots[s].alive;
 }
 
-// Device copy of an int32 list, uploaded once per distinct content.  Inside a
-// graph capture a missing list is uploaded on a side stream (the captured
-// stream is not touched), so it exists before any replay.
+// Device copy of an int32 list, uploaded once per distinct content.  The
+// upload is ordered on the context stream from pinned staging, so a new list
+// does not wait for the kernels already queued (a pageable hipMemcpy would
+// drain the stream first).  Inside a graph capture a missing list is uploaded
+// on a side stream (the captured stream is not touched), so it exists before
+// any replay.
 int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) {
   std::string key((const char*)v, (size_t)count * 4);
   auto it = ctx->lists.find(key);
@@ -250,12 +256,30 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     }
   }
   void* p = nullptr;
-  HCK(hipMalloc(&p, std::max<size_t>(4, (size_t)count * 4)));
+  const size_t bytes = (size_t)count * 4;
+  HCK(hipMalloc(&p, std::max<size_t>(4, bytes)));
   if (ctx->capturing) {
-    HCK(hipMemcpyAsync(p, v, (size_t)count * 4, hipMemcpyHostToDevice, ctx->side));
+    HCK(hipMemcpyAsync(p, v, bytes, hipMemcpyHostToDevice, ctx->side));
     HCK(hipStreamSynchronize(ctx->side));
-  } else {
-    HCK(hipMemcpy(p, v, (size_t)count * 4, hipMemcpyHostToDevice));
+  } else if (bytes > 0) {
+    if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
+      // every copy out of the staging area is done once the stream drains
+      HCK(hipStreamSynchronize(ctx->stream));
+      ctx->h_lists_off = 0;
+      if (bytes > ctx->h_lists_cap) {
+        if (ctx->h_lists) hipHostFree(ctx->h_lists);
+        ctx->h_lists = nullptr;
+        ctx->h_lists_cap = 0;
+        size_t cap = (size_t)1 << 20;
+        while (cap < bytes) cap *= 2;
+        HCK(hipHostMalloc((void**)&ctx->h_lists, cap, hipHostMallocDefault));
+        ctx->h_lists_cap = cap;
+      }
+    }
+    char* h = ctx->h_lists + ctx->h_lists_off;
+    memcpy(h, v, bytes);
+    ctx->h_lists_off += (bytes + 255) & ~(size_t)255;
+    HCK(hipMemcpyAsync(p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
   }
   ctx->lists.emplace(std::move(key), DevList{p});
   *out = (const int32_t*)p;
@@ -424,6 +448,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
   if (ctx->h_stage) hipHostFree(ctx->h_stage);
+  if (ctx->h_lists) hipHostFree(ctx->h_lists);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->tev) hipEventDestroy(e);
