@@ -1845,11 +1845,13 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
 
 }  // namespace
 
-// Resident workgroups per CU the register budget is held to: the small
-// shapes serve one-tour launches, whose latency depends on every tile being
-// resident at once.
+// Resident waves per SIMD the register budget is held to (at most 72
+// VGPRs for seven): the small shapes serve one-tour launches, whose latency
+// depends on every tile being resident at once, and many-tour launches,
+// which gain from the occupancy.  The large shapes are held to three
+// workgroups per CU by their LDS tiles anyway.
 template <int R, int C>
-__global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 5 : 1)) void k_sweep(
+__global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter,
