@@ -366,6 +366,7 @@ def main():
         best = None
         for persistent in (False, True):
             eng.eval(slots, 0x1F, 0, persistent=persistent)
+            eng.sync()  # the wall clock starts with the stream idle
             eng.region_begin()
             t1 = time.perf_counter()
             reps = 3
