@@ -259,13 +259,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--tours", type=int, default=1000)
-    ap.add_argument("--tile", type=int, default=-1, help="-1: autotune over the three shapes")
+    ap.add_argument("--tile", type=int, default=-1, help="-1: autotune over the five shapes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dvnd", action="store_true")
     ap.add_argument("--no-total", action="store_true")
     ap.add_argument("--stream-jobs", type=int, default=300)
-    ap.add_argument("--stream-rate", type=float, default=2000.0)
+    # served vs plain compares capacity: the stream arrives faster than either
+    # path solves (plain about 250 solves/s, served about 4000); the rate
+    # sweep covers the slower, arrival-limited rates
+    ap.add_argument("--stream-rate", type=float, default=5000.0)
     ap.add_argument("--stream-big-frac", type=float, default=0.01,
                     help="fraction of stream jobs replaced by clustered n = 18000")
     ap.add_argument("--copy-bytes", type=int, default=0,
