@@ -88,9 +88,10 @@ int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* n
 /* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5). */
 int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                         int32_t round, int32_t* inst);
-/* As mlp_instance_create, but the coordinate copies are only enqueued on the
- * context stream: xs and ys must stay unchanged until the stream has passed
- * them (any synchronising call, e.g. mlp_sync). */
+/* As mlp_instance_create, but the coordinates are staged in pinned host
+ * memory and their copy is only enqueued on the context stream (the call does
+ * not wait for queued kernels); xs and ys may be reused as soon as it
+ * returns. */
 int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                               int32_t round, int32_t* inst);
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst);

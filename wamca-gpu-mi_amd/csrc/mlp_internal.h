@@ -113,6 +113,14 @@ struct SweepSegsInline {
 };
 
 #define MLP_SLOTS_INLINE 128
+// 64-bit word ranges of a slot block holding row buffers (set to KEY_NONE
+// when the slot is created)
+struct SlotInitRows {
+  int64_t start[MLP_NB_MAX];
+  int64_t count[MLP_NB_MAX];
+  int32_t nbs;
+};
+
 struct SlotListInline {
   int32_t id[MLP_SLOTS_INLINE];
 };

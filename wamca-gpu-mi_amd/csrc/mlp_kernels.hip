@@ -1273,6 +1273,37 @@ __global__ void k_set_slot(SlotDev* table, int id, SlotDev d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) table[id] = d;
 }
 
+// A new slot's block in one launch: every 64-bit word zero except the row
+// buffers (KEY_NONE), and the slot's table entry.
+__global__ void k_slot_init(SlotDev* table, int id, SlotDev d, int64_t* mem, int64_t words,
+                            SlotInitRows rows) {
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < words;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = 0;
+    for (int nb = 0; nb < rows.nbs; ++nb)
+      if ((uint64_t)(w - rows.start[nb]) < (uint64_t)rows.count[nb]) v = MLP_KEY_NONE;
+    mem[w] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) table[id] = d;
+}
+
+// Tours of several slots into one array, in list order: block q copies slot
+// ids[q], whose offset is the sum of the earlier slots' N.
+__global__ void k_gather_tours(const SlotDev* __restrict__ table, const int32_t* ids, int count,
+                               int32_t* out) {
+  __shared__ int64_t s_off;
+  const int q = blockIdx.x;
+  if (q >= count) return;
+  if (threadIdx.x == 0) s_off = 0;
+  __syncthreads();
+  int64_t part = 0;
+  for (int r = threadIdx.x; r < q; r += blockDim.x) part += table[ids[r]].N;
+  if (part) atomicAdd((unsigned long long*)&s_off, (unsigned long long)part);
+  __syncthreads();
+  const SlotDev& s = table[ids[q]];
+  for (int p = threadIdx.x; p < s.N; p += blockDim.x) out[s_off + p] = s.tour[p];
+}
+
 __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* ids, int count,
                               int word, int64_t* out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2081,6 +2112,21 @@ hipError_t mlp_launch_fill_i64(hipStream_t st, int64_t* p, int64_t count, int64_
 
 hipError_t mlp_launch_set_slot(hipStream_t st, SlotDev* table, int id, const SlotDev& d) {
   hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(64), 0, st, table, id, d);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_slot_init(hipStream_t st, SlotDev* table, int id, const SlotDev& d,
+                                int64_t* mem, int64_t words, const SlotInitRows& rows) {
+  const int64_t blocks = std::min<int64_t>((words + kBlock - 1) / kBlock, 1024);
+  hipLaunchKernelGGL(k_slot_init, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(kBlock), 0, st,
+                     table, id, d, mem, words, rows);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_gather_tours(hipStream_t st, const SlotDev* table, const int32_t* ids,
+                                   int count, int32_t* out) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_tours, dim3(count), dim3(kBlock), 0, st, table, ids, count, out);
   return hipGetLastError();
 }
 

@@ -59,6 +59,10 @@ hipError_t mlp_launch_clear_rows(hipStream_t st, const SlotDev* table, const Swe
                                  const SweepSeg* segs_dev, int nsegs);
 hipError_t mlp_launch_fill_i64(hipStream_t st, int64_t* p, int64_t count, int64_t v);
 hipError_t mlp_launch_set_slot(hipStream_t st, SlotDev* table, int id, const SlotDev& d);
+hipError_t mlp_launch_slot_init(hipStream_t st, SlotDev* table, int id, const SlotDev& d,
+                                int64_t* mem, int64_t words, const SlotInitRows& rows);
+hipError_t mlp_launch_gather_tours(hipStream_t st, const SlotDev* table, const int32_t* ids,
+                                   int count, int32_t* out);
 hipError_t mlp_launch_gather_info(hipStream_t st, const SlotDev* table, const int32_t* ids,
                                   int count, int word, int64_t* out);
 

@@ -228,6 +228,28 @@ bool slot_ok(mlp_ctx* ctx, int s) {
   return s >= 0 && s < (int)ctx->slots.size() && ctx->slots[s].alive;
 }
 
+// Pinned staging space for an upload ordered on the context stream: a bump
+// area, reused (after the stream drains) when it is full.
+int stage_alloc(mlp_ctx* ctx, size_t bytes, char** h) {
+  if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
+    // every copy out of the staging area is done once the stream drains
+    HCK(hipStreamSynchronize(ctx->stream));
+    ctx->h_lists_off = 0;
+    if (bytes > ctx->h_lists_cap) {
+      if (ctx->h_lists) hipHostFree(ctx->h_lists);
+      ctx->h_lists = nullptr;
+      ctx->h_lists_cap = 0;
+      size_t cap = (size_t)1 << 20;
+      while (cap < bytes) cap *= 2;
+      HCK(hipHostMalloc((void**)&ctx->h_lists, cap, hipHostMallocDefault));
+      ctx->h_lists_cap = cap;
+    }
+  }
+  *h = ctx->h_lists + ctx->h_lists_off;
+  ctx->h_lists_off += (bytes + 255) & ~(size_t)255;
+  return MLP_OK;
+}
+
 // Device copy of an int32 list, uploaded once per distinct content.  The
 // upload is ordered on the context stream from pinned staging, so a new list
 // does not wait for the kernels already queued (a pageable hipMemcpy would
@@ -262,23 +284,10 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     HCK(hipMemcpyAsync(p, v, bytes, hipMemcpyHostToDevice, ctx->side));
     HCK(hipStreamSynchronize(ctx->side));
   } else if (bytes > 0) {
-    if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
-      // every copy out of the staging area is done once the stream drains
-      HCK(hipStreamSynchronize(ctx->stream));
-      ctx->h_lists_off = 0;
-      if (bytes > ctx->h_lists_cap) {
-        if (ctx->h_lists) hipHostFree(ctx->h_lists);
-        ctx->h_lists = nullptr;
-        ctx->h_lists_cap = 0;
-        size_t cap = (size_t)1 << 20;
-        while (cap < bytes) cap *= 2;
-        HCK(hipHostMalloc((void**)&ctx->h_lists, cap, hipHostMallocDefault));
-        ctx->h_lists_cap = cap;
-      }
-    }
-    char* h = ctx->h_lists + ctx->h_lists_off;
+    char* h = nullptr;
+    const int st = stage_alloc(ctx, bytes, &h);
+    if (st) return st;
     memcpy(h, v, bytes);
-    ctx->h_lists_off += (bytes + 255) & ~(size_t)255;
     HCK(hipMemcpyAsync(p, h, bytes, hipMemcpyHostToDevice, ctx->stream));
   }
   ctx->lists.emplace(std::move(key), DevList{p});
@@ -299,13 +308,13 @@ int slot_list(mlp_ctx* ctx, const int32_t* slots, int count, SlotListInline& inl
 
 int ensure_i64(mlp_ctx* ctx, size_t count) {
   if (count <= ctx->i64_cap) return MLP_OK;
+  const size_t cap = std::max<size_t>({count, 2 * ctx->i64_cap, 1024});
   HCK(hipStreamSynchronize(ctx->stream));
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
   ctx->d_i64 = nullptr;
   ctx->h_i64 = nullptr;
   ctx->i64_cap = 0;
-  const size_t cap = std::max<size_t>(count, 1024);
   HCK(hipMalloc(&ctx->d_i64, cap * 8));
   HCK(hipHostMalloc(&ctx->h_i64, cap * 8, hipHostMallocDefault));
   ctx->i64_cap = cap;
@@ -535,8 +544,17 @@ static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, i
   in.xs = (int32_t*)mem;
   in.ys = (int32_t*)((char*)mem + half);
   if (async) {
-    HCK(hipMemcpyAsync(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
-    HCK(hipMemcpyAsync(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    // both coordinate arrays in one copy from pinned staging, ordered on the
+    // stream (the caller's arrays may be released at once)
+    char* h = nullptr;
+    st = stage_alloc(ctx, 2 * half, &h);
+    if (st) {
+      pool_release(ctx, mem, in.cls);
+      return st;
+    }
+    memcpy(h, xs, 4 * (size_t)n);
+    memcpy(h + half, ys, 4 * (size_t)n);
+    HCK(hipMemcpyAsync(mem, h, 2 * half, hipMemcpyHostToDevice, ctx->stream));
   } else {
     HCK(hipMemcpy(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice));
     HCK(hipMemcpy(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice));
@@ -627,7 +645,6 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
   s.nbs = ctx->nbs;
   st = pool_alloc(ctx, L.total, &s.mem, &s.cls);
   if (st) return st;
-  HCK(hipMemsetAsync(s.mem, 0, L.total, ctx->stream));
   char* b = (char*)s.mem;
   SlotDev& d = s.dev;
   d.N = N;
@@ -654,9 +671,15 @@ int mlp_slot_create(mlp_ctx* ctx, int32_t inst, int32_t* slot) {
     d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
   }
   d.mscratch = b + L.ms;
-  for (int nb = 0; nb < d.nbs; ++nb)
-    HCK(mlp_launch_fill_i64(ctx->stream, d.rows[nb], 2 * (int64_t)d.rowcap, MLP_KEY_NONE));
-  HCK(mlp_launch_set_slot(ctx->stream, ctx->d_table, id, d));
+  // one launch: the block zeroed, the row buffers KEY_NONE, the table entry
+  SlotInitRows rows{};
+  rows.nbs = d.nbs;
+  for (int nb = 0; nb < d.nbs; ++nb) {
+    rows.start[nb] = (int64_t)(L.rows[nb] / 8);
+    rows.count[nb] = 2 * (int64_t)d.rowcap;
+  }
+  HCK(mlp_launch_slot_init(ctx->stream, ctx->d_table, id, d, (int64_t*)s.mem,
+                           (int64_t)(L.total / 8), rows));
   s.alive = true;
   ctx->slots[id] = s;
   *slot = id;
@@ -733,13 +756,25 @@ int mlp_get_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* to
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count && !tours) return fail(ctx, MLP_EINVAL, "tours is NULL");
-  HCK(hipStreamSynchronize(ctx->stream));
-  size_t off = 0;
-  for (int q = 0; q < count; ++q) {
-    const Slot& s = ctx->slots[slots[q]];
-    HCK(hipMemcpy(tours + off, s.dev.tour, 4 * (size_t)s.N, hipMemcpyDeviceToHost));
-    off += (size_t)s.N;
+  if (count == 0) return MLP_OK;
+  size_t total = 0;
+  for (int q = 0; q < count; ++q) total += (size_t)ctx->slots[slots[q]].N;
+  // one copy into pinned memory: a single slot's tour directly, several
+  // gathered into one device array first
+  st = ensure_i64(ctx, (total + 1) / 2);
+  if (st) return st;
+  if (count == 1) {
+    HCK(hipMemcpyAsync(ctx->h_i64, ctx->slots[slots[0]].dev.tour, 4 * total, hipMemcpyDeviceToHost,
+                       ctx->stream));
+  } else {
+    const int32_t* dev = nullptr;
+    st = device_list(ctx, slots, count, &dev);
+    if (st) return st;
+    HCK(mlp_launch_gather_tours(ctx->stream, ctx->d_table, dev, count, (int32_t*)ctx->d_i64));
+    HCK(hipMemcpyAsync(ctx->h_i64, ctx->d_i64, 4 * total, hipMemcpyDeviceToHost, ctx->stream));
   }
+  HCK(hipStreamSynchronize(ctx->stream));
+  memcpy(tours, ctx->h_i64, 4 * total);
   return MLP_OK;
 }
 

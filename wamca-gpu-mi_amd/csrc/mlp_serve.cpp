@@ -148,7 +148,9 @@ static int retire(mlp_server* s, Lane& l) {
   int st = mlp_dvnd_status(l.ctx, slots.data(), c, act.data(), rnd.data());
   if (st) return st;
   const double now = mlp_server_now(s);
-  std::vector<Job*> still;
+  std::vector<Job*> still, fin;
+  std::vector<int32_t> fslots;
+  size_t ftotal = 0;
   for (int q = 0; q < c; ++q) {
     Job* j = l.active[(size_t)q];
     if (act[(size_t)q]) {
@@ -156,13 +158,29 @@ static int retire(mlp_server* s, Lane& l) {
       continue;
     }
     j->rounds = rnd[(size_t)q];
-    st = mlp_get_tours(l.ctx, &j->slot, 1, j->tour.data());
-    if (st == MLP_OK) st = mlp_get_latency(l.ctx, &j->slot, 1, &j->latency);
+    fin.push_back(j);
+    fslots.push_back(j->slot);
+    ftotal += (size_t)j->n + 1;
+  }
+  if (!fin.empty()) {
+    // the finished jobs' tours and latencies in one read-back each
+    std::vector<int32_t> tours(ftotal);
+    std::vector<int64_t> lat(fin.size());
+    const int nf = (int)fin.size();
+    st = mlp_get_tours(l.ctx, fslots.data(), nf, tours.data());
+    if (st == MLP_OK) st = mlp_get_latency(l.ctx, fslots.data(), nf, lat.data());
     if (st) return st;
-    j->t_finish = now;
-    release(s, j);
-    s->used -= j->bytes;
-    s->done.push_back(j);
+    size_t off = 0;
+    for (size_t q = 0; q < fin.size(); ++q) {
+      Job* j = fin[q];
+      std::copy(tours.begin() + (long)off, tours.begin() + (long)(off + j->n + 1), j->tour.begin());
+      off += (size_t)j->n + 1;
+      j->latency = lat[q];
+      j->t_finish = now;
+      release(s, j);
+      s->used -= j->bytes;
+      s->done.push_back(j);
+    }
   }
   l.active.swap(still);
   l.chunk_pending = false;
