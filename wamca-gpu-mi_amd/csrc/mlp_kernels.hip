@@ -1287,21 +1287,37 @@ __global__ void k_slot_init(SlotDev* table, int id, SlotDev d, int64_t* mem, int
   if (blockIdx.x == 0 && threadIdx.x == 0) table[id] = d;
 }
 
-// Tours of several slots into one array, in list order: block q copies slot
-// ids[q], whose offset is the sum of the earlier slots' N.
-__global__ void k_gather_tours(const SlotDev* __restrict__ table, const int32_t* ids, int count,
-                               int32_t* out) {
+// Offset of slot ids[q]'s tour in a list-ordered array of tours: the sum of
+// the earlier slots' N (one block).
+__device__ int64_t tour_offset(const SlotDev* __restrict__ table, const int32_t* ids, int q) {
   __shared__ int64_t s_off;
-  const int q = blockIdx.x;
-  if (q >= count) return;
   if (threadIdx.x == 0) s_off = 0;
   __syncthreads();
   int64_t part = 0;
   for (int r = threadIdx.x; r < q; r += blockDim.x) part += table[ids[r]].N;
   if (part) atomicAdd((unsigned long long*)&s_off, (unsigned long long)part);
   __syncthreads();
+  return s_off;
+}
+
+// Tours of several slots into one array, in list order (block q: slot ids[q]).
+__global__ void k_gather_tours(const SlotDev* __restrict__ table, const int32_t* ids, int count,
+                               int32_t* out) {
+  const int q = blockIdx.x;
+  if (q >= count) return;
+  const int64_t off = tour_offset(table, ids, q);
   const SlotDev& s = table[ids[q]];
-  for (int p = threadIdx.x; p < s.N; p += blockDim.x) out[s_off + p] = s.tour[p];
+  for (int p = threadIdx.x; p < s.N; p += blockDim.x) out[off + p] = s.tour[p];
+}
+
+// The inverse: a list-ordered array of tours into the slots.
+__global__ void k_scatter_tours(const SlotDev* __restrict__ table, const int32_t* ids, int count,
+                                const int32_t* in) {
+  const int q = blockIdx.x;
+  if (q >= count) return;
+  const int64_t off = tour_offset(table, ids, q);
+  const SlotDev& s = table[ids[q]];
+  for (int p = threadIdx.x; p < s.N; p += blockDim.x) s.tour[p] = in[off + p];
 }
 
 __global__ void k_gather_info(const SlotDev* __restrict__ table, const int32_t* ids, int count,
@@ -2127,6 +2143,13 @@ hipError_t mlp_launch_gather_tours(hipStream_t st, const SlotDev* table, const i
                                    int count, int32_t* out) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather_tours, dim3(count), dim3(kBlock), 0, st, table, ids, count, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_scatter_tours(hipStream_t st, const SlotDev* table, const int32_t* ids,
+                                    int count, const int32_t* in) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_tours, dim3(count), dim3(kBlock), 0, st, table, ids, count, in);
   return hipGetLastError();
 }
 

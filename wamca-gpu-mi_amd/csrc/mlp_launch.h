@@ -63,6 +63,8 @@ hipError_t mlp_launch_slot_init(hipStream_t st, SlotDev* table, int id, const Sl
                                 int64_t* mem, int64_t words, const SlotInitRows& rows);
 hipError_t mlp_launch_gather_tours(hipStream_t st, const SlotDev* table, const int32_t* ids,
                                    int count, int32_t* out);
+hipError_t mlp_launch_scatter_tours(hipStream_t st, const SlotDev* table, const int32_t* ids,
+                                    int count, const int32_t* in);
 hipError_t mlp_launch_gather_info(hipStream_t st, const SlotDev* table, const int32_t* ids,
                                   int count, int word, int64_t* out);
 

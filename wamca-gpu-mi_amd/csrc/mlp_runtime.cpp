@@ -149,8 +149,8 @@ struct mlp_ctx {
   int* d_counter = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timed region
   bool region_open = false;
-  int32_t* h_stage = nullptr;  // pinned staging for tour uploads
-  size_t h_stage_cap = 0;
+  int32_t* d_tstage = nullptr;  // device array of several slots' tours (uploads)
+  size_t d_tstage_cap = 0;
   int64_t* d_i64 = nullptr;  // small device scratch for gathers
   int64_t* h_i64 = nullptr;  // pinned
   size_t i64_cap = 0;
@@ -456,7 +456,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   if (ctx->d_counter) hipFree(ctx->d_counter);
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
-  if (ctx->h_stage) hipHostFree(ctx->h_stage);
+  if (ctx->d_tstage) hipFree(ctx->d_tstage);
   if (ctx->h_lists) hipHostFree(ctx->h_lists);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
@@ -718,21 +718,34 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
     }
     total += (size_t)N;
   }
-  HCK(hipStreamSynchronize(ctx->stream));
-  if (total > ctx->h_stage_cap) {
-    if (ctx->h_stage) hipHostFree(ctx->h_stage);
-    ctx->h_stage = nullptr;
-    ctx->h_stage_cap = 0;
-    HCK(hipHostMalloc(&ctx->h_stage, total * 4, hipHostMallocDefault));
-    ctx->h_stage_cap = total;
+  // all tours in one copy from pinned staging, ordered on the stream (no
+  // wait for queued kernels); several slots' tours land in a device array
+  // that one kernel scatters into the slots
+  char* h = nullptr;
+  st = stage_alloc(ctx, total * 4, &h);
+  if (st) return st;
+  memcpy(h, tours, total * 4);
+  if (count == 1) {
+    HCK(hipMemcpyAsync(ctx->slots[slots[0]].dev.tour, h, total * 4, hipMemcpyHostToDevice,
+                       ctx->stream));
+  } else {
+    if (total > ctx->d_tstage_cap) {
+      const size_t cap = std::max<size_t>(total, 2 * ctx->d_tstage_cap + 4096);
+      HCK(hipStreamSynchronize(ctx->stream));
+      if (ctx->d_tstage) hipFree(ctx->d_tstage);
+      ctx->d_tstage = nullptr;
+      ctx->d_tstage_cap = 0;
+      HCK(hipMalloc((void**)&ctx->d_tstage, cap * 4));
+      ctx->d_tstage_cap = cap;
+    }
+    HCK(hipMemcpyAsync(ctx->d_tstage, h, total * 4, hipMemcpyHostToDevice, ctx->stream));
+    const int32_t* dev = nullptr;
+    st = device_list(ctx, slots, count, &dev);
+    if (st) return st;
+    HCK(mlp_launch_scatter_tours(ctx->stream, ctx->d_table, dev, count, ctx->d_tstage));
   }
-  memcpy(ctx->h_stage, tours, total * 4);
-  size_t off = 0;
   for (int q = 0; q < count; ++q) {
     Slot& s = ctx->slots[slots[q]];
-    HCK(hipMemcpyAsync(s.dev.tour, ctx->h_stage + off, 4 * (size_t)s.N, hipMemcpyHostToDevice,
-                       ctx->stream));
-    off += (size_t)s.N;
     for (int nb = 0; nb < MLP_NB_MAX; ++nb) {
       s.eval_rowmap[nb] = -1;
       s.merged[nb] = false;
@@ -778,20 +791,26 @@ int mlp_get_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* to
   return MLP_OK;
 }
 
-static int gather_info(mlp_ctx* ctx, const int32_t* slots, int count, int word, int64_t* out) {
+// Info words of several slots: word w0 into out0 and, when out1 is given,
+// word w1 into out1, with one read-back.
+static int gather_info(mlp_ctx* ctx, const int32_t* slots, int count, int w0, int64_t* out0,
+                       int w1 = 0, int64_t* out1 = nullptr) {
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count == 0) return MLP_OK;
-  st = ensure_i64(ctx, (size_t)count);
+  const size_t words = (size_t)count * (out1 ? 2 : 1);
+  st = ensure_i64(ctx, words);
   if (st) return st;
   const int32_t* dev = nullptr;
   st = device_list(ctx, slots, count, &dev);
   if (st) return st;
-  HCK(mlp_launch_gather_info(ctx->stream, ctx->d_table, dev, count, word, ctx->d_i64));
-  HCK(hipMemcpyAsync(ctx->h_i64, ctx->d_i64, 8 * (size_t)count, hipMemcpyDeviceToHost,
-                     ctx->stream));
+  HCK(mlp_launch_gather_info(ctx->stream, ctx->d_table, dev, count, w0, ctx->d_i64));
+  if (out1)
+    HCK(mlp_launch_gather_info(ctx->stream, ctx->d_table, dev, count, w1, ctx->d_i64 + count));
+  HCK(hipMemcpyAsync(ctx->h_i64, ctx->d_i64, 8 * words, hipMemcpyDeviceToHost, ctx->stream));
   HCK(hipStreamSynchronize(ctx->stream));
-  memcpy(out, ctx->h_i64, 8 * (size_t)count);
+  memcpy(out0, ctx->h_i64, 8 * (size_t)count);
+  if (out1) memcpy(out1, ctx->h_i64 + count, 8 * (size_t)count);
   return MLP_OK;
 }
 
@@ -1715,16 +1734,12 @@ int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* 
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count == 0) return MLP_OK;
-  std::vector<int64_t> v((size_t)count);
-  if (active) {
-    st = gather_info(ctx, slots, count, MLP_INFO_ACTIVE, v.data());
-    if (st) return st;
-    for (int q = 0; q < count; ++q) active[q] = (int32_t)v[(size_t)q];
-  }
-  if (rounds) {
-    st = gather_info(ctx, slots, count, MLP_INFO_ROUNDS, v.data());
-    if (st) return st;
-    for (int q = 0; q < count; ++q) rounds[q] = (int32_t)v[(size_t)q];
+  std::vector<int64_t> a((size_t)count), r((size_t)count);
+  st = gather_info(ctx, slots, count, MLP_INFO_ACTIVE, a.data(), MLP_INFO_ROUNDS, r.data());
+  if (st) return st;
+  for (int q = 0; q < count; ++q) {
+    if (active) active[q] = (int32_t)a[(size_t)q];
+    if (rounds) rounds[q] = (int32_t)r[(size_t)q];
   }
   return MLP_OK;
 }
