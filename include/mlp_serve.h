@@ -38,7 +38,7 @@ typedef struct {
   int64_t hbm_budget;      /* device bytes resident jobs may hold (<= 0: 64 GiB) */
   int32_t max_active;      /* resident jobs at most (<= 0: no limit but the budget) */
   int32_t rowmap;          /* MLP_ROWMAP_ADS or MLP_ROWMAP_ODC */
-  int32_t rounds_per_step; /* DVND rounds per step (<= 0: 2) */
+  int32_t rounds_per_step; /* DVND rounds per step (<= 0: 4) */
   int32_t max_rounds;      /* per job, 0 = to the local optimum */
   int32_t big_n;           /* jobs of n >= big_n run in a second lane (own context and
                               stream, same GPU); 0: 4096, < 0: one lane */

@@ -76,7 +76,7 @@ int mlp_server_create(mlp_ctx* ctx, const mlp_server_config* cfg, mlp_server** o
   if (!s) return MLP_ENOMEM;
   if (cfg) s->cfg = *cfg;
   if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
-  if (s->cfg.rounds_per_step <= 0) s->cfg.rounds_per_step = 2;
+  if (s->cfg.rounds_per_step <= 0) s->cfg.rounds_per_step = 4;
   if (s->cfg.big_n == 0) s->cfg.big_n = 4096;
   if (s->cfg.rowmap != MLP_ROWMAP_ADS && s->cfg.rowmap != MLP_ROWMAP_ODC) {
     delete s;

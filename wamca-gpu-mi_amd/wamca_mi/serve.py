@@ -20,7 +20,7 @@ from .engine import ROWMAP_ADS
 
 
 class Server:
-    def __init__(self, eng, hbm_budget=0, max_active=0, rowmap=ROWMAP_ADS, rounds_per_step=2,
+    def __init__(self, eng, hbm_budget=0, max_active=0, rowmap=ROWMAP_ADS, rounds_per_step=4,
                  max_rounds=0, big_n=0):
         """big_n: jobs of n >= big_n run in a second lane (0: 4096, < 0: one lane)."""
         self.eng = eng
@@ -85,7 +85,7 @@ def _summary(lat, span, count):
 
 
 def run_served(eng, jobs, arrivals, rnd=1, rowmap=ROWMAP_ADS, hbm_budget=0, max_active=0,
-               rounds_per_step=2, big_n=0):
+               rounds_per_step=4, big_n=0):
     """jobs: list of (xs, ys, tour); arrivals: seconds from the start (sorted)."""
     srv = Server(eng, hbm_budget, max_active, rowmap, rounds_per_step, big_n=big_n)
     try:
