@@ -366,7 +366,11 @@ struct PrepLds {
   int64_t lsum[kBlock / kWave];
 };
 
-__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
+// xyc: packed coordinates indexed by client (the merge's copy of the old
+// tour's xy, so the new tour needs no gather from the instance arrays), or
+// nullptr to read the instance arrays.
+__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
+                        const int32_t* xyc = nullptr) {
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
   int64_t lacc = 0;  // this thread's share of L = sum_p T[p]
@@ -381,7 +385,7 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P) {
 #pragma unroll
     for (int e = 0; e < kPrepPer + 2; ++e) {
       const int c = tour[min(max(p0 - 1 + e, 0), N - 1)];
-      xy[e] = (ldg(s.xs + c) << 16) | ldg(s.ys + c);
+      xy[e] = xyc ? xyc[c] : (ldg(s.xs + c) << 16) | ldg(s.ys + c);
     }
     int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
 #pragma unroll
@@ -764,7 +768,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   int64_t* keys = GMEM ? (int64_t*)s.mscratch : lds;
   int32_t* told = (int32_t*)(keys + pcap);
   int32_t* tnew = told + ncap;
-  uint32_t* bits = (uint32_t*)(tnew + ncap);
+  int32_t* xyc = tnew + ncap;  // packed coordinates by client
+  uint32_t* bits = (uint32_t*)(xyc + ncap);
   const int nwords = (N + 32) / 32 + 1;
   uint16_t* ctxL = (uint16_t*)(bits + (ncap + 32) / 32 + 1);  // ctx of each improving row
 
@@ -773,8 +778,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   __syncthreads();
   // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
   // wrote the buffer opposite info[PARITY0 + nb]; this merge commits it.
-  // The tour (for the apply step) is copied in the same loop, so its loads
-  // go out with the row keys'.
+  // The tour (for the apply step) and its coordinates (for the prefix
+  // rebuild, by client) are copied in the same loop, so their loads go out
+  // with the row keys'.
   const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
   const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
   if (skip_inactive) {
@@ -784,9 +790,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   for (int q = tid; q < max(R, N); q += kBlock) {
     const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
     const int c = q < N ? s.tour[q] : 0;
+    const int xq = q < N ? s.xy[q] : 0;
     if (q < N) {
       told[q] = c;
       tnew[q] = c;
+      xyc[c] = xq;
     }
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
       const int at = atomicAdd(&s_m, 1);
@@ -882,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   const int K = s_kept;
   // 4. apply the kept moves, rebuild the prefix arrays
   apply_kept(s, nb, K, [&](int q) { return keys[q]; }, told, tnew, s_long, s_longq, true);
-  prep_wg(s, tnew, PL);
+  prep_wg(s, tnew, PL, xyc);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
     s.info[MLP_INFO_ROUND_MOVES] += K;
@@ -2003,7 +2011,7 @@ void mlp_tile_shape(int tile, int* rows, int* cols) {
 // keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {
   const int nwords = (ncap + 32) / 32 + 1;
-  return (size_t)pcap * 8 + (size_t)ncap * 8 + (size_t)nwords * 4 + (size_t)pcap * 2;
+  return (size_t)pcap * 8 + (size_t)ncap * 12 + (size_t)nwords * 4 + (size_t)pcap * 2;
 }
 
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
