@@ -742,6 +742,49 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
 // One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
 // tail != 0: this is the last merge of a DVND round, and it also ends the
 // round for each active slot (round_tail, max_rounds = tail - 1).
+// Bitonic sort of keys[0..M) over the workgroup (keys[M..P) padded with
+// KEY_NONE, P the next power of two).  Each phase has P/2 compare-swaps; a
+// thread takes up to four pairs at a time, loading all of them before
+// comparing.
+__device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
+  const int tid = threadIdx.x;
+  int P = 1;
+  while (P < M) P <<= 1;
+  for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
+  __syncthreads();
+  const int half = P >> 1;
+  for (int kk = 2; kk <= P; kk <<= 1) {
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      const int lj = 31 - __clz(jj);
+      for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
+        int lo[4];
+        int64_t va[4], vb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int pr = min(p0 + u * kBlock, half - 1);
+          lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
+          va[u] = keys[lo[u]];
+          vb[u] = keys[lo[u] + jj];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool up = (lo[u] & kk) == 0;
+          if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
+            keys[lo[u]] = vb[u];
+            keys[lo[u] + jj] = va[u];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// A merge whose working arrays are in global memory still sorts up to
+// kGmemSortCap improving rows in LDS; the rest of its dynamic LDS holds the
+// conflict bitmap.
+constexpr int kGmemSortCap = 8192;
+
 template <bool GMEM>
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
@@ -769,9 +812,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   int32_t* told = (int32_t*)(keys + pcap);
   int32_t* tnew = told + ncap;
   int32_t* xyc = tnew + ncap;  // packed coordinates by client
-  uint32_t* bits = (uint32_t*)(xyc + ncap);
   const int nwords = (N + 32) / 32 + 1;
-  uint16_t* ctxL = (uint16_t*)(bits + (ncap + 32) / 32 + 1);  // ctx of each improving row
+  // the conflict bitmap: in the LDS layout, or after the LDS sort buffer
+  uint32_t* bits = GMEM ? (uint32_t*)(lds + kGmemSortCap) : (uint32_t*)(xyc + ncap);
+  uint16_t* ctxL = (uint16_t*)((uint32_t*)(xyc + ncap) + (ncap + 32) / 32 + 1);  // ctx of each improving row
+  int64_t* sk = keys;  // the sorted keys (greedy, apply)
 
   if (tid == 0) s_m = 0;
   for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
@@ -836,47 +881,22 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
     __syncthreads();
     if (tid < M) keys[rank] = my;
     __syncthreads();
+  } else if (GMEM && M <= kGmemSortCap) {
+    // the slot's working arrays are in global memory, but its improving rows
+    // fit the LDS sort buffer: sort them there, and let the greedy and the
+    // apply read them from LDS
+    int64_t* sb = lds;
+    for (int t = tid; t < M; t += kBlock) sb[t] = keys[t];
+    bitonic_keys(sb, M);
+    sk = sb;
   } else {
-    int P = 1;
-    while (P < M) P <<= 1;
-    for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
-    __syncthreads();
-    // Each phase has P/2 compare-swaps; a thread takes up to four pairs at a
-    // time, loading all of them before comparing.
-    {
-      const int half = P >> 1;
-      for (int kk = 2; kk <= P; kk <<= 1) {
-        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-          const int lj = 31 - __clz(jj);
-          for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
-            int lo[4];
-            int64_t va[4], vb[4];
-  #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int pr = min(p0 + u * kBlock, half - 1);
-              lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
-              va[u] = keys[lo[u]];
-              vb[u] = keys[lo[u] + jj];
-            }
-  #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const bool up = (lo[u] & kk) == 0;
-              if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
-                keys[lo[u]] = vb[u];
-                keys[lo[u] + jj] = va[u];
-              }
-            }
-          }
-          __syncthreads();
-        }
-      }
-    }
+    bitonic_keys(keys, M);
   }
   // 3. greedy filter on wave 0
   if (tid < kWave) {
     int kept = 0;
 #define MLP_GREEDY(NB) \
-  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, keys, ctxL, bits, tid); break
+  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, sk, ctxL, bits, tid); break
     switch (nb) {
       MLP_GREEDY(0); MLP_GREEDY(1); MLP_GREEDY(2); MLP_GREEDY(3); MLP_GREEDY(4);
       MLP_GREEDY(5); MLP_GREEDY(6); MLP_GREEDY(7); MLP_GREEDY(8); MLP_GREEDY(9);
@@ -889,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   __syncthreads();
   const int K = s_kept;
   // 4. apply the kept moves, rebuild the prefix arrays
-  apply_kept(s, nb, K, [&](int q) { return keys[q]; }, told, tnew, s_long, s_longq, true);
+  apply_kept(s, nb, K, [&](int q) { return sk[q]; }, told, tnew, s_long, s_longq, true);
   prep_wg(s, tnew, PL, xyc);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
@@ -2014,10 +2034,16 @@ size_t mlp_merge_lds_bytes(int pcap, int ncap) {
   return (size_t)pcap * 8 + (size_t)ncap * 12 + (size_t)nwords * 4 + (size_t)pcap * 2;
 }
 
+// Dynamic LDS of a merge whose working arrays are in global memory: the
+// sort buffer and the conflict bitmap of the largest N.
+constexpr size_t kGmemMergeLds = kGmemSortCap * 8 + ((MLP_MAX_N + 32) / 32 + 1) * 4;
+
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
-  const hipError_t e = hipFuncSetAttribute((const void*)k_merge_apply<false>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)max_dyn_lds);
+  hipError_t e = hipFuncSetAttribute((const void*)k_merge_apply<false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_merge_apply<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGmemMergeLds);
   if (e != hipSuccess) (void)hipGetLastError();  // do not leave it for the next launch check
   return e;
 }
@@ -2026,8 +2052,8 @@ hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const Sl
                                   const int32_t* list_dev, int count, int nb, int rowmap,
                                   int pcap, int ncap, int skip_inactive, int gmem, int tail) {
   if (gmem)
-    hipLaunchKernelGGL(k_merge_apply<true>, dim3(count), dim3(kBlock), 0, st, table, inl,
-                       list_dev, nb, rowmap, pcap, ncap, skip_inactive, tail);
+    hipLaunchKernelGGL(k_merge_apply<true>, dim3(count), dim3(kBlock), kGmemMergeLds, st, table,
+                       inl, list_dev, nb, rowmap, pcap, ncap, skip_inactive, tail);
   else
     hipLaunchKernelGGL(k_merge_apply<false>, dim3(count), dim3(kBlock),
                        mlp_merge_lds_bytes(pcap, ncap), st, table, inl, list_dev, nb, rowmap, pcap,
