@@ -1791,7 +1791,11 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
       }
     }
   }
-  if (!p32) {
+  // Tours whose 2-opt deltas may leave int32: the wide tiles keep the
+  // diagonal tiling (shared distances, column tiles) with 64-bit prefix
+  // arithmetic; the small ones, whose register budget sets their occupancy,
+  // take one wave per row over the whole row.
+  if (!p32 && C < 1024) {
     sweep_2opt_wide(s, rb, ct, parity);
     return;
   }
@@ -1810,6 +1814,48 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
     }
   }
   __syncthreads();
+  if (!p32) {
+    // delta = (N-i)(e[x] - w[i]) + (N-j-1)(e[x+1] - w[j+1]) + 2(Q[j] - Q[i])
+    //         - (i+j+1)(T[j] - T[i])   (the ADS form of mlk2opt.cu, 64-bit)
+    // each lane keeps the smallest key delta * 2^20 + ctx of its moves
+    int64_t best = MLP_KEY_NONE;
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const uint16_t* e = L.e[wave][part];
+      const int m = mm[part], lo = ilo[part], hi = ihi[part];
+      for (int i0 = lo; i0 <= hi; i0 += CU * kWave) {
+        int32_t wi[CU], wj[CU];
+        int64_t qi[CU], qj[CU], ti[CU], tj[CU];
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int i = min(i0 + lane + u * kWave, hi);
+          const int j = i + m;
+          wi[u] = ldg(s.w + i);
+          wj[u] = ldg(s.w + j + 1);
+          qi[u] = ldg(s.q64 + i);
+          qj[u] = ldg(s.q64 + j);
+          ti[u] = ldg(s.t64 + i);
+          tj[u] = ldg(s.t64 + j);
+        }
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+          const int iu = i0 + lane + u * kWave;
+          const int i = min(iu, hi);
+          const int j = i + m;
+          const int x = i - lo;
+          const int t1 = __mul24(N - i, (int)e[x] - wi[u]);
+          const int t2 = __mul24(N - j - 1, (int)e[x + 1] - wj[u]);
+          const int64_t d = (int64_t)t1 + t2 + 2 * (qj[u] - qi[u]) - (int64_t)(i + j + 1) * (tj[u] - ti[u]);
+          const int ctx = part == 0 ? by - i : i + by - 1;
+          const int64_t key = d * (int64_t)(1 << MLP_KEY_SHIFT) + ctx;
+          best = (iu <= hi && key < best) ? key : best;
+        }
+      }
+    }
+    best = wave_min64(best);
+    if (live && lane == 0) fold_key(row_buf(s, 1, par) + by, best);
+    return;
+  }
   // The prefix arrays are read from global memory in batches of CU x 64
   // positions; the batches of both parts form one sequence and batch k + 1
   // is loaded while batch k is evaluated, so each batch's memory latency
