@@ -304,7 +304,9 @@ def main():
     tune = {}
     tile = args.tile
     if tile < 0:
-        probe = slots[:min(T, 128)]
+        # every tour, as in the timed region: a shorter probe stays hotter in
+        # the caches than the 1000-tour pass and can rank the shapes wrongly
+        probe = slots
         for t in TILES:
             eng.set_tile(t)
             eng.eval_each(probe, 0x1F, 0)
