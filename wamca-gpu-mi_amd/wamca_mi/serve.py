@@ -93,14 +93,17 @@ def run_served(eng, jobs, arrivals, rnd=1, rowmap=ROWMAP_ADS, hbm_budget=0, max_
         nxt, results, active, queued = 0, {}, 0, 0
         while len(results) < len(jobs):
             now = srv.now() - t0
+            submitted = False
             while nxt < len(jobs) and arrivals[nxt] <= now:
                 xs, ys, tour = jobs[nxt]
                 srv.submit(xs, ys, tour, rnd, tag=nxt)
                 nxt += 1
-            if active == 0 and queued == 0 and nxt < len(jobs) and arrivals[nxt] > now:
+                submitted = True
+            # idle (nothing resident, queued or just submitted): wait for the
+            # next arrival; a job submitted now is stepped at once
+            if not submitted and active == 0 and queued == 0 and nxt < len(jobs):
                 time.sleep(min(arrivals[nxt] - now, 0.05))
-                if nxt < len(jobs) and arrivals[nxt] > srv.now() - t0:
-                    continue
+                continue
             active, queued = srv.step()
             for r in srv.poll():
                 results[r["tag"]] = r
