@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "wamca-gpu-mi_amd"))
 PEAK_HBM_GBS = 8000.0
 CUS, LDS_BYTES_PER_CLK, CLOCK_GHZ = 256, 128, 2.4
 PEAK_LDS_GBS = CUS * LDS_BYTES_PER_CLK * CLOCK_GHZ  # 78,643 GB/s
+PEAK_VALU_TLOPS = CUS * 4 * 16 * CLOCK_GHZ / 1e3  # 39.3 T lane-instructions/s
 
 # lane-varying operand bytes per move of the reference's ADS formulation
 # (swap 7 words, 2-opt 5, or-opt 5 forward / 6 backward -> 22 B average)
@@ -241,15 +242,33 @@ def dvnd_legs(eng, args):
     return res
 
 
-def traffic_per_launch(tile):
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (calibrated against a copy kernel), if present."""
+def profile_figure(tile, key):
+    """A per-launch figure of this workload's tile shape from the committed
+    rocprofv3 summaries: HBM bytes per launch (FETCH_SIZE/WRITE_SIZE,
+    calibrated against a copy kernel) or VALU lane-operations per move
+    (SQ_INSTS_VALU x 64 / moves), if present."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        return t["kernels"][TILES[tile]]["hbm_bytes_per_launch"]
+        return t["kernels"][TILES[tile]][key]
     except (OSError, ValueError, KeyError):
         return None
+
+
+def traffic_per_launch(tile):
+    return profile_figure(tile, "hbm_bytes_per_launch")
+
+
+def valu_record(tile, moves_per_s):
+    """VALU issue rate of the sweep (its binding resource): lane-operations
+    per move from the counters x moves/s, against 256 CU x 4 SIMD x 16 lanes
+    x 2.4 GHz."""
+    per_move = profile_figure(tile, "valu_lane_ops_per_move")
+    if per_move is None:
+        return None
+    ach = per_move * moves_per_s / 1e12
+    return {"lane_ops_per_move": per_move, "achieved": ach, "peak": PEAK_VALU_TLOPS,
+            "unit": "T lane-instructions/s", "frac": ach / PEAK_VALU_TLOPS}
 
 
 def main():
@@ -360,7 +379,8 @@ def main():
                      "kernel": TILES[tile], "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": lds_bytes,
                      "hbm_algorithmic_bytes_per_launch": hbm_alg,
-                     "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS},
+                     "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS,
+                     "valu": valu_record(tile, per_tour / avg_launch_s)},
         "autotune_us_per_launch": tune,
         "device": info,
     }
@@ -381,7 +401,8 @@ def main():
             dtb = (time.perf_counter() - t1) / reps
             rec = {"moves_per_s": T * per_tour / dtb, "tours_per_launch": T,
                    "kernel": TILES[2], "persistent": persistent, "avg_launch_ms": bms,
-                   "lds_frac": lds_bytes * T / (bms / 1e3) / 1e9 / PEAK_LDS_GBS}
+                   "lds_frac": lds_bytes * T / (bms / 1e3) / 1e9 / PEAK_LDS_GBS,
+                   "valu": valu_record(2, T * per_tour / (bms / 1e3))}
             if best is None or rec["moves_per_s"] > best["moves_per_s"]:
                 best = rec
         out["batched"] = best
