@@ -34,7 +34,9 @@ enum {
   MLP_INFO_WORDS = 40
 };
 
-enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2 };
+// info[PATH32] bits: 2-opt / swap and or-opt deltas fit int32; swap and
+// or-opt deltas below 2^26 (keys packed with a 5-bit column iteration)
+enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2, MLP_PK_SWAP_OROPT = 4 };
 
 struct SlotDev {
   int32_t N;      // positions (n + 1)

@@ -462,7 +462,8 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
     s.info[MLP_INFO_LATENCY] = L;
     s.info[MLP_INFO_TTOT] = Ttot;
     s.info[MLP_INFO_PATH32] = (b2 < 2147483647LL ? MLP_P32_2OPT : 0) |
-                              (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0);
+                              (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0) |
+                              (bso < (1LL << 26) ? MLP_PK_SWAP_OROPT : 0);
   }
   __syncthreads();
 }
@@ -1470,13 +1471,31 @@ __device__ __forceinline__ void take_min_if(int32_t& bd, int32_t& bc, uint32_t d
 template <bool B>
 using Flag = std::integral_constant<bool, B>;
 
+// Packed per-lane key of a tour whose |delta| < 2^26: delta * 32 + the
+// lane's column iteration (below 32), so one 32-bit minimum keeps the lowest
+// delta and, among equal ones, the lane's lowest column.
+__device__ __forceinline__ int32_t pack_key(uint32_t d, int it) {
+  return (int32_t)((d << 5) + (uint32_t)it);
+}
+
+// The row key from per-lane packed keys: one wave minimum, then the lowest
+// lane holding it (its column is the lowest among equal deltas).  Column of
+// lane l at iteration it: jb0 + it * 64 + l; ctx = column + off.
+__device__ __forceinline__ int64_t packed_row_key(int32_t bk, int jb0, int off) {
+  const int32_t m = wave_min_i32(bk);
+  if (m == INT32_MAX) return MLP_KEY_NONE;
+  const int l0 = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(bk == m));
+  const int ctx = jb0 + (m & 31) * kWave + l0 + off;
+  return (int64_t)(m >> 5) * (int64_t)(1 << MLP_KEY_SHIFT) + ctx;
+}
+
 // Swap rows of a tile: each wave owns RW consecutive rows (r0 = wave * RW),
 // lanes stride over the tile's columns j and evaluate all RW rows for their
 // column.  Row r: i = b0+r+1, j in [i+2, N-2].  delta = Cs_r + Bs(j) +
 // (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j] + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
 template <int R, int C, int RW>
 __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                          int parity, int wave, int lane) {
+                          int parity, int wave, int lane, bool pk) {
   const int N = s.N;
   const int r0 = wave * RW;
   const int rv = min(RW, N - 4 - b0 - r0);  // rows with i <= N-4
@@ -1499,10 +1518,14 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - 1);
   const int jlo = max(q0, b0 + r0 + 3);
-  // FR: every row of the wave is in range (no per-row guard in the loop)
-  auto cols = [&](auto fr) {
+  const int jb0 = jlo - ((jlo - q0) & (kWave - 1));
+  // FR: every row of the wave is in range (no per-row guard in the loop);
+  // PK: packed keys in bd (bc unused)
+  auto cols = [&](auto fr, auto pkc) {
     constexpr bool FR = decltype(fr)::value;
-    for (int jb = jlo - ((jlo - q0) & (kWave - 1)); jb < jhi; jb += kWave) {
+    constexpr bool PK = decltype(pkc)::value;
+    int it = 0;
+    for (int jb = jb0; jb < jhi; jb += kWave, ++it) {
       const int j = jb + lane;
       const int jj = min(max(j, jlo), jhi - 1);
       const int x = jj - cbase;
@@ -1518,15 +1541,32 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
           const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
           const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) +
                              umul(nj, V[r + 1][x - 1]) + umul(nj - 1u, V[r + 1][x + 1]);
-          if (full) take_min(bd[r], bc[r], d, j);
-          else take_min_if(bd[r], bc[r], d, j, j < jhi && j >= b0 + r0 + r + 3);
+          if (PK) {
+            const int32_t k = pack_key(d, it);
+            if (full) bd[r] = min(bd[r], k);
+            else bd[r] = (j < jhi && j >= b0 + r0 + r + 3) ? min(bd[r], k) : bd[r];
+          } else if (full) {
+            take_min(bd[r], bc[r], d, j);
+          } else {
+            take_min_if(bd[r], bc[r], d, j, j < jhi && j >= b0 + r0 + r + 3);
+          }
         }
       }
     }
   };
-  if (rv == RW) cols(Flag<true>{});
-  else cols(Flag<false>{});
   int64_t* out = row_buf(s, 0, parity & 1) + b0 + r0;
+  if (pk) {
+    if (rv == RW) cols(Flag<true>{}, Flag<true>{});
+    else cols(Flag<false>{}, Flag<true>{});
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int64_t key = packed_row_key(bd[r], jb0, 0);
+      if (lane == 0 && r < rv) fold_key(out + r, key);
+    }
+    return;
+  }
+  if (rv == RW) cols(Flag<true>{}, Flag<false>{});
+  else cols(Flag<false>{}, Flag<false>{});
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int64_t key = wave_argmin(bd[r], bc[r]);
@@ -1539,7 +1579,7 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
 // [1, N-K-1]; row-local index j - 1.
 template <int R, int C, int RW, int K>
 __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                           int parity, int wave, int lane) {
+                           int parity, int wave, int lane, bool pk) {
   const int N = s.N;
   const float dr = round_dr(s.round);
   const int r0 = wave * RW;
@@ -1581,9 +1621,15 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
   }
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - K);
-  auto cols = [&](auto fr) {
+  // PK: packed keys in bd (bc unused)
+  auto upd = [&](auto pkc, int r, uint32_t d, int it, int j, bool ok) {
+    if constexpr (decltype(pkc)::value) bd[r] = ok ? min(bd[r], pack_key(d, it)) : bd[r];
+    else take_min_if(bd[r], bc[r], d, j - 1, ok);
+  };
+  auto cols = [&](auto fr, auto pkc) {
     constexpr bool FR = decltype(fr)::value;
-    for (int jb = q0; jb < jhi; jb += kWave) {
+    int it = 0;
+    for (int jb = q0; jb < jhi; jb += kWave, ++it) {
       const int j = jb + lane;
       const int jj = min(j, jhi - 1);
       const int x = jj - cbase;
@@ -1598,7 +1644,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
           if (FR || r < rv) {
             uint32_t d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
             if (K > 1) d += smul(jj, nSF[r]);
-            take_min(bd[r], bc[r], d, j - 1);
+            upd(pkc, r, d, it, j, true);
           }
         }
       } else if (allB) {
@@ -1608,7 +1654,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
           if (FR || r < rv) {
             uint32_t d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
             if (K > 1) d += smul(jj, nSB[r]);
-            take_min(bd[r], bc[r], d, j - 1);
+            upd(pkc, r, d, it, j, true);
           }
         }
       } else {
@@ -1624,15 +1670,25 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
               dF += smul(jj, nSF[r]);
               dB += smul(jj, nSB[r]);
             }
-            take_min_if(bd[r], bc[r], fwd ? dF : dB, j - 1, j < jhi);
+            upd(pkc, r, fwd ? dF : dB, it, j, j < jhi);
           }
         }
       }
     }
   };
-  if (rv == RW) cols(Flag<true>{});
-  else cols(Flag<false>{});
   int64_t* out = row_buf(s, K + 1, (parity >> (K + 1)) & 1) + bb;
+  if (pk) {
+    if (rv == RW) cols(Flag<true>{}, Flag<true>{});
+    else cols(Flag<false>{}, Flag<true>{});
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int64_t key = packed_row_key(bd[r], q0, -1);  // ctx = j - 1
+      if (lane == 0 && r < rv) fold_key(out + r, key);
+    }
+    return;
+  }
+  if (rv == RW) cols(Flag<true>{}, Flag<false>{});
+  else cols(Flag<false>{}, Flag<false>{});
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const int64_t key = wave_argmin(bd[r], bc[r]);
@@ -1674,7 +1730,9 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
   const int cend = min(q1 + 3, N);
   // every global load of the tile is issued before the first wait
-  const bool p32 = (ldg(s.info + MLP_INFO_PATH32) & MLP_P32_SWAP_OROPT) != 0;
+  const int64_t pflags = ldg(s.info + MLP_INFO_PATH32);
+  const bool p32 = (pflags & MLP_P32_SWAP_OROPT) != 0;
+  const bool pk = (pflags & MLP_PK_SWAP_OROPT) != 0;  // |delta| < 2^26: packed keys
   int xa[R + 4];
 #pragma unroll
   for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
@@ -1732,16 +1790,16 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
         if (q == 0) nb = b;
         --q;
       }
-    if (nb == 0) tile_swap<R, C, R>(s, b0, q0, q1, cbase, L, parity, 0, lane);
-    else if (nb == 2) tile_oropt<R, C, R, 1>(s, b0, q0, q1, cbase, L, parity, 0, lane);
-    else if (nb == 3) tile_oropt<R, C, R, 2>(s, b0, q0, q1, cbase, L, parity, 0, lane);
-    else if (nb == 4) tile_oropt<R, C, R, 3>(s, b0, q0, q1, cbase, L, parity, 0, lane);
+    if (nb == 0) tile_swap<R, C, R>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
+    else if (nb == 2) tile_oropt<R, C, R, 1>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
+    else if (nb == 3) tile_oropt<R, C, R, 2>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
+    else if (nb == 4) tile_oropt<R, C, R, 3>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
     return;
   }
-  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane);
-  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane);
+  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
 }
 
 __device__ __forceinline__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
