@@ -41,43 +41,61 @@ def moves_per_nb(N):
             2: (N - 3) * (N - 2), 3: (N - 4) * (N - 3), 4: (N - 5) * (N - 4)}
 
 
-def cpu_baseline(xs, ys, tours, budget_s=12.0):
-    """Oracle per-row bests of the 5 neighborhoods on a sample of the tours,
-    one tour per task on a pool of host threads (the oracle's C calls release
-    the interpreter lock), as many threads as the box's CPU share allows."""
-    import os
-    import threading
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(xs, ys, tours, budget_s=30.0, chunk=100):
+    """The oracle's per-row bests of the 5 neighborhoods (the ADS concatenation
+    over the prefix form of the ADS, g++ -O3 -march=x86-64-v3), OpenMP over
+    (tour x row) on the host cores the box grants this process, timed tour
+    chunk by tour chunk until every tour is done or the budget is spent.
+    Returns the record and the oracle's rows {nb: cost, i, j (tours, rows)}."""
+    import numpy as np
     from oracle import oracle as O
-    W = O.weights(xs, ys, 1)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    mp = sum(moves_per_nb(len(tours[0])).values())
-    lock = threading.Lock()
-    state = {"next": 0, "done": 0}
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16,
+                         os.cpu_count() or 1))
+    rows = {nb: {"cost": [], "i": [], "j": []} for nb in range(5)}
+    done = moves = 0
     t0 = time.perf_counter()
-
-    def work():
-        while True:
-            with lock:
-                q = state["next"]
-                if q >= len(tours) or time.perf_counter() - t0 >= budget_s:
-                    return
-                state["next"] = q + 1
-            for nb in range(5):
-                O.eval_rows(nb, 0, W, tours[q])
-            with lock:
-                state["done"] += 1
-
-    pool = [threading.Thread(target=work) for _ in range(threads)]
-    for th in pool:
-        th.start()
-    for th in pool:
-        th.join()
+    while done < len(tours) and time.perf_counter() - t0 < budget_s:
+        part, m = O.eval_batch_xy(0x1F, 0, xs, ys, 1, tours[done:done + chunk], threads)
+        for nb in range(5):
+            for k in ("cost", "i", "j"):
+                rows[nb][k].append(part[nb][k])
+        moves += m
+        done += len(tours[done:done + chunk])
     dt = time.perf_counter() - t0
-    done = state["done"]
-    moves = done * mp
-    return {"value": moves / dt, "unit": "moves/s", "cores": threads, "kind": "port",
-            "sample": f"{done} of the 1000 n=1000 tours x 5 neighborhoods "
-                      f"({moves} moves, {dt:.1f} s, oracle eval_rows, {threads} threads)"}
+    rows = {nb: {k: np.concatenate(v) for k, v in r.items()} for nb, r in rows.items()}
+    rec = {"value": moves / dt, "unit": "moves/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"{done} of the {len(tours)} n=1000 tours x 5 neighborhoods ({moves} moves, "
+                     f"{dt:.1f} s): the oracle's per-row bests, g++ -O3 -march=x86-64-v3, "
+                     f"OpenMP over (tour x row) on {threads} threads"}
+    return rec, rows, done
+
+
+def parity_check(eng, slots, rows, count):
+    """GPU rows of the first `count` tours (their last evaluation) against the
+    oracle's: every row's cost, and (i, j) where the row has a move."""
+    import numpy as np
+    none = np.iinfo(np.int64).max
+    bad = 0
+    for q in range(count):
+        for nb in range(5):
+            g = eng.rows(slots[q], nb)
+            c = rows[nb]["cost"][q]
+            has = c != none
+            bad += int(((g["cost"] != c) | (has & ((g["i"] != rows[nb]["i"][q]) |
+                                                   (g["j"] != rows[nb]["j"][q])))).sum())
+    return bad
 
 
 def batched_cfg2(eng):
@@ -193,6 +211,12 @@ def dvnd_legs(eng, args):
                          "rounds": int(rounds[0])}
     eng.destroy_slot(s)
     eng.destroy_instance(inst)
+    fixtures = {}
+    try:  # the oracle's DVND results of these workloads
+        with open(os.path.join(ROOT, "tests", "golden", "dvnd_fixtures.json")) as f:
+            fixtures = json.load(f)
+    except (OSError, ValueError):
+        pass
     # 64 instances, n log-uniform [50, 2000], half uniform / half clustered
     jobs, meta = WL.dvnd_jobs(64, 3)
     serve.run_served(eng, jobs[:8], [0.0] * 8)  # warm-up
@@ -203,7 +227,10 @@ def dvnd_legs(eng, args):
         "served_solves_per_s": sv["solves_per_s"], "plain_solves_per_s": pl["solves_per_s"],
         "ratio": sv["solves_per_s"] / pl["solves_per_s"],
         "served_makespan_s": sv["makespan_s"], "plain_makespan_s": pl["makespan_s"],
-        "identical_results": sv["latency"] == pl["latency"]}
+        "identical_results": sv["latency"] == pl["latency"],
+        "oracle_matches": (sum(int(a == r["latency"] and b == r["rounds"]) for a, b, r in
+                               zip(sv["latency"], sv["rounds"], fixtures["batched_64"]))
+                           if "batched_64" in fixtures else None)}
     # n log-uniform in [100, 1000]: all jobs at t = 0 (capacity) and a Poisson
     # stream; n in [50, 2000] with a fraction of clustered n = 18000 jobs
     for name, lo, hi, big, rate in (("burst_100_1000", 100, 1000, 0.0, 0.0),
@@ -223,6 +250,13 @@ def dvnd_legs(eng, args):
             "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
             "ratio": sv["solves_per_s"] / pl["solves_per_s"],
             "identical_results": sv["latency"] == pl["latency"]}
+        rec = fixtures.get("big")
+        if big > 0 and rec and rec["stream"] == {"count": args.stream_jobs, "seed": 4, "lo": lo,
+                                                 "hi": hi, "big_frac": big,
+                                                 "job": rec["stream"]["job"]}:
+            q = rec["stream"]["job"]
+            res[name]["n18000_oracle_match"] = (sv["latency"][q] == rec["latency"]
+                                                and sv["rounds"][q] == rec["rounds"])
     # Poisson rate sweep, n log-uniform [100, 1000] (no n = 18000 jobs): about
     # one second of arrivals per rate (60..300 jobs)
     sweep = {}
@@ -346,6 +380,13 @@ def main():
     region_ms = eng.region_end()
     dt = time.perf_counter() - t0
     eng.set_tile(-1)  # later legs use the autotuned table
+    # the CPU baseline right after the timed region: its rows check the GPU
+    # rows every slot still holds from its last timed launch
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, cpu_rows, checked = cpu_baseline(xs, ys, tours)
+        parity = {"tours": checked, "mismatched_rows": parity_check(eng, slots, cpu_rows, checked)}
+        del cpu_rows
     launches = args.steps * T
     value = launches * per_tour / dt
     avg_launch_s = region_ms / 1e3 / launches
@@ -382,6 +423,8 @@ def main():
                      "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS,
                      "valu": valu_record(tile, per_tour / avg_launch_s)},
         "autotune_us_per_launch": tune,
+        "parity_checked": parity["tours"] if parity else 0,
+        "parity": parity,
         "device": info,
     }
 
@@ -412,8 +455,8 @@ def main():
         out["dvnd"] = dvnd_legs(eng, args)
     if not args.no_total:
         out["total_mode"] = total_legs(eng)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(xs, ys, tours)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     eng.close()
     print(json.dumps(out))
 

@@ -85,7 +85,16 @@ int mlp_sync(mlp_ctx* ctx);
 int mlp_pool_stats(mlp_ctx* ctx, int64_t* live_bytes, int64_t* reserved_bytes);
 int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* name, int32_t name_len);
 
-/* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5). */
+/* Device memory free and total on the context's GPU (hipMemGetInfo). */
+int mlp_mem_info(mlp_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+
+/* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5).
+ * Coordinates may be any int32 values whose span (max - min) is at most 32767
+ * in x and in y; they are stored translated to [0, span], which leaves every
+ * distance unchanged (the reference reads TSPLIB coordinates as they are,
+ * mlproblem.cpp:110-117, 153-166).  MLP_ERANGE: a larger span, n + 1 above
+ * MLP_MAX_N, or N (N + 32) maxd >= 2^43 (maxd bounds a distance; the row keys
+ * hold delta * 2^20 + index in 64 bits). */
 int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                         int32_t round, int32_t* inst);
 /* As mlp_instance_create, but the coordinates are staged in pinned host
@@ -95,6 +104,11 @@ int mlp_instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int3
 int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                               int32_t round, int32_t* inst);
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst);
+/* The checks mlp_instance_create and mlp_set_tours apply to one job (n
+ * clients, a tour of n + 1 positions), without device work: MLP_OK, or the
+ * status and message those calls would return. */
+int mlp_check_job(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n, int32_t round,
+                  const int32_t* tour);
 /* Device-computed client-indexed distance matrix (n*n int32) -- used to check
  * the device distance function against the host one. */
 int mlp_instance_weights(mlp_ctx* ctx, int32_t inst, int32_t* w_out);

@@ -34,6 +34,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // ---- MT19937 (mtrand.cpp:51-61 seed, :99-121 genState; mtrand.h:168-182) ----
@@ -194,36 +196,37 @@ bool move_of(int nb, int rowmap, int N, int by, int ctx, int* pi, int* pj) {
 // ---- move cost by O(1) subsequence concatenation over the ADS -------------------
 // Restates the per-thread cost of the ADS kernels: swap mlkswap.cu:210-278,
 // 2-opt mlk2opt.cu:168-219, or-opt mlkoropt.cu:187-300.
-int64_t delta_ads(int nb, const int32_t* W, int n, const int32_t* t, int N, const int64_t* T,
-                  const int64_t* C, int64_t L, int i, int j) {
+// Cm(a, b) is the ADS cost entry C[a][b], Dd(a, b) the distance between the
+// clients at tour positions a and b; T is the prefix time and L the latency.
+template <class CM, class DD>
+int64_t delta_concat(int nb, int N, const int64_t* T, int64_t L, int i, int j, CM Cm, DD Dd) {
   const int last = N - 1;
-  auto Cm = [&](int a, int b) { return C[(int64_t)a * N + b]; };
   int64_t dist, cost, time, wait;
   if (nb == 0) {
-    dist = D(W, n, t, i - 1, j);
+    dist = Dd(i - 1, j);
     cost = Cm(0, i - 1) + 1 * (T[i - 1] + dist);
     time = T[i - 1] + dist;
-    dist = D(W, n, t, j, i + 1);
+    dist = Dd(j, i + 1);
     wait = j - i - 1;
     cost += Cm(i + 1, j - 1) + wait * (time + dist);
     time += (T[j - 1] - T[i + 1]) + dist;
-    dist = D(W, n, t, j - 1, i);
+    dist = Dd(j - 1, i);
     cost += time + dist;
     time += dist;
     if (j + 1 <= last) {
-      dist = D(W, n, t, i, j + 1);
+      dist = Dd(i, j + 1);
       wait = last - j;
       cost += Cm(j + 1, last) + wait * (time + dist);
     }
     return cost - L;
   }
   if (nb == 1) {
-    dist = D(W, n, t, i - 1, j);
+    dist = Dd(i - 1, j);
     wait = j - i + 1;
     cost = Cm(0, i - 1) + Cm(j, i) + wait * (T[i - 1] + dist);
     time = T[i - 1] + (T[j] - T[i]) + dist;
     if (j + 1 <= last) {
-      dist = D(W, n, t, i, j + 1);
+      dist = Dd(i, j + 1);
       wait = last - j;
       cost += Cm(j + 1, last) + wait * (time + dist);
     }
@@ -231,34 +234,87 @@ int64_t delta_ads(int nb, const int32_t* W, int n, const int32_t* t, int N, cons
   }
   const int k = nb_k(nb);
   if (i < j) {
-    dist = D(W, n, t, i - 1, i + k);
+    dist = Dd(i - 1, i + k);
     wait = j - i;
     cost = Cm(0, i - 1) + Cm(i + k, j + k - 1) + wait * (T[i - 1] + dist);
     time = T[i - 1] + (T[j + k - 1] - T[i + k]) + dist;
-    dist = D(W, n, t, j + k - 1, i);
+    dist = Dd(j + k - 1, i);
     cost += Cm(i, i + k - 1) + (int64_t)k * (time + dist);
     time += (T[i + k - 1] - T[i]) + dist;
     if (j + k <= last) {
-      dist = D(W, n, t, i + k - 1, j + k);
+      dist = Dd(i + k - 1, j + k);
       wait = last - j - k + 1;
       cost += Cm(j + k, last) + wait * (time + dist);
     }
   } else {
-    dist = D(W, n, t, j - 1, i);
+    dist = Dd(j - 1, i);
     cost = Cm(0, j - 1) + Cm(i, i + k - 1) + (int64_t)k * (T[j - 1] + dist);
     time = T[j - 1] + (T[i + k - 1] - T[i]) + dist;
-    dist = D(W, n, t, i + k - 1, j);
+    dist = Dd(i + k - 1, j);
     wait = i - j;
     cost += Cm(j, i - 1) + wait * (time + dist);
     time += (T[i - 1] - T[j]) + dist;
     if (i + k <= last) {
-      dist = D(W, n, t, i - 1, i + k);
+      dist = Dd(i - 1, i + k);
       wait = last - i - k + 1;
       cost += Cm(i + k, last) + wait * (time + dist);
     }
   }
   return cost - L;
 }
+
+int64_t delta_ads(int nb, const int32_t* W, int n, const int32_t* t, int N, const int64_t* T,
+                  const int64_t* C, int64_t L, int i, int j) {
+  return delta_concat(
+      nb, N, T, L, i, j, [&](int a, int b) { return C[(int64_t)a * N + b]; },
+      [&](int a, int b) { return D(W, n, t, a, b); });
+}
+
+// ---- the ADS in O(N) prefix form (closed form of ldsUpdate) ---------------------
+// With S[p] = sum_{q<=p} T[q] (S[-1] = 0) the recurrences of
+// mlsolution.cpp:460-478 sum to
+//   C[a][b] = (S[b] - S[a]) - (b - a) T[a]              for a < b,
+//   C[a][b] = (a - b) T[a] - (S[a-1] - S[b-1])          for a > b,   C[a][a] = 0.
+// Distances are computed from tour-ordered coordinates (no n x n matrix), so a
+// tour of N positions needs O(N) memory: the path for instances whose C or W
+// matrices would not fit (n = 18000: 2.6 GB of C per evaluation).
+struct PrefixAds {
+  int N = 0, round = 0;
+  std::vector<int32_t> px, py;
+  std::vector<int64_t> T, S;
+  int64_t L = 0;
+  void build(const int32_t* xs, const int32_t* ys, int rnd, const int32_t* t, int n_pos) {
+    N = n_pos;
+    round = rnd;
+    px.resize(N);
+    py.resize(N);
+    T.assign(N, 0);
+    S.assign(N, 0);
+    for (int p = 0; p < N; ++p) {
+      px[p] = xs[t[p]];
+      py[p] = ys[t[p]];
+    }
+    L = 0;
+    for (int p = 1; p < N; ++p) {
+      const int64_t w = d(p - 1, p);
+      T[p] = T[p - 1] + w;
+      L += (int64_t)(N - p) * w;
+    }
+    S[0] = T[0];
+    for (int p = 1; p < N; ++p) S[p] = S[p - 1] + T[p];
+  }
+  int64_t d(int a, int b) const { return dist_xy(px[a], py[a], px[b], py[b], round); }
+  int64_t C(int a, int b) const {
+    if (a < b) return (S[b] - S[a]) - (int64_t)(b - a) * T[a];
+    if (a > b) return (int64_t)(a - b) * T[a] - ((a > 0 ? S[a - 1] : 0) - (b > 0 ? S[b - 1] : 0));
+    return 0;
+  }
+  int64_t delta(int nb, int i, int j) const {
+    return delta_concat(
+        nb, N, T.data(), L, i, j, [&](int a, int b) { return C(a, b); },
+        [&](int a, int b) { return d(a, b); });
+  }
+};
 
 // ---- move cost by positional edge weights (ODC kernels) ------------------------
 // swap mlkswap.cu:770-785, 2-opt mlk2opt.cu:565-571, or-opt mlkoropt.cu:704-736.
@@ -365,6 +421,29 @@ void eval_rows(int nb, int rowmap, const int32_t* W, int n, const int32_t* t, in
       int i, j;
       if (!move_of(nb, rowmap, N, by, ctx, &i, &j)) continue;
       int64_t c = delta_ads(nb, W, n, t, N, T.data(), C.data(), L, i, j);
+      if (c < b.cost) b = RowBest{c, i, j, ctx};
+    }
+    out[by] = b;
+  }
+}
+
+// eval_rows over the prefix form (same enumeration and tie rule), rows split
+// over OpenMP threads (threads <= 0: the OpenMP default).
+void eval_rows_prefix(int nb, int rowmap, const PrefixAds& P, std::vector<RowBest>& out,
+                      int threads) {
+  const int N = P.N;
+  const int rows = rows_of(nb, rowmap, N);
+  out.assign(rows > 0 ? rows : 0, RowBest{kInf, 0, 0, -1});
+  if (rows <= 0) return;
+  const int nctx = ctx_of(nb, rowmap, N);
+  const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nt)
+  for (int by = 0; by < rows; ++by) {
+    RowBest b{kInf, 0, 0, -1};
+    for (int ctx = 0; ctx < nctx; ++ctx) {
+      int i, j;
+      if (!move_of(nb, rowmap, N, by, ctx, &i, &j)) continue;
+      const int64_t c = P.delta(nb, i, j);
       if (c < b.cost) b = RowBest{c, i, j, ctx};
     }
     out[by] = b;
@@ -729,5 +808,113 @@ int64_t or_dvnd(int rowmap, const int32_t* W, int n, int32_t* tour, int N, int m
                 int32_t* rounds_out, int64_t* moves_out) {
   return or_dvnd_nbs(rowmap, W, n, tour, N, max_rounds, 5, rounds_out, moves_out);
 }
+
+// ---------------------------------------------------------------------------------
+// O(N)-memory forms over client coordinates (xs, ys, distance rounding rnd):
+// the same evaluation, merge and DVND as above with the ADS in prefix form,
+// rows split over `threads` OpenMP threads (<= 0: the OpenMP default).
+// ---------------------------------------------------------------------------------
+int64_t or_delta_prefix(int nb, const int32_t* xs, const int32_t* ys, int rnd,
+                        const int32_t* tour, int N, int i, int j) {
+  PrefixAds P;
+  P.build(xs, ys, rnd, tour, N);
+  return P.delta(nb, i, j);
+}
+
+int64_t or_latency_xy(const int32_t* xs, const int32_t* ys, int rnd, const int32_t* tour, int N) {
+  PrefixAds P;
+  P.build(xs, ys, rnd, tour, N);
+  return P.L;
+}
+
+int or_eval_rows_xy(int nb, int rowmap, const int32_t* xs, const int32_t* ys, int rnd,
+                    const int32_t* tour, int N, int64_t* cost, int32_t* is, int32_t* js,
+                    int32_t* ctxs, int threads) {
+  PrefixAds P;
+  P.build(xs, ys, rnd, tour, N);
+  std::vector<RowBest> rb;
+  eval_rows_prefix(nb, rowmap, P, rb, threads);
+  for (size_t r = 0; r < rb.size(); ++r) {
+    cost[r] = rb[r].cost;
+    is[r] = rb[r].i;
+    js[r] = rb[r].j;
+    if (ctxs) ctxs[r] = rb[r].ctx;
+  }
+  return (int)rb.size();
+}
+
+// DVND fixed point as or_dvnd_nbs, over coordinates.
+int64_t or_dvnd_xy(int rowmap, const int32_t* xs, const int32_t* ys, int rnd, int32_t* tour,
+                   int N, int max_rounds, int nbs, int32_t* rounds_out, int64_t* moves_out,
+                   int threads) {
+  int rounds = 0;
+  int64_t moves = 0;
+  PrefixAds P;
+  std::vector<RowBest> rb;
+  std::vector<int> kept;
+  while (max_rounds <= 0 || rounds < max_rounds) {
+    int applied = 0;
+    for (int nb = 0; nb < nbs; ++nb) {
+      if (rows_of(nb, rowmap, N) <= 0) continue;
+      P.build(xs, ys, rnd, tour, N);
+      eval_rows_prefix(nb, rowmap, P, rb, threads);
+      int64_t s;
+      const int c = merge_rows(nb, rb, kept, &s);
+      for (int q : kept) apply_move(nb, tour, rb[q].i, rb[q].j);
+      applied += c;
+    }
+    ++rounds;
+    moves += applied;
+    if (applied == 0) break;
+  }
+  if (rounds_out) *rounds_out = rounds;
+  if (moves_out) *moves_out = moves;
+  PrefixAds F;
+  F.build(xs, ys, rnd, tour, N);
+  return F.L;
+}
+
+// Per-row bests of every neighborhood in nb_mask (ids 0..4) for `count` tours
+// of N positions each (concatenated), OpenMP over (tour x row).  Output for
+// tour q and neighborhood nb starts at q * stride + off[nb], where off[] is the
+// prefix sum of rows_of(nb, rowmap, N) over the ids in the mask and stride
+// their total.  Returns the moves evaluated.
+int64_t or_eval_batch_xy(uint32_t nb_mask, int rowmap, const int32_t* xs, const int32_t* ys,
+                         int rnd, const int32_t* tours, int count, int N, int64_t* cost,
+                         int32_t* is, int32_t* js, int threads) {
+  int off[5], stride = 0;
+  for (int nb = 0; nb < 5; ++nb) {
+    off[nb] = stride;
+    if ((nb_mask >> nb) & 1u) stride += std::max(0, rows_of(nb, rowmap, N));
+  }
+  std::vector<PrefixAds> P((size_t)count);
+  const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int q = 0; q < count; ++q) P[(size_t)q].build(xs, ys, rnd, tours + (int64_t)q * N, N);
+  const int64_t items = (int64_t)count * stride;
+  int64_t moves = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : moves)
+  for (int64_t it = 0; it < items; ++it) {
+    const int q = (int)(it / stride);
+    int r = (int)(it % stride), nb = 0;
+    while (nb < 4 && (!((nb_mask >> nb) & 1u) || r >= off[nb] + rows_of(nb, rowmap, N))) ++nb;
+    const int by = r - off[nb];
+    const int nctx = ctx_of(nb, rowmap, N);
+    RowBest b{kInf, 0, 0, -1};
+    for (int ctx = 0; ctx < nctx; ++ctx) {
+      int i, j;
+      if (!move_of(nb, rowmap, N, by, ctx, &i, &j)) continue;
+      const int64_t c = P[(size_t)q].delta(nb, i, j);
+      ++moves;
+      if (c < b.cost) b = RowBest{c, i, j, ctx};
+    }
+    cost[it] = b.cost;
+    is[it] = b.i;
+    js[it] = b.j;
+  }
+  return moves;
+}
+
+int or_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

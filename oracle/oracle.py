@@ -76,6 +76,16 @@ def lib():
             "or_dvnd": (C.c_int64, [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int, I32P, I64P]),
             "or_dvnd_nbs": (C.c_int64, [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int, C.c_int,
                                         I32P, I64P]),
+            "or_delta_prefix": (C.c_int64, [C.c_int, I32P, I32P, C.c_int, I32P, C.c_int, C.c_int,
+                                            C.c_int]),
+            "or_latency_xy": (C.c_int64, [I32P, I32P, C.c_int, I32P, C.c_int]),
+            "or_eval_rows_xy": (C.c_int, [C.c_int, C.c_int, I32P, I32P, C.c_int, I32P, C.c_int,
+                                          I64P, I32P, I32P, I32P, C.c_int]),
+            "or_dvnd_xy": (C.c_int64, [C.c_int, I32P, I32P, C.c_int, I32P, C.c_int, C.c_int,
+                                       C.c_int, I32P, I64P, C.c_int]),
+            "or_eval_batch_xy": (C.c_int64, [C.c_uint32, C.c_int, I32P, I32P, C.c_int, I32P,
+                                             C.c_int, C.c_int, I64P, I32P, I32P, C.c_int]),
+            "or_max_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -270,3 +280,72 @@ def dvnd(W, tour, rowmap=0, max_rounds=0, nbs=5):
     m = np.zeros(1, np.int64)
     L = lib().or_dvnd_nbs(rowmap, Wf, n, t, len(t), max_rounds, nbs, r, m)
     return t, int(L), int(r[0]), int(m[0])
+
+
+# -- O(N)-memory forms over coordinates (the ADS in prefix form) ---------------
+def _xy(xs, ys, tour):
+    return (np.ascontiguousarray(xs, np.int32), np.ascontiguousarray(ys, np.int32),
+            np.ascontiguousarray(tour, np.int32))
+
+
+def delta_prefix(nb, xs, ys, rnd, tour, i, j) -> int:
+    """Move delta from the prefix form of the ADS and coordinate distances."""
+    xs, ys, t = _xy(xs, ys, tour)
+    return lib().or_delta_prefix(nb, xs, ys, rnd, t, len(t), i, j)
+
+
+def latency_xy(xs, ys, rnd, tour) -> int:
+    xs, ys, t = _xy(xs, ys, tour)
+    return lib().or_latency_xy(xs, ys, rnd, t, len(t))
+
+
+def eval_rows_xy(nb, rowmap, xs, ys, rnd, tour, threads=0):
+    """eval_rows over coordinates (rows split over threads)."""
+    xs, ys, t = _xy(xs, ys, tour)
+    N = len(t)
+    R = max(rows(nb, rowmap, N), 0)
+    cost = np.empty(R, np.int64)
+    i = np.empty(R, np.int32)
+    j = np.empty(R, np.int32)
+    ctx = np.empty(R, np.int32)
+    lib().or_eval_rows_xy(nb, rowmap, xs, ys, rnd, t, N, cost, i, j, ctx, threads)
+    return {"cost": cost, "i": i, "j": j, "ctx": ctx}
+
+
+def dvnd_xy(xs, ys, rnd, tour, rowmap=0, max_rounds=0, nbs=5, threads=0):
+    """dvnd over coordinates in O(N) memory: (final tour, latency, rounds, moves)."""
+    xs, ys, t = _xy(xs, ys, tour)
+    t = t.copy()
+    r = np.zeros(1, np.int32)
+    m = np.zeros(1, np.int64)
+    L = lib().or_dvnd_xy(rowmap, xs, ys, rnd, t, len(t), max_rounds, nbs, r, m, threads)
+    return t, int(L), int(r[0]), int(m[0])
+
+
+def eval_batch_xy(nb_mask, rowmap, xs, ys, rnd, tours, threads=0):
+    """Per-row bests of the neighborhoods in nb_mask for every tour (OpenMP
+    over tour x row).  Returns ({nb: {"cost", "i", "j"} arrays of shape
+    (tours, rows)}, moves evaluated)."""
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    flat = np.ascontiguousarray(np.stack([np.asarray(t, np.int32) for t in tours]))
+    count, N = flat.shape
+    nbs = [nb for nb in range(5) if (nb_mask >> nb) & 1]
+    R = {nb: max(rows(nb, rowmap, N), 0) for nb in nbs}
+    stride = sum(R.values())
+    cost = np.empty(count * stride, np.int64)
+    i = np.empty(count * stride, np.int32)
+    j = np.empty(count * stride, np.int32)
+    moves = lib().or_eval_batch_xy(nb_mask, rowmap, xs, ys, rnd, flat.ravel(), count, N, cost, i,
+                                   j, threads)
+    out, off = {}, 0
+    for nb in nbs:
+        sl = slice(off, off + R[nb])
+        out[nb] = {k: a.reshape(count, stride)[:, sl] for k, a in (("cost", cost), ("i", i),
+                                                                   ("j", j))}
+        off += R[nb]
+    return out, int(moves)
+
+
+def max_threads() -> int:
+    return int(lib().or_max_threads())

@@ -215,3 +215,36 @@ def test_best_neighbor_berlin52_swap_pins_367403(L):
     zero = (C.c_uint * 1)(0)
     assert L.bestNeighbor(BERLIN, sol, len(t), 0, False, 0, zero, None, None, None, None,
                           False, 1, None) == 367403
+
+
+GIL = os.path.join(GOLDEN, "instances", "09_gil262.tsp").encode()
+
+
+def test_just_calc_negative_coordinates(L, monkeypatch):
+    """gil262 (negative coordinates): the latency of a tour, from the raw
+    TSPLIB coordinates as the reference reads them."""
+    monkeypatch.setenv("MLP_DVND_STRICT", "0")  # problems keyed by hostCode
+    xs, ys = load_tsplib_coords("09_gil262.tsp")
+    W = O.weights(xs, ys, 0)
+    t = O.random_tour(O.MT(1), W, 0.5)
+    sol = (C.c_int * len(t))(*t.tolist())
+    assert L.bestNeighbor(GIL, sol, len(t), 0, True, 262, None, None, None, None, None,
+                          False, 1, None) == O.latency_u32(W, t)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nb", [0, 1, 2, 3, 4])
+def test_best_neighbor_negative_coordinates(L, nb, monkeypatch):
+    monkeypatch.setenv("MLP_DVND_STRICT", "0")
+    xs, ys = load_tsplib_coords("09_gil262.tsp")
+    W = O.weights(xs, ys, 0)
+    t = O.random_tour(O.MT(20 + nb), W, 0.5)
+    N = len(t)
+    t2, c, _ = O.step(nb, 1, W, t)
+    assert c > 0
+    sol = (C.c_int * N)(*t.tolist())
+    zero = (C.c_uint * 1)(0)
+    val = L.bestNeighbor(GIL, sol, N, nb, False, 262, zero, None, None, None, None, False, 1,
+                         None)
+    assert val == O.latency_u32(W, t2)
+    assert list(sol) == t2.tolist()

@@ -166,3 +166,33 @@ def test_standalone_eval_after_dvnd(eng):
     finally:
         eng.destroy_slot(s)
         eng.destroy_instance(inst)
+
+
+def test_cached_round_graph_follows_neighborhood_count(eng):
+    """A DVND chunk captured as a graph with one neighborhood set is not
+    replayed after the set changes: slots with buffers for 15 ids run DVND
+    over 10 neighborhoods until the chunk graph is in use, then over 5."""
+    eng.set_neighborhoods(15)
+    try:
+        xs, ys = O.instance_uniform(120, 44)
+        W = O.weights(xs, ys, 1)
+        tours = [O.random_tour(O.MT(440 + q), W, 0.5) for q in range(3)]
+        inst = eng.instance(xs, ys, 1)
+        S = [eng.slot(inst) for _ in tours]
+        eng.dvnd_config(1, True, False)  # one round per chunk: the graph is reused at once
+        for nbs in (10, 5):
+            eng.dvnd_neighborhoods(nbs)
+            eng.set_tours(S, tours)
+            rounds = eng.dvnd(S, 0)
+            for q, s in enumerate(S):
+                tref, Lref, rref, _ = O.dvnd(W, tours[q], 0, nbs=nbs)
+                assert int(eng.latency([s])[0]) == Lref, (nbs, q)
+                assert int(rounds[q]) == rref, (nbs, q)
+                assert np.array_equal(eng.get_tour(s), tref), (nbs, q)
+        for s in S:
+            eng.destroy_slot(s)
+        eng.destroy_instance(inst)
+    finally:
+        eng.dvnd_neighborhoods(5)
+        eng.set_neighborhoods(5)
+        eng.dvnd_config(4, True, False)

@@ -510,54 +510,95 @@ int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* n
   return MLP_OK;
 }
 
-static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
-                           int32_t round, int32_t* inst, bool async) {
-  if (!ctx || !inst) return MLP_EINVAL;
+// Instance validation shared by instance creation and mlp_check_job.  The
+// coordinates may be any int32 values whose span is at most 32767 in x and in
+// y: they are translated to [0, span] (distances depend on differences only,
+// so the reference's raw-coordinate distances, mlproblem.cpp:153-166, are
+// unchanged), which keeps dx^2 + dy^2 within int32 and the device's packed
+// 16-bit coordinates exact.  *maxd receives a bound of every distance.
+static int check_instance(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                          int32_t round, int32_t* xmin_out, int32_t* ymin_out, int* maxd) {
   if (!xs || !ys || n < 2) return fail(ctx, MLP_EINVAL, "need at least 2 clients");
   if (n + 1 > MLP_MAX_N) return fail(ctx, MLP_ERANGE, "n = %d exceeds %d", n, MLP_MAX_N - 1);
   if (round != 0 && round != 1) return fail(ctx, MLP_EINVAL, "round must be 0 or 1");
-  int xmin = INT32_MAX, xmax = INT32_MIN, ymin = INT32_MAX, ymax = INT32_MIN;
+  int64_t xmin = INT64_MAX, xmax = INT64_MIN, ymin = INT64_MAX, ymax = INT64_MIN;
   for (int c = 0; c < n; ++c) {
-    if (xs[c] < 0 || ys[c] < 0 || xs[c] > 32767 || ys[c] > 32767)
-      return fail(ctx, MLP_ERANGE, "coordinate of client %d outside [0, 32767]", c);
-    xmin = std::min(xmin, xs[c]);
-    xmax = std::max(xmax, xs[c]);
-    ymin = std::min(ymin, ys[c]);
-    ymax = std::max(ymax, ys[c]);
+    xmin = std::min<int64_t>(xmin, xs[c]);
+    xmax = std::max<int64_t>(xmax, xs[c]);
+    ymin = std::min<int64_t>(ymin, ys[c]);
+    ymax = std::max<int64_t>(ymax, ys[c]);
   }
-  Instance in;
-  in.n = n;
-  in.round = round;
-  const double dx = xmax - xmin, dy = ymax - ymin;
-  in.maxd = (int)std::ceil(std::sqrt(dx * dx + dy * dy)) + 1;
+  if (xmax - xmin > 32767 || ymax - ymin > 32767)
+    return fail(ctx, MLP_ERANGE, "coordinate span %lld x %lld exceeds 32767",
+                (long long)(xmax - xmin), (long long)(ymax - ymin));
+  const double dx = (double)(xmax - xmin), dy = (double)(ymax - ymin);
+  const int md = (int)std::ceil(std::sqrt(dx * dx + dy * dy)) + 1;
   // move keys hold delta * 2^20 + index in 64 bits: every |delta| is below
   // N (N + 32) maxd, which must stay below 2^43
   const uint64_t N = (uint64_t)n + 1;
-  if (N * (N + 32) * (uint64_t)in.maxd >= (1ull << 43))
+  if (N * (N + 32) * (uint64_t)md >= (1ull << 43))
     return fail(ctx, MLP_ERANGE, "n = %d with a coordinate span of %d exceeds the 64-bit move keys",
-                n, in.maxd);
+                n, md);
+  if (xmin_out) *xmin_out = (int32_t)xmin;
+  if (ymin_out) *ymin_out = (int32_t)ymin;
+  if (maxd) *maxd = md;
+  return MLP_OK;
+}
+
+// A tour of N positions [0, perm(1..N-2), 0].
+static int check_tour(mlp_ctx* ctx, const int32_t* t, int N, int q, std::vector<char>& seen) {
+  const int n = N - 1;
+  if (t[0] != 0 || t[N - 1] != 0)
+    return fail(ctx, MLP_EINVAL, "tour %d must start and end at the depot 0", q);
+  seen.assign((size_t)n, 0);
+  for (int p = 1; p < N - 1; ++p) {
+    if (t[p] <= 0 || t[p] >= n || seen[(size_t)t[p]])
+      return fail(ctx, MLP_EINVAL, "tour %d is not a permutation (position %d)", q, p);
+    seen[(size_t)t[p]] = 1;
+  }
+  return MLP_OK;
+}
+
+static int instance_create(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
+                           int32_t round, int32_t* inst, bool async) {
+  if (!ctx || !inst) return MLP_EINVAL;
+  Instance in;
+  int32_t x0 = 0, y0 = 0;
+  int st = check_instance(ctx, xs, ys, n, round, &x0, &y0, &in.maxd);
+  if (st) return st;
+  in.n = n;
+  in.round = round;
   HCK(hipSetDevice(ctx->device));
   const size_t half = align_up(4 * (size_t)n, 256);
   void* mem = nullptr;
-  int st = pool_alloc(ctx, 2 * half, &mem, &in.cls);
+  st = pool_alloc(ctx, 2 * half, &mem, &in.cls);
   if (st) return st;
   in.xs = (int32_t*)mem;
   in.ys = (int32_t*)((char*)mem + half);
+  // both coordinate arrays, translated to [0, span], in one copy
+  auto fill = [&](char* h) {
+    int32_t* hx = (int32_t*)h;
+    int32_t* hy = (int32_t*)(h + half);
+    for (int c = 0; c < n; ++c) {
+      hx[c] = xs[c] - x0;
+      hy[c] = ys[c] - y0;
+    }
+  };
   if (async) {
-    // both coordinate arrays in one copy from pinned staging, ordered on the
-    // stream (the caller's arrays may be released at once)
+    // from pinned staging, ordered on the stream (the caller's arrays may be
+    // released at once)
     char* h = nullptr;
     st = stage_alloc(ctx, 2 * half, &h);
     if (st) {
       pool_release(ctx, mem, in.cls);
       return st;
     }
-    memcpy(h, xs, 4 * (size_t)n);
-    memcpy(h + half, ys, 4 * (size_t)n);
+    fill(h);
     HCK(hipMemcpyAsync(mem, h, 2 * half, hipMemcpyHostToDevice, ctx->stream));
   } else {
-    HCK(hipMemcpy(in.xs, xs, 4 * (size_t)n, hipMemcpyHostToDevice));
-    HCK(hipMemcpy(in.ys, ys, 4 * (size_t)n, hipMemcpyHostToDevice));
+    std::vector<char> h(2 * half);
+    fill(h.data());
+    HCK(hipMemcpy(mem, h.data(), 2 * half, hipMemcpyHostToDevice));
   }
   in.alive = true;
   for (size_t q = 0; q < ctx->insts.size(); ++q)
@@ -581,6 +622,24 @@ int mlp_instance_create_async(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys
                               int32_t round, int32_t* inst) {
   NO_CAPTURE(ctx);
   return instance_create(ctx, xs, ys, n, round, inst, true);
+}
+
+int mlp_check_job(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n, int32_t round,
+                  const int32_t* tour) {
+  int st = check_instance(ctx, xs, ys, n, round, nullptr, nullptr, nullptr);
+  if (st || !tour) return st ? st : fail(ctx, MLP_EINVAL, "tour is NULL");
+  std::vector<char> seen;
+  return check_tour(ctx, tour, n + 1, 0, seen);
+}
+
+int mlp_mem_info(mlp_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!ctx) return MLP_EINVAL;
+  size_t fr = 0, tot = 0;
+  HCK(hipSetDevice(ctx->device));
+  HCK(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = (int64_t)fr;
+  if (total_bytes) *total_bytes = (int64_t)tot;
+  return MLP_OK;
 }
 
 int mlp_instance_destroy(mlp_ctx* ctx, int32_t inst) {
@@ -707,16 +766,9 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
   for (int q = 0; q < count; ++q) {
     const Slot& s = ctx->slots[slots[q]];
     const int32_t* t = tours + total;
-    const int N = s.N, n = N - 1;
-    if (t[0] != 0 || t[N - 1] != 0)
-      return fail(ctx, MLP_EINVAL, "tour %d must start and end at the depot 0", q);
-    seen.assign((size_t)n, 0);
-    for (int p = 1; p < N - 1; ++p) {
-      if (t[p] <= 0 || t[p] >= n || seen[(size_t)t[p]])
-        return fail(ctx, MLP_EINVAL, "tour %d is not a permutation (position %d)", q, p);
-      seen[(size_t)t[p]] = 1;
-    }
-    total += (size_t)N;
+    st = check_tour(ctx, t, s.N, q, seen);
+    if (st) return st;
+    total += (size_t)s.N;
   }
   // all tours in one copy from pinned staging, ordered on the stream (no
   // wait for queued kernels); several slots' tours land in a device array
@@ -889,6 +941,18 @@ int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask
   if (count <= 2) return MLP_TILE_4x256;
   if (count <= 16) return MLP_TILE_8x512;
   return MLP_TILE_16x1024;
+}
+
+// Forget the cached DVND chunk graph (its launches embed the neighborhood
+// set, tile choices and merge mode of the moment it was captured).
+int drop_dvnd_graph(mlp_ctx* ctx) {
+  if (ctx->graph_exec) {
+    if (!ctx->capturing) HCK(hipStreamSynchronize(ctx->stream));
+    HCK(hipGraphExecDestroy(ctx->graph_exec));
+    ctx->graph_exec = nullptr;
+  }
+  ctx->graph_key.clear();
+  return MLP_OK;
 }
 
 // A work counter cleared by a kernel (graph-capturable, unlike a memset).
@@ -1143,7 +1207,7 @@ int mlp_set_neighborhoods(mlp_ctx* ctx, int32_t count) {
   if (count < MLP_NB_COUNT || count > MLP_NB_MAX)
     return fail(ctx, MLP_EINVAL, "neighborhood count must be in [%d, %d]", MLP_NB_COUNT, MLP_NB_MAX);
   ctx->nbs = count;
-  return MLP_OK;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_dvnd_neighborhoods(mlp_ctx* ctx, int32_t count) {
@@ -1151,7 +1215,7 @@ int mlp_dvnd_neighborhoods(mlp_ctx* ctx, int32_t count) {
   if (count < 1 || count > MLP_NB_MAX)
     return fail(ctx, MLP_EINVAL, "neighborhood count must be in [1, %d]", MLP_NB_MAX);
   ctx->dvnd_nbs = count;
-  return MLP_OK;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_slot_bytes(mlp_ctx* ctx, int32_t n, int64_t* bytes) {
@@ -1164,7 +1228,7 @@ int mlp_slot_bytes(mlp_ctx* ctx, int32_t n, int64_t* bytes) {
 int mlp_merge_force_global(mlp_ctx* ctx, int32_t on) {
   if (!ctx) return MLP_EINVAL;
   ctx->merge_force_global = on != 0;
-  return MLP_OK;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_mask,
@@ -1208,7 +1272,7 @@ int mlp_autotune(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb_
   if (st) return st;
   ctx->tune[nb_mask][work_bucket(launch_moves(ctx, slots, count, nb_mask))] = (int8_t)bt;
   if (best_tile) *best_tile = bt;
-  return MLP_OK;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_tune_get(mlp_ctx* ctx, int8_t* table) {
@@ -1223,7 +1287,7 @@ int mlp_tune_set(mlp_ctx* ctx, const int8_t* table) {
     if (table[q] < -1 || table[q] >= MLP_TILE_COUNT)
       return fail(ctx, MLP_EINVAL, "tune entry %zu = %d is not a tile shape", q, (int)table[q]);
   memcpy(ctx->tune, table, sizeof ctx->tune);
-  return MLP_OK;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_set_tile(mlp_ctx* ctx, int32_t tile) {
@@ -1641,13 +1705,14 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
   int tile = ctx->tile_override;  // (the per-neighborhood shapes follow from the slots)
   // the captured launches embed the slot table address, every slot's id and
   // size (segment tables, merge capacities) and the settings
-  const int32_t hdr[6] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count};
+  const int32_t hdr[8] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count,
+                          ctx->dvnd_nbs, ctx->merge_force_global ? 1 : 0};
   std::string key((const char*)hdr, sizeof hdr);
   key.append((const char*)&ctx->d_table, sizeof ctx->d_table);
   key.append((const char*)slots, 4 * (size_t)count);
   for (int q = 0; q < count; ++q) {
-    const int32_t N = ctx->slots[slots[q]].N;
-    key.append((const char*)&N, 4);
+    const int32_t Nn[2] = {ctx->slots[slots[q]].N, ctx->slots[slots[q]].nbs};
+    key.append((const char*)Nn, sizeof Nn);
   }
   if (key == ctx->graph_key && ctx->graph_exec) {
     HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));

@@ -26,8 +26,9 @@ struct Job {
   double t_submit = 0, t_admit = 0, t_finish = 0;
   int32_t inst = -1, slot = -1;
   int64_t bytes = 0, latency = 0;
-  int32_t rounds = 0;
+  int32_t rounds = 0, status = MLP_OK;
   int lane = 0;
+  int bypassed = 0;  // later jobs admitted while this one waited for memory
 };
 
 // The pool rounds every block up to a power of two of at least 4 KiB.
@@ -75,7 +76,16 @@ int mlp_server_create(mlp_ctx* ctx, const mlp_server_config* cfg, mlp_server** o
   mlp_server* s = new (std::nothrow) mlp_server();
   if (!s) return MLP_ENOMEM;
   if (cfg) s->cfg = *cfg;
-  if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
+  if (s->cfg.hbm_budget <= 0) {
+    int64_t fr = 0;
+    const int st = mlp_mem_info(ctx, &fr, nullptr);
+    if (st) {
+      delete s;
+      return st;
+    }
+    s->cfg.hbm_budget = fr / 100 * 85;
+  }
+  if (s->cfg.max_bypass == 0) s->cfg.max_bypass = 64;
   if (s->cfg.rounds_per_step <= 0) s->cfg.rounds_per_step = 4;
   if (s->cfg.big_n == 0) s->cfg.big_n = 4096;
   if (s->cfg.rowmap != MLP_ROWMAP_ADS && s->cfg.rowmap != MLP_ROWMAP_ODC) {
@@ -116,6 +126,10 @@ int mlp_server_destroy(mlp_server* s) {
 int mlp_server_submit(mlp_server* s, const int32_t* xs, const int32_t* ys, int32_t n,
                       int32_t round, const int32_t* tour, int64_t tag, int64_t* job) {
   if (!s || !xs || !ys || !tour || n < 2) return MLP_EINVAL;
+  // the checks instance and slot set-up would apply, so that no queued job
+  // fails them at admission
+  const int chk = mlp_check_job(s->lane[0].ctx, xs, ys, n, round, tour);
+  if (chk) return chk;
   const int64_t bytes = mlp_server_job_bytes(n);
   if (bytes < 0) return MLP_ERANGE;
   if (bytes > s->cfg.hbm_budget) return MLP_ENOMEM;
@@ -203,14 +217,30 @@ int mlp_server_step(mlp_server* s, int32_t* active_out, int32_t* queued_out) {
       st = retire(s, s->lane[L]);
       if (st) return st;
     }
-  // 2. admission in arrival order while the footprint fits
+  // 2. admission in arrival order while the footprint fits; when the oldest
+  // queued job does not fit yet, later jobs that fit may pass it, at most
+  // max_bypass of them in all, so that it is not starved
   std::vector<Job*> admitted[2];
   const int total_active = (int)(s->lane[0].active.size() + s->lane[1].active.size());
   int added = 0;
+  const double now0 = mlp_server_now(s);
+  auto fail_job = [&](Job* j, int code) {  // retired with an error status
+    release(s, j);
+    j->status = code;
+    j->latency = 0;
+    j->t_finish = now0;
+    s->done.push_back(j);
+  };
+  Job* blocked = nullptr;  // oldest queued job whose footprint does not fit
   for (auto it = s->queued.begin(); it != s->queued.end();) {
     Job* j = *it;
     if (s->cfg.max_active > 0 && total_active + added >= s->cfg.max_active) break;
-    if (s->used + j->bytes > s->cfg.hbm_budget) break;  // memory is granted in arrival order
+    if (s->used + j->bytes > s->cfg.hbm_budget) {
+      if (!blocked) blocked = j;
+      if (s->cfg.max_bypass < 0 || blocked->bypassed >= s->cfg.max_bypass) break;
+      ++it;
+      continue;
+    }
     if (!ready[j->lane]) {  // its lane is still running a chunk: later jobs may go first
       ++it;
       continue;
@@ -229,11 +259,16 @@ int mlp_server_step(mlp_server* s, int32_t* active_out, int32_t* queued_out) {
     }
     st = mlp_instance_create_async(l.ctx, j->xs.data(), j->ys.data(), j->n, j->round, &j->inst);
     if (st == MLP_OK) st = mlp_slot_create(l.ctx, j->inst, &j->slot);
-    if (st) {
+    if (st == MLP_ENOMEM) {  // the device is full: the job waits for memory to be freed
       release(s, j);
-      return st;
+      break;
     }
     it = s->queued.erase(it);
+    if (st) {
+      fail_job(j, st);
+      continue;
+    }
+    if (blocked) ++blocked->bypassed;
     s->used += j->bytes;
     admitted[j->lane].push_back(j);
     ++added;
@@ -248,7 +283,13 @@ int mlp_server_step(mlp_server* s, int32_t* active_out, int32_t* queued_out) {
     }
     st = mlp_set_tours(l.ctx, slots.data(), (int32_t)slots.size(), tours.data());
     if (st == MLP_OK) st = mlp_dvnd_start(l.ctx, slots.data(), (int32_t)slots.size());
-    if (st) return st;
+    if (st) {  // none of these jobs started: retire them with the error
+      for (Job* j : admitted[L]) {
+        s->used -= j->bytes;
+        fail_job(j, st);
+      }
+      continue;
+    }
     const double now = mlp_server_now(s);
     for (Job* j : admitted[L]) {
       j->t_admit = now;
@@ -279,7 +320,7 @@ int mlp_server_poll(mlp_server* s, mlp_job_result* out, int32_t cap, int32_t* co
   for (int q = 0; q < take; ++q) {
     Job* j = s->done[(size_t)q];
     out[q] = mlp_job_result{j->id, j->tag, j->latency, j->rounds, j->n,
-                            j->t_submit, j->t_admit, j->t_finish, j->tour.data()};
+                            j->t_submit, j->t_admit, j->t_finish, j->tour.data(), j->status};
     s->handed.push_back(j);
   }
   s->done.erase(s->done.begin(), s->done.begin() + take);

@@ -33,14 +33,15 @@ class MoveRec(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("rowmap", C.c_int32),
-                ("rounds_per_step", C.c_int32), ("max_rounds", C.c_int32), ("big_n", C.c_int32)]
+                ("rounds_per_step", C.c_int32), ("max_rounds", C.c_int32), ("big_n", C.c_int32),
+                ("max_bypass", C.c_int32)]
 
 
 class JobResult(C.Structure):
     _fields_ = [("job", C.c_int64), ("tag", C.c_int64), ("latency", C.c_int64),
                 ("rounds", C.c_int32), ("n", C.c_int32), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_finish", C.c_double),
-                ("tour", C.POINTER(C.c_int32))]
+                ("tour", C.POINTER(C.c_int32)), ("status", C.c_int32)]
 
 
 _lib = None
@@ -61,6 +62,8 @@ _SIGS = {
     "mlp_instance_create_async": (C.c_int, [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32,
                                             C.POINTER(C.c_int32)]),
     "mlp_instance_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_check_job": (C.c_int, [C.c_void_p, I32P, I32P, C.c_int32, C.c_int32, I32P]),
+    "mlp_mem_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlp_pool_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlp_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "mlp_server_destroy": (C.c_int, [C.c_void_p]),

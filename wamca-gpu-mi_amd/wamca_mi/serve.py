@@ -21,12 +21,14 @@ from .engine import ROWMAP_ADS
 
 class Server:
     def __init__(self, eng, hbm_budget=0, max_active=0, rowmap=ROWMAP_ADS, rounds_per_step=4,
-                 max_rounds=0, big_n=0):
-        """big_n: jobs of n >= big_n run in a second lane (0: 4096, < 0: one lane)."""
+                 max_rounds=0, big_n=0, max_bypass=0):
+        """big_n: jobs of n >= big_n run in a second lane (0: 4096, < 0: one lane);
+        max_bypass: later jobs that may pass a queued job that does not fit
+        the free budget yet (0: 64, < 0: strict arrival order)."""
         self.eng = eng
         self._L = eng._L
         cfg = ServerConfig(int(hbm_budget), int(max_active), int(rowmap), int(rounds_per_step),
-                           int(max_rounds), int(big_n))
+                           int(max_rounds), int(big_n), int(max_bypass))
         h = C.c_void_p()
         eng._ck(self._L.mlp_server_create(eng.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -69,7 +71,7 @@ class Server:
             tour = np.ctypeslib.as_array(r.tour, shape=(r.n + 1,)).copy()
             out.append({"job": r.job, "tag": r.tag, "latency": r.latency, "rounds": r.rounds,
                         "n": r.n, "t_submit": r.t_submit, "t_admit": r.t_admit,
-                        "t_finish": r.t_finish, "tour": tour})
+                        "t_finish": r.t_finish, "tour": tour, "status": r.status})
         return out
 
     @staticmethod

@@ -31,22 +31,30 @@ def dvnd_jobs(count: int, seed: int, lo: int = 50, hi: int = 2000, big_frac: flo
         iseed = seed * 100003 + q
         if big:
             n = big_n
+            xs, ys, tour = big_job(iseed, n, rnd)
+            jobs.append((xs, ys, tour))
+            meta.append({"n": n, "kind": "clustered", "seed": iseed})
+            continue
+        n = I.loguniform_n(seed, q, lo, hi)
+        if q % 2 == 0:
+            xs, ys = I.uniform(n, iseed, 1000)
+            kind = "uniform"
+        else:
             xs, ys = I.clustered(n, iseed)
             kind = "clustered"
-        else:
-            n = I.loguniform_n(seed, q, lo, hi)
-            if q % 2 == 0:
-                xs, ys = I.uniform(n, iseed, 1000)
-                kind = "uniform"
-            else:
-                xs, ys = I.clustered(n, iseed)
-                kind = "clustered"
         # the reference's construction sorts per step (O(n^2 log n)); large
         # jobs use the selection-based variant
         tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, rnd, iseed)
         jobs.append((xs, ys, tour))
         meta.append({"n": n, "kind": kind, "seed": iseed})
     return jobs, meta
+
+
+def big_job(iseed: int, n: int = 18000, rnd: int = 1):
+    """A large stream job: clustered instance of seed iseed and the
+    selection-based random-insertion start tour of the same seed."""
+    xs, ys = I.clustered(n, iseed)
+    return xs, ys, I.random_tour_fast(xs, ys, rnd, iseed)
 
 
 def poisson_arrivals(rate: float, count: int, seed: int):
