@@ -305,6 +305,35 @@ def valu_record(tile, moves_per_s):
             "unit": "T lane-instructions/s", "frac": ach / PEAK_VALU_TLOPS}
 
 
+def probe_masks(eng, n, tours, tiles, masks):
+    """Per-launch timing by tile shape and neighborhood mask (for counter and
+    trace runs): one uniform instance (nint) with `tours` random-permutation
+    tours; for every (tile, mask) one launch over all tours ("batched") and one
+    over the first tour ("single"), each bracketed by HIP events on the
+    engine's stream.  One JSON line per launch, in launch order, so counter or
+    trace output of the same command pairs with them."""
+    from wamca_mi import inputs as I
+    xs, ys = I.uniform(n, 1, 1000)
+    inst = eng.instance(xs, ys, 1)
+    slots = eng.slots(inst, tours)
+    eng.set_tours(slots, [I.perm_tour(s, n) for s in range(1, tours + 1)])
+    eng.sync()
+    mp = moves_per_nb(n + 1)
+    for tile in tiles:
+        eng.set_tile(tile)
+        for mask in masks:
+            for mode, sl in (("batched", slots), ("single", slots[:1])):
+                eng.sync()
+                eng.region_begin()
+                eng.eval(sl, mask, 0)
+                us = eng.region_end() * 1e3
+                mv = sum(v for nb, v in mp.items() if (mask >> nb) & 1) * len(sl)
+                print(json.dumps({"mode": mode, "tile": tile, "kernel": TILES[tile], "mask": mask,
+                                  "tours": len(sl), "moves": mv, "us": round(us, 2),
+                                  "gmoves_per_s": round(mv / us / 1e3, 1)}), flush=True)
+    eng.set_tile(-1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,6 +355,11 @@ def main():
                     help="fraction of stream jobs replaced by clustered n = 18000")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run device copies of this size (counter calibration)")
+    ap.add_argument("--probe-masks", action="store_true",
+                    help="only time one launch per (tile, mask) over --tours tours and over one "
+                         "tour (per-launch lines for counter and trace runs)")
+    ap.add_argument("--probe-tiles", default="2,4")
+    ap.add_argument("--probe-mask-list", default="31,29,2,1,4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -338,6 +372,12 @@ def main():
     if args.copy_bytes:
         eng.copy_probe(args.copy_bytes, 4)
         print(json.dumps({"copy_bytes": args.copy_bytes, "reps": 4}))
+        eng.close()
+        return
+
+    if args.probe_masks:
+        probe_masks(eng, args.n, args.tours, [int(t) for t in args.probe_tiles.split(",")],
+                    [int(m) for m in args.probe_mask_list.split(",")])
         eng.close()
         return
 

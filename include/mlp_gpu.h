@@ -203,6 +203,39 @@ int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* 
  * (default on), persistent evaluation grids (default off). */
 int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t persistent);
 
+/* Device buffers and copies for callers without HIP headers (the C++
+ * surface's MLKernel::moveData, mlkernel.h:128-130, allocated in
+ * MLKernel::init, mlkernel.cu:551-556).  mlp_dev_alloc returns device memory
+ * of the context's GPU, released by mlp_dev_free or with the context;
+ * mlp_dev_memset is enqueued on the context stream; the copies are ordered
+ * on it and return when done. */
+int mlp_dev_alloc(mlp_ctx* ctx, int64_t bytes, void** dev);
+int mlp_dev_free(mlp_ctx* ctx, void* dev);
+int mlp_dev_memset(mlp_ctx* ctx, void* dev, int32_t value, int64_t bytes);
+int mlp_copy_to_host(mlp_ctx* ctx, void* host, const void* dev, int64_t bytes);
+int mlp_copy_to_device(mlp_ctx* ctx, void* dev, const void* host, int64_t bytes);
+
+/* The rows of the slot's last evaluation of nb, as MLMove64 words written by
+ * the device into the device array dev (cap entries; *rows receives the row
+ * count): the per-row records the reference's kernels leave in moveData
+ * (mlkswap.cu:298-331, mlk2opt.cu:230-266, mlkoropt.cu:313-346), the cost as
+ * the reference's int32 and COST_INFTY with move word 0 for a row without a
+ * move.  Enqueued on the context stream.  MLP_ERANGE when N > 16384. */
+int mlp_rows_moves64_device(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* dev, int64_t cap,
+                            int32_t* rows);
+/* The same for the entries of the slot's last mlp_eval_total. */
+int mlp_total_moves64_device(mlp_ctx* ctx, int32_t slot, uint64_t* dev, int64_t cap,
+                             int64_t* entries);
+
+/* Timers of HIP event pairs on the context stream (MLKernel::timerStart /
+ * timerStop / timerElapsed, mlkernel.h:263-276).  elapsed waits for the stop
+ * event. */
+int mlp_timer_create(mlp_ctx* ctx, int32_t* timer);
+int mlp_timer_start(mlp_ctx* ctx, int32_t timer);
+int mlp_timer_stop(mlp_ctx* ctx, int32_t timer);
+int mlp_timer_elapsed(mlp_ctx* ctx, int32_t timer, float* ms);
+int mlp_timer_destroy(mlp_ctx* ctx, int32_t timer);
+
 /* The reference's ADS buffer (mlads.h:38-74) for the slot's current tour, built
  * on the device: 128-byte header, coords, [client|weight], T, C[N][N] (uint32
  * with the reference's wrap-around).  bytes = 128 + 4*N*(N+3). */

@@ -16,7 +16,9 @@
 //   WAMCAExperiment::runKernel / runWAMCA2016             WamcaExperiment.hpp:292-581
 //
 // Differences: device work goes through the C ABI (the ADS is built on the
-// device from the tour; sendSolution uploads the tour only); latencies are
+// device from the tour; sendSolution uploads the tour only; moveData is
+// device memory of the library's context, cleared with mlp_dev_memset or
+// hipMemset); latencies are
 // kept as uint32 like the reference (cost64 holds the exact value); errors
 // throw wamca::Error instead of exiting.  TOTAL kernels (isTotal = true)
 // use `totalBlock` threads per row: the reference takes the block size from
@@ -82,6 +84,25 @@ struct MLMove {
   int cost;
 };
 static_assert(sizeof(MLMove64) == 8, "MLMove64 is 8 bytes");
+
+// mlads.h:221-237: typed views of one buffer address
+union MLPointer {
+  const void* p_cvoid;
+  void* p_void;
+  char* p_char;
+  unsigned char* p_byte;
+  short* p_short;
+  ushort* p_ushort;
+  int* p_int;
+  uint* p_uint;
+  long* p_long;
+  unsigned long* p_ulong;
+  long long* p_llong;
+  unsigned long long* p_ullong;
+  MLMove* p_move;
+  MLMove64* p_move64;
+  MLMovePack* p_mpack;
+};
 
 extern const char* nameMove[15];  // mlads.cpp:41-72, ids 0..14
 
@@ -186,7 +207,10 @@ class MLKernel {
 
   void init(bool solCreate);
   void term();
-  void resetStats() { callCount = mergeCount = imprvCount = 0; }
+  void resetStats() {
+    callCount = mergeCount = imprvCount = 0;
+    timeMove = 0;
+  }
   void setSolution(MLSolution* sol, bool ads = true) { solution->assign(sol, ads); }
   void getSolution(MLSolution* sol, bool ads = true) { sol->assign(solution, ads); }
   void sendSolution();
@@ -198,6 +222,12 @@ class MLKernel {
   int bestMove(MLMove& move);
   int mergeGreedy(MLMove64* merge, int& count);
   virtual void applyMove(MLMove& move) = 0;
+  // HIP events on the problem's stream around the calls in between
+  // (mlkernel.h:263-276); timerElapsed waits for the stop event and returns
+  // microseconds
+  void timerStart();
+  void timerStop();
+  unsigned long long timerElapsed();
 
   MLProblem& problem;
   int id;
@@ -207,17 +237,34 @@ class MLKernel {
   int rowmap = MLP_ROWMAP_ODC;  // the shipped build's kernels; MLP_ROWMAP_ADS for the logged ones
                                 // (TOTAL kernels always use the ADS row map)
   uint totalBlock = 0;          // TOTAL: threads (entries) per row; 0 = default, set before init
-  uint moveElems = 0;
-  std::vector<MLMove64> moves;      // per-row (TOTAL: per-entry) records (recvResult) or
-                                    // merged list (mergeGPU)
-  MLMove64* transBuffer = nullptr;  // = moves.data()
+  unsigned long long time = 0;  // last timerElapsed(), microseconds
+
+  // Sizes and buffers of mlkernel.h:117-136 (set by init, mlkernel.cu:442-556).
+  uint solSize = 0;      // positions of a tour (problem.size)
+  uint adsDataSize = 0;  // bytes of the reference's ADS buffer for solSize: 128 + 4 N (N + 3);
+                         // the device keeps the ADS in O(N) prefix form instead
+                         // (mlp_ads_materialize builds this layout on request)
+  uint adsRowElems = 0;  // = solSize
+  // Device buffer of MLMove64 records (N entries; TOTAL: N * N): launchKernel
+  // leaves the per-row (per-entry) records in it, mergeGPU the merged list
+  // (kept moves first in merge order, every other entry with cost 0);
+  // recvResult copies moveElems entries to transBuffer.  Callers may clear it
+  // (mlp_dev_memset) before a launch, as the reference's callers do.
+  MLMovePack* moveData = nullptr;
+  uint moveDataSize = 0;  // bytes
+  uint moveElems = 0;     // records of the last launch (rows, or rows * totalBlock)
+  MLPointer transBuffer{};  // host copy of moveData (transBufferSize bytes)
+  uint transBufferSize = 0;
   MLSolution* solution = nullptr;
   bool solDestroy = false;
   uint callCount = 0, mergeCount = 0, imprvCount = 0;
+  unsigned long long timeMove = 0;
 
  protected:
+  void reserveMoves(size_t entries);
+  std::vector<MLMove64> host_;  // storage behind transBuffer
   int32_t slot_ = -1;
-  bool merged_ = false;
+  int32_t timer_ = -1;
 };
 
 class MLKernelSwap : public MLKernel {

@@ -18,9 +18,16 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "wamca/mlkernel.hpp"
 
 using namespace wamca;
+
+// The reference's callers reach the device through gpu* macros (gpu.h:51-134);
+// here they map to the HIP runtime the library runs on.
+#define gpuMemset hipMemset
+#define gpuDeviceSynchronize hipDeviceSynchronize
 
 static int g_fail = 0, g_checks = 0;
 #define EXPECT_EQ(a, b)                                                                     \
@@ -139,14 +146,42 @@ static void test_wamca_init_kernel() {
   EXPECT_EQ(solDevice->costCalc(), 494309);
   kernel->defineKernelGrid();
   EXPECT_EQ(kernel->moveElems, 26);  // ceil(53/2) - 1 ODC swap rows
+  // the members the reference's callers use (WamcaExperiment.hpp:238,339,
+  // tests/WamcaExperiment.Test.cpp:18,115): sizes, the device move buffer
+  // cleared before the launch, the pinned-style host view, the event timers
+  EXPECT_EQ(kernel->solSize, 53);
+  EXPECT_EQ(kernel->adsRowElems, 53);
+  EXPECT_EQ(kernel->adsDataSize, 128 + 4 * 53 * 56);
+  EXPECT_EQ(kernel->moveDataSize, 53 * 8);
+  EXPECT_EQ(kernel->transBufferSize, kernel->moveDataSize);
+  EXPECT_EQ(gpuMemset(kernel->moveData, 0, kernel->moveDataSize), hipSuccess);
+  EXPECT_EQ(gpuDeviceSynchronize(), hipSuccess);
+  MLMove64* h_moves = kernel->transBuffer.p_move64;
+  kernel->timerStart();
   kernel->launchKernel();
+  kernel->timerStop();
   kernel->sync();
+  EXPECT_EQ(kernel->timerElapsed() == kernel->time, 1);
+  // the per-row records the launch left in moveData
+  std::vector<MLMove64> dev_rows(kernel->moveElems);
+  EXPECT_EQ(hipMemcpy(dev_rows.data(), kernel->moveData, kernel->moveElems * 8,
+                      hipMemcpyDeviceToHost), hipSuccess);
+  kernel->recvResult();
+  int best_row = 0;
+  for (uint it = 0; it < kernel->moveElems; ++it) {
+    EXPECT_EQ(h_moves[it].cost, dev_rows[it].cost);
+    EXPECT_EQ(h_moves[it].i * 100000 + h_moves[it].j, dev_rows[it].i * 100000 + dev_rows[it].j);
+    if (h_moves[it].cost < h_moves[best_row].cost) best_row = (int)it;
+  }
+  MLMove bm;
+  EXPECT_EQ(kernel->bestMove(bm), h_moves[best_row].cost);
   kernel->mergeGPU();
   kernel->recvResult();
   kernel->sync();
+  EXPECT_EQ(kernel->transBuffer.p_move64, h_moves);
   int impr = 0, count = 0;
   for (uint it = 0; it < kernel->moveElems; ++it) {
-    const MLMove64& m = kernel->transBuffer[it];
+    const MLMove64& m = kernel->transBuffer.p_move64[it];
     if (m.cost < 0) {
       MLMove mv{(MLMoveId)m.id, (int)m.i, (int)m.j, m.cost};
       impr += m.cost;
@@ -166,8 +201,8 @@ static void test_wamca_init_kernel() {
   kernel->sync();
   kernel->recvResult();
   EXPECT_EQ(kernel->moveElems, 26);
-  int mn = kernel->transBuffer[0].cost;
-  for (uint it = 1; it < kernel->moveElems; ++it) mn = std::min(mn, kernel->transBuffer[it].cost);
+  int mn = kernel->transBuffer.p_move64[0].cost;
+  for (uint it = 1; it < kernel->moveElems; ++it) mn = std::min(mn, kernel->transBuffer.p_move64[it].cost);
   MLMove best;
   EXPECT_EQ(kernel->bestMove(best), mn);
   EXPECT_EQ(best.cost, mn);
@@ -230,7 +265,7 @@ static int run_total(const char* tsp, int seed, int sols, int argc, char** argv)
       MLMove best;
       k->bestMove(best);
       int imp = 0;
-      for (uint e = 0; e < k->moveElems; ++e) imp += k->transBuffer[e].cost < 0;
+      for (uint e = 0; e < k->moveElems; ++e) imp += k->transBuffer.p_move64[e].cost < 0;
       const uint elems = k->moveElems;
       std::vector<MLMove64> merged(elems);
       int ccount = 0;
@@ -238,8 +273,8 @@ static int run_total(const char* tsp, int seed, int sols, int argc, char** argv)
       k->mergeGPU();
       long long gsum = 0;
       int gcount = 0;
-      for (uint e = 0; e < k->moveElems && k->transBuffer[e].cost < 0; ++e) {
-        gsum += k->transBuffer[e].cost;
+      for (uint e = 0; e < k->moveElems && k->transBuffer.p_move64[e].cost < 0; ++e) {
+        gsum += k->transBuffer.p_move64[e].cost;
         ++gcount;
       }
       std::printf("%d %s elems %u best %d %d %d improving %d gpu %lld %d cpu %d %d\n", q, k->name,

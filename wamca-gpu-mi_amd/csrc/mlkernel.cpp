@@ -276,21 +276,57 @@ void MLKernel::init(bool solCreate) {
   // or-opt-k for k > 3 needs device buffers for ids 5..14 (slots created from now on)
   if (id >= 5) check(problem.ctx(), mlp_set_neighborhoods(problem.ctx(), 15), "mlp_set_neighborhoods");
   check(problem.ctx(), mlp_slot_create(problem.ctx(), problem.instance(), &slot_), "mlp_slot_create");
+  // sizes of mlkernel.cu:442-556
+  solSize = problem.size;
+  adsRowElems = solSize;
+  adsDataSize = 128u + (solSize + 3u) * adsRowElems * 4u;
+  reserveMoves(isTotal ? (size_t)solSize * solSize : (size_t)solSize);
+  check(problem.ctx(), mlp_timer_create(problem.ctx(), &timer_), "mlp_timer_create");
   defineKernelGrid();
+}
+
+// moveData and transBuffer hold at least `entries` records.
+void MLKernel::reserveMoves(size_t entries) {
+  entries = std::max<size_t>(entries, 1);
+  if (moveData && entries * 8 <= moveDataSize) return;
+  if (entries * 8 > 0xffffffffull) throw Error("move buffer above 4 GiB");
+  void* dev = nullptr;
+  check(problem.ctx(), mlp_dev_alloc(problem.ctx(), (int64_t)entries * 8, &dev), "moveData");
+  if (moveData) mlp_dev_free(problem.ctx(), moveData);
+  moveData = (MLMovePack*)dev;
+  moveDataSize = (uint)(entries * 8);
+  host_.assign(entries, MLMove64{0, 0, 0, 0});
+  transBuffer.p_move64 = host_.data();
+  transBufferSize = moveDataSize;
 }
 
 void MLKernel::term() {
   if (slot_ >= 0) mlp_slot_destroy(problem.ctx(), slot_);
   slot_ = -1;
+  if (moveData) mlp_dev_free(problem.ctx(), moveData);
+  moveData = nullptr;
+  moveDataSize = transBufferSize = 0;
+  if (timer_ >= 0) mlp_timer_destroy(problem.ctx(), timer_);
+  timer_ = -1;
   if (solDestroy && solution) delete solution;
   solution = nullptr;
+}
+
+void MLKernel::timerStart() { check(problem.ctx(), mlp_timer_start(problem.ctx(), timer_), "timerStart"); }
+
+void MLKernel::timerStop() { check(problem.ctx(), mlp_timer_stop(problem.ctx(), timer_), "timerStop"); }
+
+unsigned long long MLKernel::timerElapsed() {
+  float ms = 0.0F;
+  check(problem.ctx(), mlp_timer_elapsed(problem.ctx(), timer_, &ms), "timerElapsed");
+  time = (unsigned long long)(ms * 1000.0F);
+  return time;
 }
 
 void MLKernel::sendSolution() {
   if (solution->clientCount != problem.size) throw Error("solution is not a full tour");
   std::vector<int32_t> t(solution->clients.begin(), solution->clients.begin() + problem.size);
   check(problem.ctx(), mlp_set_tours(problem.ctx(), &slot_, 1, t.data()), "sendSolution");
-  merged_ = false;
 }
 
 void MLKernel::defineKernelGrid() {
@@ -301,89 +337,71 @@ void MLKernel::defineKernelGrid() {
   } else {
     moveElems = (uint)std::max(0, rows_of(id, rowmap, (int)problem.size));
   }
-  moves.assign(std::max<uint>(moveElems, 1), MLMove64{0, 0, 0, 0});
-  transBuffer = moves.data();
+  reserveMoves(moveElems);
 }
 
+// The evaluation, then its records written into moveData by the device.
 void MLKernel::launchKernel() {
   ++callCount;
-  merged_ = false;
-  if (isTotal)
-    check(problem.ctx(), mlp_eval_total(problem.ctx(), &slot_, 1, id, (int32_t)totalBlock),
+  mlp_ctx* c = problem.ctx();
+  if (isTotal) {
+    check(c, mlp_eval_total(c, &slot_, 1, id, (int32_t)totalBlock), "launchKernel");
+    int64_t M = 0;
+    reserveMoves((size_t)std::max(0, rows_of(id, MLP_ROWMAP_ADS, (int)problem.size)) * totalBlock);
+    check(c, mlp_total_moves64_device(c, slot_, (uint64_t*)moveData, moveDataSize / 8, &M),
           "launchKernel");
-  else
-    check(problem.ctx(), mlp_eval(problem.ctx(), &slot_, 1, 1u << id, rowmap, 0), "launchKernel");
+    moveElems = (uint)M;
+  } else {
+    check(c, mlp_eval(c, &slot_, 1, 1u << id, rowmap, 0), "launchKernel");
+    int32_t R = 0;
+    check(c, mlp_rows_moves64_device(c, slot_, id, (uint64_t*)moveData, moveDataSize / 8, &R),
+          "launchKernel");
+    moveElems = (uint)R;
+  }
 }
 
 void MLKernel::sync() { check(problem.ctx(), mlp_sync(problem.ctx()), "sync"); }
 
+// mlkernel.h:240-244: moveElems records from moveData to transBuffer.
 void MLKernel::recvResult() {
-  if (merged_) return;  // mergeGPU already left the merged list in transBuffer
-  if (isTotal) {  // entries in the 64-bit wire format; COST_INFTY where empty (mlads.h:24)
-    if (problem.size > 16384) throw Error("recvResult: N > 16384 (14-bit i/j of MLMove64)");
-    int64_t M = 0;
-    check(problem.ctx(), mlp_get_total(problem.ctx(), slot_, nullptr, nullptr, nullptr, 0, &M),
-          "recvResult");
-    std::vector<int64_t> c((size_t)M);
-    std::vector<int32_t> ii((size_t)M), jj((size_t)M);
-    check(problem.ctx(),
-          mlp_get_total(problem.ctx(), slot_, c.data(), ii.data(), jj.data(), M, &M), "recvResult");
-    moveElems = (uint)M;
-    moves.assign(std::max<uint>(moveElems, 1), MLMove64{(unsigned)id, 0, 0, 0x7fffffff});
-    for (int64_t e = 0; e < M; ++e) {
-      const int64_t v = c[(size_t)e];
-      if (v == INT64_MAX) continue;
-      if (v < INT32_MIN || v > INT32_MAX) throw Error("recvResult: cost outside int32");
-      moves[(size_t)e] = MLMove64{(unsigned)id, (unsigned)ii[(size_t)e] & 0x3fffu,
-                                  (unsigned)jj[(size_t)e] & 0x3fffu, (int)v};
-    }
-    transBuffer = moves.data();
-    return;
-  }
-  std::vector<uint64_t> w(MLP_MAX_N);
-  int32_t rows = 0;
-  check(problem.ctx(), mlp_get_moves64(problem.ctx(), slot_, id, w.data(), &rows), "recvResult");
-  moveElems = (uint)rows;
-  moves.resize(std::max<uint>(moveElems, 1));
-  for (int r = 0; r < rows; ++r) {
-    MLMovePack pk;
-    pk.w = w[(size_t)r];
-    moves[(size_t)r] = pk.s;
-  }
-  transBuffer = moves.data();
+  check(problem.ctx(),
+        mlp_copy_to_host(problem.ctx(), transBuffer.p_void, moveData, (int64_t)moveElems * 8),
+        "recvResult");
 }
 
-// Device merge: the moves kept by the greedy over the stable cost order come
-// first, in that order, with their costs; every other entry carries cost 0, so
-// a caller that applies the entries with cost < 0 in order applies exactly the
-// kept moves (WamcaExperiment.hpp:229-290).
+// Device merge and apply; moveData then holds the merged list in the
+// reference's post-merge form (mlkernel.cu:254-384 sorts moveData by cost and
+// zeroes the cost of every move the greedy rejects): the kept moves first, in
+// merge order, with their costs, every other entry with cost 0, so a caller
+// that applies the entries with cost < 0 in order applies exactly the kept
+// moves (WamcaExperiment.hpp:229-290).
 void MLKernel::mergeGPU() {
+  mlp_ctx* c = problem.ctx();
   if (isTotal)
-    check(problem.ctx(), mlp_merge_total(problem.ctx(), &slot_, 1), "mergeGPU");
+    check(c, mlp_merge_total(c, &slot_, 1), "mergeGPU");
   else
-    check(problem.ctx(), mlp_merge_apply(problem.ctx(), &slot_, 1, id), "mergeGPU");
+    check(c, mlp_merge_apply(c, &slot_, 1, id), "mergeGPU");
   std::vector<mlp_move> mv(std::max<uint>(moveElems, 1));
   int32_t count = 0;
-  check(problem.ctx(), mlp_get_applied(problem.ctx(), slot_, id, mv.data(), (int32_t)mv.size(), &count),
-        "mergeGPU");
-  moves.assign(std::max<uint>(moveElems, 1), MLMove64{(unsigned)id, 0, 0, 0});
+  check(c, mlp_get_applied(c, slot_, id, mv.data(), (int32_t)mv.size(), &count), "mergeGPU");
+  std::vector<MLMove64> list(std::max<uint>(moveElems, 1), MLMove64{(unsigned)id, 0, 0, 0});
   for (int q = 0; q < count; ++q)
-    moves[(size_t)q] = MLMove64{(unsigned)id, (unsigned)mv[(size_t)q].i & 0x3fffu,
-                                (unsigned)mv[(size_t)q].j & 0x3fffu, (int)mv[(size_t)q].cost};
-  transBuffer = moves.data();
-  merged_ = true;
+    list[(size_t)q] = MLMove64{(unsigned)id, (unsigned)mv[(size_t)q].i & 0x3fffu,
+                               (unsigned)mv[(size_t)q].j & 0x3fffu, (int)mv[(size_t)q].cost};
+  check(c, mlp_copy_to_device(c, moveData, list.data(), (int64_t)moveElems * 8), "mergeGPU");
   ++mergeCount;
 }
 
 // mlkernel.cu:617-647: first minimum of the record list.
 int MLKernel::bestMove(MLMove& move) {
   uint b = 0;
+  const MLMove64* t = transBuffer.p_move64;
   for (uint i = 1; i < moveElems; ++i)
-    if (transBuffer[i].cost < transBuffer[b].cost) b = i;
-  move.id = (MLMoveId)transBuffer[b].id;
-  move.i = transBuffer[b].i;
-  move.j = transBuffer[b].j;
-  move.cost = transBuffer[b].cost;
+    if (t[i].cost < t[b].cost) b = i;
+  move.id = (MLMoveId)t[b].id;
+  move.i = t[b].i;
+  move.j = t[b].j;
+  move.cost = t[b].cost;
   return move.cost;
 }
 
@@ -410,7 +428,7 @@ bool can_merge(const MLMove64& m1, const MLMove64& m2) {
 // 120000 improving moves (mlkernel.cu:778-787); the same limit applies here,
 // so this merge can differ from mergeGPU exactly where the reference's does.
 int MLKernel::mergeGreedy(MLMove64* merge, int& count) {
-  MLMove64* mv = transBuffer;
+  MLMove64* mv = transBuffer.p_move64;
   int n = (int)moveElems;
   std::sort(mv, mv + n, [](const MLMove64& a, const MLMove64& b) { return a.cost < b.cost; });
   if (isTotal) n = std::min(n, 120000);
@@ -515,7 +533,7 @@ void WAMCAExperiment::runKernel(int k, std::vector<MLMove>* moves) {
   kernel->recvResult();
   kernel->sync();
   for (uint it = 0; it < kernel->moveElems; ++it) {
-    const MLMove64& m = kernel->transBuffer[it];
+    const MLMove64& m = kernel->transBuffer.p_move64[it];
     if (m.cost < 0) {
       MLMove mv{(MLMoveId)m.id, (int)m.i, (int)m.j, m.cost};
       kernel->applyMove(mv);

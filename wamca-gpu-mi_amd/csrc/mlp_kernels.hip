@@ -384,7 +384,9 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
     int xy[kPrepPer + 2];
 #pragma unroll
     for (int e = 0; e < kPrepPer + 2; ++e) {
-      const int c = tour[min(max(p0 - 1 + e, 0), N - 1)];
+      // client ids clamped into the instance: a tour read from caller memory
+      // never indexes outside the coordinate arrays
+      const int c = min(max(tour[min(max(p0 - 1 + e, 0), N - 1)], 0), s.n - 1);
       xy[e] = xyc ? xyc[c] : (ldg(s.xs + c) << 16) | ldg(s.ys + c);
     }
     int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
@@ -1194,6 +1196,37 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
     s.info[MLP_INFO_ROUND_MOVES] += K;
     s.info[MLP_INFO_TOTAL_MOVES] += K;
   }
+}
+
+// MLMove64 words (mlads.h:190-206) of per-row keys: low word id:4 | i:14 |
+// j:14, high word the cost as the reference's int32 (low 32 bits of the
+// delta); a row without a move is COST_INFTY with move word 0
+// (mlkswap.cu:175-176).
+__device__ __forceinline__ uint64_t move64_of(int64_t key, int nb, int rowmap, int N, int row) {
+  if (key == MLP_KEY_NONE) return (uint64_t)0x7fffffffu << 32;
+  int i, j;
+  move_ij(nb, rowmap, N, row, (int)(key & MLP_KEY_MASK), i, j);
+  const uint32_t lo =
+      ((uint32_t)(j & 0x3fff) << 18) | ((uint32_t)(i & 0x3fff) << 4) | (uint32_t)(nb & 15);
+  return ((uint64_t)(uint32_t)(int32_t)(key >> MLP_KEY_SHIFT) << 32) | lo;
+}
+
+// Rows of the last evaluation of nb (the buffer info[PARITY0 + nb] names).
+__global__ void k_rows_moves64(const SlotDev* __restrict__ table, int id, int nb, int rowmap,
+                               int R, uint64_t* out) {
+  const SlotDev& s = table[id];
+  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1);
+  const int64_t* keys = s.rows[nb] + (par ? s.rowcap : 0);
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < R; r += gridDim.x * kBlock)
+    out[r] = move64_of(keys[r], nb, rowmap, s.N, r);
+}
+
+// TOTAL entries row * B + t of nb (ADS row map).
+__global__ void k_total_moves64(const int64_t* __restrict__ tk, int64_t M, int B, int nb, int N,
+                                uint64_t* out) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < M;
+       e += (int64_t)gridDim.x * kBlock)
+    out[e] = move64_of(tk[e], nb, 0, N, (int)(e / B));
 }
 
 // The reference's ADS buffer: header, coords, [client|w], T, C (uint32 wrap).
@@ -2221,6 +2254,21 @@ hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, u
   int grid = (int)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_ads_fill, dim3(grid), dim3(kBlock), 0, st, table, id, S, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_rows_moves64(hipStream_t st, const SlotDev* table, int id, int nb,
+                                  int rowmap, int R, uint64_t* out) {
+  const int grid = std::max(1, std::min((R + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_rows_moves64, dim3(grid), dim3(kBlock), 0, st, table, id, nb, rowmap, R,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_total_moves64(hipStream_t st, const int64_t* tk, int64_t M, int B, int nb,
+                                   int N, uint64_t* out) {
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((M + kBlock - 1) / kBlock, 8192));
+  hipLaunchKernelGGL(k_total_moves64, dim3(grid), dim3(kBlock), 0, st, tk, M, B, nb, N, out);
   return hipGetLastError();
 }
 

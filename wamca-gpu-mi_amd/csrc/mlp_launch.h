@@ -48,6 +48,11 @@ hipError_t mlp_sort_pairs_u64(void* tmp, size_t* tmp_bytes, const uint64_t* kin,
                               hipStream_t st);
 hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, uint32_t* S,
                           uint32_t* out);
+// MLMove64 words of the rows of the last evaluation of nb / of TOTAL entries.
+hipError_t mlp_launch_rows_moves64(hipStream_t st, const SlotDev* table, int id, int nb,
+                                  int rowmap, int R, uint64_t* out);
+hipError_t mlp_launch_total_moves64(hipStream_t st, const int64_t* tk, int64_t M, int B, int nb,
+                                   int N, uint64_t* out);
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);
 hipError_t mlp_launch_dist_list(hipStream_t st, const int32_t* s, int count, int round,
                                 int32_t* out);

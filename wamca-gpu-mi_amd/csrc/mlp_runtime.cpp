@@ -6,6 +6,7 @@
 // across the ABI and nothing calls exit().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -190,6 +191,9 @@ struct mlp_ctx {
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
   int graph_uses = 0;
+  // caller-owned device buffers (mlp_dev_alloc) and event-pair timers
+  std::vector<void*> dev_bufs;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timers;
   // per-launch timing of k_eval (HIP events on the context stream)
   bool timing = false;
   std::vector<hipEvent_t> tev;  // pairs
@@ -461,6 +465,12 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->tev) hipEventDestroy(e);
+  for (void* p : ctx->dev_bufs) hipFree(p);
+  for (auto& t : ctx->timers)
+    if (t.first) {
+      hipEventDestroy(t.first);
+      hipEventDestroy(t.second);
+    }
   hipStreamDestroy(ctx->stream);
   if (ctx->side) hipStreamDestroy(ctx->side);
   delete ctx;
@@ -908,16 +918,20 @@ hipError_t launch_eval_cb(mlp_ctx* ctx, void* p) {
 // Resident workgroups per CU of each sweep tile shape (persistent grids).
 const int kTileWgPerCu[MLP_TILE_COUNT] = {5, 4, 3, 4, 5};
 
+// Moves one evaluation of `mask` over one tour of N positions (ADS row map).
+int64_t tour_moves(int64_t N, uint32_t mask) {
+  int64_t m = 0;
+  if (mask & 1u) m += std::max<int64_t>(0, (N - 4) * (N - 3) / 2);
+  if (mask & 2u) m += std::max<int64_t>(0, ((N + 1) / 2) * (N - 3));
+  for (int k = 1; k <= 3; ++k)
+    if (mask & (1u << (k + 1))) m += std::max<int64_t>(0, (N - k - 2) * (N - k - 1));
+  return m;
+}
+
 // Moves one evaluation of `mask` over the slots covers (ADS row map).
 int64_t launch_moves(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
   int64_t m = 0;
-  for (int q = 0; q < count; ++q) {
-    const int64_t N = ctx->slots[slots[q]].N;
-    if (mask & 1u) m += std::max<int64_t>(0, (N - 4) * (N - 3) / 2);
-    if (mask & 2u) m += std::max<int64_t>(0, ((N + 1) / 2) * (N - 3));
-    for (int k = 1; k <= 3; ++k)
-      if (mask & (1u << (k + 1))) m += std::max<int64_t>(0, (N - k - 2) * (N - k - 1));
-  }
+  for (int q = 0; q < count; ++q) m += tour_moves(ctx->slots[slots[q]].N, mask);
   return m;
 }
 
@@ -931,8 +945,8 @@ int work_bucket(int64_t moves) {
 // nearest tuned bucket of that mask; untuned masks use the launch size (small
 // tiles give a small launch enough workgroups, large tiles evaluate fewer
 // distances per move).
-int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
-  const int b = work_bucket(launch_moves(ctx, slots, count, mask));
+int auto_tile_moves(const mlp_ctx* ctx, int64_t moves, int count, uint32_t mask) {
+  const int b = work_bucket(moves);
   const int8_t* row = ctx->tune[mask & 0x1fu];
   for (int d = 0; d < MLP_TUNE_BUCKETS; ++d) {
     if (b - d >= 0 && row[b - d] >= 0) return row[b - d];
@@ -941,6 +955,10 @@ int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask
   if (count <= 2) return MLP_TILE_4x256;
   if (count <= 16) return MLP_TILE_8x512;
   return MLP_TILE_16x1024;
+}
+
+int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask) {
+  return auto_tile_moves(ctx, launch_moves(ctx, slots, count, mask), count, mask);
 }
 
 // Forget the cached DVND chunk graph (its launches embed the neighborhood
@@ -1842,6 +1860,142 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
     active.swap(next);
   }
   if (rounds_out) return mlp_dvnd_status(ctx, slots, count, nullptr, rounds_out);
+  return MLP_OK;
+}
+
+int mlp_dev_alloc(mlp_ctx* ctx, int64_t bytes, void** dev) {
+  NO_CAPTURE(ctx);
+  if (!ctx || !dev || bytes < 0) return MLP_EINVAL;
+  *dev = nullptr;
+  HCK(hipSetDevice(ctx->device));
+  void* p = nullptr;
+  if (hipMalloc(&p, (size_t)std::max<int64_t>(bytes, 8)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, MLP_ENOMEM, "device allocation of %lld bytes failed", (long long)bytes);
+  }
+  ctx->dev_bufs.push_back(p);
+  *dev = p;
+  return MLP_OK;
+}
+
+int mlp_dev_free(mlp_ctx* ctx, void* dev) {
+  NO_CAPTURE(ctx);
+  if (!ctx) return MLP_EINVAL;
+  if (!dev) return MLP_OK;
+  auto it = std::find(ctx->dev_bufs.begin(), ctx->dev_bufs.end(), dev);
+  if (it == ctx->dev_bufs.end()) return fail(ctx, MLP_EINVAL, "not a buffer of mlp_dev_alloc");
+  ctx->dev_bufs.erase(it);
+  HCK(hipStreamSynchronize(ctx->stream));
+  HCK(hipFree(dev));
+  return MLP_OK;
+}
+
+int mlp_dev_memset(mlp_ctx* ctx, void* dev, int32_t value, int64_t bytes) {
+  NO_CAPTURE(ctx);  // a memset node in a captured graph is not replay-safe here
+  if (!ctx || !dev || bytes < 0) return MLP_EINVAL;
+  HCK(hipMemsetAsync(dev, value & 0xff, (size_t)bytes, ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_copy_to_host(mlp_ctx* ctx, void* host, const void* dev, int64_t bytes) {
+  NO_CAPTURE(ctx);
+  if (!ctx || bytes < 0 || (bytes && (!host || !dev))) return MLP_EINVAL;
+  if (bytes == 0) return MLP_OK;
+  HCK(hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HCK(hipStreamSynchronize(ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_copy_to_device(mlp_ctx* ctx, void* dev, const void* host, int64_t bytes) {
+  NO_CAPTURE(ctx);
+  if (!ctx || bytes < 0 || (bytes && (!host || !dev))) return MLP_EINVAL;
+  if (bytes == 0) return MLP_OK;
+  HCK(hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+  HCK(hipStreamSynchronize(ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_rows_moves64_device(mlp_ctx* ctx, int32_t slot, int32_t nb, uint64_t* dev, int64_t cap,
+                            int32_t* rows) {
+  if (!ctx || !dev) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  if (nb < 0 || nb >= ctx->slots[slot].nbs) return fail(ctx, MLP_EINVAL, "bad neighborhood %d", nb);
+  const Slot& s = ctx->slots[slot];
+  if (s.N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j of MLMove64)");
+  const int rm = s.eval_rowmap[nb];
+  if (rm < 0) return fail(ctx, MLP_ESTATE, "neighborhood %d of slot %d not evaluated", nb, slot);
+  const int R = std::max(0, rows_of(nb, rm, s.N));
+  if (R > cap) return fail(ctx, MLP_EINVAL, "%d rows do not fit %lld entries", R, (long long)cap);
+  if (rows) *rows = R;
+  if (R > 0) HCK(mlp_launch_rows_moves64(ctx->stream, ctx->d_table, slot, nb, rm, R, dev));
+  return MLP_OK;
+}
+
+int mlp_total_moves64_device(mlp_ctx* ctx, int32_t slot, uint64_t* dev, int64_t cap,
+                             int64_t* entries) {
+  if (!ctx || !dev) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  const Slot& s = ctx->slots[slot];
+  if (s.N > 16384) return fail(ctx, MLP_ERANGE, "N > 16384 (14-bit i/j of MLMove64)");
+  if (s.total_nb < 0) return fail(ctx, MLP_ESTATE, "slot %d has no TOTAL evaluation", slot);
+  const int64_t M = (int64_t)s.total_rows * s.total_block;
+  if (M > cap) return fail(ctx, MLP_EINVAL, "%lld entries do not fit %lld", (long long)M, (long long)cap);
+  if (entries) *entries = M;
+  if (M > 0)
+    HCK(mlp_launch_total_moves64(ctx->stream, s.total, M, s.total_block, s.total_nb, s.N, dev));
+  return MLP_OK;
+}
+
+int mlp_timer_create(mlp_ctx* ctx, int32_t* timer) {
+  if (!ctx || !timer) return MLP_EINVAL;
+  hipEvent_t a = nullptr, b = nullptr;
+  HCK(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return fail(ctx, MLP_EDEVICE, "hipEventCreate failed");
+  }
+  for (size_t q = 0; q < ctx->timers.size(); ++q)
+    if (!ctx->timers[q].first) {
+      ctx->timers[q] = {a, b};
+      *timer = (int32_t)q;
+      return MLP_OK;
+    }
+  ctx->timers.emplace_back(a, b);
+  *timer = (int32_t)ctx->timers.size() - 1;
+  return MLP_OK;
+}
+
+static bool timer_ok(mlp_ctx* ctx, int32_t t) {
+  return t >= 0 && t < (int32_t)ctx->timers.size() && ctx->timers[(size_t)t].first;
+}
+
+int mlp_timer_start(mlp_ctx* ctx, int32_t timer) {
+  if (!ctx || !timer_ok(ctx, timer)) return MLP_EINVAL;
+  HCK(hipEventRecord(ctx->timers[(size_t)timer].first, ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_timer_stop(mlp_ctx* ctx, int32_t timer) {
+  if (!ctx || !timer_ok(ctx, timer)) return MLP_EINVAL;
+  HCK(hipEventRecord(ctx->timers[(size_t)timer].second, ctx->stream));
+  return MLP_OK;
+}
+
+int mlp_timer_elapsed(mlp_ctx* ctx, int32_t timer, float* ms) {
+  NO_CAPTURE(ctx);
+  if (!ctx || !ms || !timer_ok(ctx, timer)) return MLP_EINVAL;
+  HCK(hipEventSynchronize(ctx->timers[(size_t)timer].second));
+  HCK(hipEventElapsedTime(ms, ctx->timers[(size_t)timer].first,
+                          ctx->timers[(size_t)timer].second));
+  return MLP_OK;
+}
+
+int mlp_timer_destroy(mlp_ctx* ctx, int32_t timer) {
+  if (!ctx || !timer_ok(ctx, timer)) return MLP_EINVAL;
+  auto& t = ctx->timers[(size_t)timer];
+  hipEventDestroy(t.first);
+  hipEventDestroy(t.second);
+  t = {nullptr, nullptr};
   return MLP_OK;
 }
 

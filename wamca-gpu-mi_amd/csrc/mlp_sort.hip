@@ -1,16 +1,16 @@
 // mlp_sort.hip -- device radix sort of (uint64 key, int32 value) pairs for the
-// TOTAL-mode merge.  rocPRIM's LSD radix sort (through hipCUB) is stable, so
-// entries with equal keys keep their input order.  Kept in its own
-// translation unit: the library templates are slow to compile.
+// TOTAL-mode merge.  rocPRIM's LSD radix sort is stable, so entries with
+// equal keys keep their input order.  Kept in its own translation unit: the
+// library templates are slow to compile.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "mlp_launch.h"
 
 hipError_t mlp_sort_pairs_u64(void* tmp, size_t* tmp_bytes, const uint64_t* kin, uint64_t* kout,
                               const int32_t* vin, int32_t* vout, int n, int end_bit,
                               hipStream_t st) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, kin, kout, vin, vout, n, 0, end_bit,
-                                            st);
+  return rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u,
+                                   (unsigned)end_bit, st);
 }

@@ -1,6 +1,6 @@
 """Sweep tile-shape autotuning.
 
-``python -m wamca_mi.tune --out FILE`` times the three sweep tile shapes on
+``python -m wamca_mi.tune --out FILE`` times the five sweep tile shapes on
 launches of every neighborhood mask a DVND round or a full sweep issues, over
 instance sizes and tours per launch spanning 2^10 .. 2^33 moves, and writes the
 fastest shape per (mask, log2 moves) as JSON.  ``Engine`` loads the table of
