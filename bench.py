@@ -334,6 +334,40 @@ def probe_masks(eng, n, tours, tiles, masks):
     eng.set_tile(-1)
 
 
+def floor_probe(eng, n=1000, reps=2000):
+    """The one-tour sweep's fixed cost, split: kernels with its launch shape
+    (the 4x256 grid of an n=1000 sweep, 256 threads, its kernel-argument
+    table) that do nothing, then add LDS and a barrier, the dependent loads a
+    tile starts with, and the row folds' atomics; then the sweep itself on one
+    tour.  Back-to-back launches; for kernel traces (the durations are the
+    trace's), with HIP-event averages printed beside."""
+    from wamca_mi import inputs as I
+    xs, ys = I.uniform(n, 1, 1000)
+    inst = eng.instance(xs, ys, 1)
+    s = eng.slot(inst)
+    eng.set_tour(s, I.perm_tour(1, n))
+    N = n + 1
+    grid = ((N - 3 + 3) // 4) * ((N - 2 + 255) // 256) + ((N + 1) // 2 + 3) // 4 * ((N - 3 + 255) // 256)
+    names = ("empty", "lds", "loads", "loads_atomics")
+    for kind, name in enumerate(names):
+        eng.launch_probe(s, kind, grid, 50)
+        eng.region_begin()
+        eng.launch_probe(s, kind, grid, reps)
+        us = eng.region_end() * 1e3 / reps
+        print(json.dumps({"probe": name, "grid": grid, "reps": reps, "us_per_launch": round(us, 3)}),
+              flush=True)
+    eng.set_tile(0)
+    eng.eval_each([s] * 50, 0x1F, 0)
+    eng.region_begin()
+    eng.eval_each([s] * reps, 0x1F, 0)
+    us = eng.region_end() * 1e3 / reps
+    print(json.dumps({"probe": "k_sweep<4,256> n=1000", "grid": grid, "reps": reps,
+                      "us_per_launch": round(us, 3)}), flush=True)
+    eng.set_tile(-1)
+    eng.destroy_slot(s)
+    eng.destroy_instance(inst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +393,8 @@ def main():
                     help="only time one launch per (tile, mask) over --tours tours and over one "
                          "tour (per-launch lines for counter and trace runs)")
     ap.add_argument("--probe-tiles", default="2,4")
+    ap.add_argument("--floor-probe", action="store_true",
+                    help="only run the fixed-cost probe kernels and the one-tour sweep")
     ap.add_argument("--probe-mask-list", default="31,29,2,1,4")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -375,6 +411,10 @@ def main():
         eng.close()
         return
 
+    if args.floor_probe:
+        floor_probe(eng)
+        eng.close()
+        return
     if args.probe_masks:
         probe_masks(eng, args.n, args.tours, [int(t) for t in args.probe_tiles.split(",")],
                     [int(m) for m in args.probe_mask_list.split(",")])

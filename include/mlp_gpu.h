@@ -90,8 +90,8 @@ int mlp_mem_info(mlp_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /* Instances.  round = 0 truncates, 1 rounds to nearest (dr = 0.5).
  * Coordinates may be any int32 values whose span (max - min) is at most 32767
- * in x and in y; they are stored translated to [0, span], which leaves every
- * distance unchanged (the reference reads TSPLIB coordinates as they are,
+ * in x and in y; an axis whose values leave [0, 32767] is stored translated
+ * to [0, span], which leaves every distance unchanged (the reference reads TSPLIB coordinates as they are,
  * mlproblem.cpp:110-117, 153-166).  MLP_ERANGE: a larger span, n + 1 above
  * MLP_MAX_N, or N (N + 32) maxd >= 2^43 (maxd bounds a distance; the row keys
  * hold delta * 2^20 + index in 64 bits). */
@@ -304,6 +304,12 @@ int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb
 /* Device-to-device copies of `bytes` (reps times): a kernel of known traffic
  * for calibrating memory counters. */
 int mlp_copy_probe(mlp_ctx* ctx, int64_t bytes, int32_t reps);
+/* Fixed-cost probes with a sweep's launch shape (grid workgroups of 256
+ * threads, the sweep's kernel-argument table), reps launches back to back:
+ * kind 0 empty, 1 + LDS and a barrier, 2 + the dependent loads a sweep tile
+ * starts with (the slot's table entry, info and xy words), 3 + four 64-bit
+ * atomic minima per workgroup.  For kernel traces; no result. */
+int mlp_launch_probe(mlp_ctx* ctx, int32_t slot, int32_t kind, int32_t grid, int32_t reps);
 /* Per-launch eval timing: between begin and end every mlp_eval launch is
  * bracketed by its own pair of HIP events (up to capacity launches); end
  * synchronises and returns the summed kernel time and the launch count. */

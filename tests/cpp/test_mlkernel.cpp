@@ -271,6 +271,7 @@ static int run_total(const char* tsp, int seed, int sols, int argc, char** argv)
       int ccount = 0;
       const int csum = k->mergeGreedy(merged.data(), ccount);
       k->mergeGPU();
+      k->recvResult();  // the merged list from moveData (WamcaExperiment.hpp:229-290)
       long long gsum = 0;
       int gcount = 0;
       for (uint e = 0; e < k->moveElems && k->transBuffer.p_move64[e].cost < 0; ++e) {

@@ -35,8 +35,10 @@ enum {
 };
 
 // info[PATH32] bits: 2-opt / swap and or-opt deltas fit int32; swap and
-// or-opt deltas below 2^26 (keys packed with a 5-bit column iteration)
-enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2, MLP_PK_SWAP_OROPT = 4 };
+// or-opt deltas below 2^26 (keys packed with a 5-bit column iteration);
+// T[N-1] below 2^24 (products of a position and a prefix time are exact in
+// 24-bit multiplies)
+enum { MLP_P32_2OPT = 1, MLP_P32_SWAP_OROPT = 2, MLP_PK_SWAP_OROPT = 4, MLP_T24 = 8 };
 
 struct SlotDev {
   int32_t N;      // positions (n + 1)
@@ -85,12 +87,14 @@ struct EvalSeg {
 #define MLP_EVALSEG_INTS 5
 
 // Sweep segments: kind 0 = fixed-i row blocks (swap + or-opt-1/2/3 rows of
-// the ADS row map sharing distance vectors), kind 1 = 2-opt diagonal rows.
+// the ADS row map sharing distance vectors; with mask bit 1 also the 2-opt
+// moves (i, j > i) of the block's rows, folded per diagonal row), kind 1 =
+// 2-opt diagonal rows.
 #define MLP_2OPT_ROWS 4
 struct SweepSeg {
   int32_t slot;
   int32_t kind;
-  int32_t mask;    // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4)
+  int32_t mask;    // neighborhoods of kind 0 to evaluate (bits 0, 2, 3, 4; bit 1 with bit 0)
   int32_t blk0;
   int32_t ncols;   // column tiles per row block
   int32_t parity;  // bit nb: buffer written by a standalone eval
@@ -105,6 +109,9 @@ struct SweepSeg {
 #define MLP_TILE_8x256 3
 #define MLP_TILE_4x512 4
 #define MLP_TILE_COUNT 5
+// Launch flag beside the shape: the fixed-row tiles also evaluate 2-opt
+// (shapes of eight or more rows; the launch then has no 2-opt diagonal tiles).
+#define MLP_TILE_TWO 0x100
 
 #define MLP_SEGS_INLINE 32
 struct EvalSegsInline {

@@ -465,7 +465,8 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
     s.info[MLP_INFO_TTOT] = Ttot;
     s.info[MLP_INFO_PATH32] = (b2 < 2147483647LL ? MLP_P32_2OPT : 0) |
                               (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0) |
-                              (bso < (1LL << 26) ? MLP_PK_SWAP_OROPT : 0);
+                              (bso < (1LL << 26) ? MLP_PK_SWAP_OROPT : 0) |
+                              (Ttot < (1LL << 24) ? MLP_T24 : 0);
   }
   __syncthreads();
 }
@@ -1325,6 +1326,40 @@ __global__ __launch_bounds__(kBlock) void k_clear_rows(const SlotDev* __restrict
     }
 }
 
+// Fixed-cost probes with the one-tour sweep's launch shape and kernel
+// argument size (a SweepSegsInline table): kind 0 does nothing, 1 touches the
+// tile's LDS and synchronises, 2 adds the dependent loads a sweep tile starts
+// with (slot-table entry, then info and xy of the slot) and one store per
+// workgroup, 3 adds four 64-bit atomic minima per workgroup and a row reset
+// store, as a tile's row folds do.  out holds at least 2 * 8192 int64.
+__global__ __launch_bounds__(kBlock) void k_probe(const SlotDev* __restrict__ table,
+                                                  SweepSegsInline inl, int kind, int64_t* out) {
+  __shared__ int L[1536];
+  if (kind == 0) {
+    if (threadIdx.x == 1023) out[0] = inl.s[blockIdx.x & 31].blk0;
+    return;
+  }
+  int v = threadIdx.x;
+  if (kind >= 2) {
+    const SlotDev& s = table[inl.s[0].slot];
+    v += (int)ldg(s.info + MLP_INFO_PATH32) + ldg(s.xy + (int)((blockIdx.x * 4 + threadIdx.x) % s.N));
+  }
+  L[threadIdx.x] = v;
+  __syncthreads();
+  if (kind == 1) {
+    if (threadIdx.x == 1023) out[0] = L[(threadIdx.x + 1) & 255];
+    return;
+  }
+  if (kind == 2) {
+    if (threadIdx.x == 0) out[blockIdx.x % 8192] = L[255];
+    return;
+  }
+  if (threadIdx.x < 4) out[8192 + (blockIdx.x % 2048) * 4 + threadIdx.x] = MLP_KEY_NONE;
+  if ((threadIdx.x & 63) == 0)
+    atomicMin((long long*)(out + (blockIdx.x % 2048) * 4 + threadIdx.x / 64),
+              (long long)(L[threadIdx.x] + (int)blockIdx.x));
+}
+
 __global__ void k_fill_i64(int64_t* p, int64_t count, int64_t v) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
        t += (int64_t)gridDim.x * blockDim.x)
@@ -1424,14 +1459,34 @@ struct TileLds {
   int32_t rxy[R + 8];
 };
 
+// 2-opt inside a fixed-row tile: the column constants h, the row constants g
+// and the tile's per-diagonal minimum keys (diagonal D = j - i at slot
+// D - (q0 - b0 - R)); dgx holds the second key of the middle diagonals that
+// belong to two rows of the 2-opt row map.
+template <int R, int C>
+struct TwoLds {
+  int32_t h[C + 4];
+  int32_t rg[R + 8];
+  int64_t dg[C + R];
+  int64_t dgx[4];
+};
+
+struct NoTwoLds {};
+
+template <int R, int C, bool TWO>
+struct FixedLds {
+  TileLds<R, C> t;
+  std::conditional_t<TWO, TwoLds<R, C>, NoTwoLds> two;
+};
+
 template <int C>
 struct DiagLds {
   uint16_t e[MLP_2OPT_ROWS][2][C + 2];
 };
 
-template <int R, int C>
+template <int R, int C, bool TWO>
 union SweepLds {
-  TileLds<R, C> t;
+  FixedLds<R, C, TWO> f;
   DiagLds<C> d;
 };
 
@@ -1526,32 +1581,62 @@ __device__ __forceinline__ int64_t packed_row_key(int32_t bk, int jb0, int off) 
 // lanes stride over the tile's columns j and evaluate all RW rows for their
 // column.  Row r: i = b0+r+1, j in [i+2, N-2].  delta = Cs_r + Bs(j) +
 // (N-i) v_{i-1}[j] + (N-i-1) v_{i+1}[j] + (N-j) v_i[j-1] + (N-j-1) v_i[j+1].
-template <int R, int C, int RW>
+//
+// TWO: the same loop also evaluates the 2-opt moves (i, j), j in [i+1, N-2],
+// of rows i <= N-3 (mlk2opt.cu:168-219 in the closed form
+// g[i] + h[j] + (N-i) v_{i-1}[j] + (N-j-1) v_i[j+1] + (j-i)(T[i] + T[j]),
+// written A_i + B_j + j T[i] - i T[j] with A_i = g[i] - i T[i] and
+// B_j = h[j] + j T[j]); its two distance products are swap's first and last.
+// The 2-opt row map's rows are diagonals (row D-1 for j - i = D <= ceil(N/2),
+// row N-2-D for D > N-2-ceil(N/2), both in between), so each move's key
+// (delta << 20 | row-local index) is folded into the tile's per-diagonal
+// minimum in LDS; the tile flushes one key per diagonal afterwards.  Needs
+// the int32 2-opt path and T[N-1] < 2^24 (the j T[i] and i T[j] products in
+// 24-bit multiplies).
+template <int R, int C, int RW, bool TWO, typename TL>
 __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                          int parity, int wave, int lane, bool pk) {
+                          TL& T2, int parity, int wave, int lane, bool pk) {
   const int N = s.N;
   const int r0 = wave * RW;
   const int rv = min(RW, N - 4 - b0 - r0);  // rows with i <= N-4
-  if (rv <= 0) return;
+  const int rv2 = TWO ? min(RW, N - 3 - b0 - r0) : rv;  // 2-opt rows: i <= N-3
+  if (rv2 <= 0) return;
   int32_t bd[RW], bc[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     bd[r] = INT32_MAX;
     bc[r] = -1;
   }
-  uint32_t myCs = 0u;
-  if (lane < rv) {
+  uint32_t myCs = 0u, myA = 0u, myT = 0u;
+  if (lane < rv2) {
     const int o = r0 + lane + 1;  // i - b0
     const int i = b0 + o;
     myCs = 0u - umul(N - i, (uint32_t)L.rw[o]) - umul(N - i - 1, (uint32_t)L.rw[o + 1]);
+    if constexpr (TWO) {
+      myT = (uint32_t)L.rt[o];
+      myA = (uint32_t)T2.rg[o] - (uint32_t)i * myT;
+    }
   }
-  uint32_t Cs[RW];
+  uint32_t Cs[RW], A2[RW], T2i[RW];
 #pragma unroll
-  for (int r = 0; r < RW; ++r) Cs[r] = lane_bcast(myCs, r);
+  for (int r = 0; r < RW; ++r) {
+    Cs[r] = lane_bcast(myCs, r);
+    if constexpr (TWO) {
+      A2[r] = lane_bcast(myA, r);
+      T2i[r] = lane_bcast(myT, r);
+    }
+  }
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - 1);
-  const int jlo = max(q0, b0 + r0 + 3);
+  const int jlo = max(q0, b0 + r0 + (TWO ? 2 : 3));
   const int jb0 = jlo - ((jlo - q0) & (kWave - 1));
+  // 2-opt: diagonal slot of (r, j) is j - (b0 + r0 + r + 1) - dlo; row-map
+  // split of the diagonals (rows2 = ceil(N/2)): D <= rows2 -> row D - 1 with
+  // index j - 2; D > N - 2 - rows2 -> row N - 2 - D with index N - 2 - j
+  const int rows2 = (N + 1) >> 1;
+  const int dlo = q0 - b0 - R;
+  const int mid0 = N - 1 - rows2;  // first diagonal that also has a part-0 row
+  const bool mids = TWO && (q0 - b0 - R <= rows2) && (q1 - 1 - b0 - 1 >= mid0);
   // FR: every row of the wave is in range (no per-row guard in the loop);
   // PK: packed keys in bd (bc unused)
   auto cols = [&](auto fr, auto pkc) {
@@ -1564,16 +1649,26 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
       const int x = jj - cbase;
       const uint32_t nj = (uint32_t)(N - jj);
       const uint32_t cs = opaque(0u - umul(nj, L.w[x]) - umul(nj - 1u, L.w[x + 1]));
+      uint32_t Bj = 0u, Tj = 0u;
+      int ctx1 = 0, ctx0 = 0;
+      if constexpr (TWO) {
+        Tj = (uint32_t)L.t[x];
+        Bj = opaque((uint32_t)T2.h[x] + umul((uint32_t)jj, Tj));
+        ctx1 = jj - 2;
+        ctx0 = N - 2 - jj;
+      }
       uint32_t c[RW + 2];
 #pragma unroll
       for (int a = 0; a < RW + 2; ++a) c[a] = V[a][x];
       const bool full = jb >= b0 + r0 + RW + 2 && jb + kWave <= jhi;
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
+        const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
+        const uint32_t v4 = V[r + 1][x + 1];
+        const uint32_t X = umul(a1, c[r]) + umul(nj - 1u, v4);  // shared by swap and 2-opt
         if (FR || r < rv) {
-          const uint32_t a1 = (uint32_t)(N - b0 - r0 - r - 1);
-          const uint32_t d = Cs[r] + cs + umul(a1, c[r]) + umul(a1 - 1u, c[r + 2]) +
-                             umul(nj, V[r + 1][x - 1]) + umul(nj - 1u, V[r + 1][x + 1]);
+          const uint32_t d = Cs[r] + cs + X + umul(a1 - 1u, c[r + 2]) +
+                             umul(nj, V[r + 1][x - 1]);
           if (PK) {
             const int32_t k = pack_key(d, it);
             if (full) bd[r] = min(bd[r], k);
@@ -1584,12 +1679,25 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
             take_min_if(bd[r], bc[r], d, j, j < jhi && j >= b0 + r0 + r + 3);
           }
         }
+        if constexpr (TWO) {
+          if (FR || r < rv2) {
+            const int i = b0 + r0 + r + 1;
+            const uint32_t d2 =
+                A2[r] + Bj + X + umul((uint32_t)jj, T2i[r]) - umul((uint32_t)i, Tj);
+            const int D = jj - i;
+            const bool ok = full || (j < jhi && j >= i + 1);
+            const int64_t base = (int64_t)(int32_t)d2 * (int64_t)(1 << MLP_KEY_SHIFT);
+            if (ok) atomicMin((long long*)&T2.dg[D - dlo], (long long)(base + (D <= rows2 ? ctx1 : ctx0)));
+            if (mids && ok && D >= mid0 && D <= rows2)
+              atomicMin((long long*)&T2.dgx[D - mid0], (long long)(base + ctx0));
+          }
+        }
       }
     }
   };
   int64_t* out = row_buf(s, 0, parity & 1) + b0 + r0;
   if (pk) {
-    if (rv == RW) cols(Flag<true>{}, Flag<true>{});
+    if (rv2 == RW) cols(Flag<true>{}, Flag<true>{});
     else cols(Flag<false>{}, Flag<true>{});
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
@@ -1598,7 +1706,7 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
     }
     return;
   }
-  if (rv == RW) cols(Flag<true>{}, Flag<false>{});
+  if (rv2 == RW) cols(Flag<true>{}, Flag<false>{});
   else cols(Flag<false>{}, Flag<false>{});
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -1750,11 +1858,30 @@ __device__ __forceinline__ void sweep_fixed_wide(const SlotDev& s, int rb, int c
   }
 }
 
-template <int R, int C>
+// 2-opt rows [b0, b0 + R) of a fixed-row tile whose launch includes 2-opt
+// but whose tour does not take the in-tile path (64-bit deltas, or
+// T[N-1] >= 2^24): column tile 0 resets them and evaluates each diagonal row
+// with one wave.
+template <int R>
+__device__ __forceinline__ void two_rows_wide(const SlotDev& s, int rb, int ct, int parity,
+                                              bool p32) {
+  if (ct != 0) return;
+  const int N = s.N, wave = wave_id(), lane = threadIdx.x % kWave;
+  const int rows2 = (N + 1) >> 1, b0 = rb * R;
+  const int par = (parity >> 1) & 1;
+  reset_other(s, 1, par, b0, min(b0 + R, rows2), rb == 0);
+  for (int by = b0 + wave; by < min(b0 + R, rows2); by += kBlock / kWave) {
+    if (p32) eval_row<1, true>(s, 1, 0, by, lane, row_buf(s, 1, par) + by);
+    else eval_row<1, false>(s, 1, 0, by, lane, row_buf(s, 1, par) + by);
+  }
+}
+
+template <int R, int C, bool TWO>
 __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
-                            TileLds<R, C>& L) {
+                            FixedLds<R, C, TWO>& F) {
   constexpr int RW = R / (kBlock / kWave);
   constexpr int FI = (C + 4 + kBlock - 1) / kBlock;  // fill iterations per thread
+  TileLds<R, C>& L = F.t;
   const int N = s.N, tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int b0 = rb * R;
@@ -1766,24 +1893,30 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   const int64_t pflags = ldg(s.info + MLP_INFO_PATH32);
   const bool p32 = (pflags & MLP_P32_SWAP_OROPT) != 0;
   const bool pk = (pflags & MLP_PK_SWAP_OROPT) != 0;  // |delta| < 2^26: packed keys
+  // 2-opt in this tile: int32 2-opt deltas and 24-bit prefix times
+  const bool two = TWO && (mask & 2) && p32 && (pflags & MLP_P32_2OPT) && (pflags & MLP_T24);
   int xa[R + 4];
 #pragma unroll
   for (int va = 0; va < R + 4; ++va) xa[va] = ldg(s.xy + min(b0 + va, N - 1));
-  int xc[FI], wv[FI], tv[FI];
+  int xc[FI], wv[FI], tv[FI], hv[FI];
 #pragma unroll
   for (int it = 0; it < FI; ++it) {
     const int c = min(cbase + tid + it * kBlock, cend - 1);
     xc[it] = ldg(s.xy + c);
     wv[it] = ldg(s.w + c);
     tv[it] = ldg(s.t32 + c);
+    if (TWO) hv[it] = ldg(s.h32 + c);
   }
-  int rwv = 0, rtv = 0, rxv = 0;
+  int rwv = 0, rtv = 0, rxv = 0, rgv = 0;
   if (tid < R + 8) {
     const int p = min(b0 + tid, N - 1);
     rwv = ldg(s.w + p);
     rtv = ldg(s.t32 + p);
     rxv = ldg(s.xy + p);
+    if (TWO) rgv = ldg(s.g32 + p);
   }
+  if (TWO && (mask & 2) && !two)
+    two_rows_wide<R>(s, rb, ct, parity, (pflags & MLP_P32_2OPT) != 0);
   if (!p32) {
     sweep_fixed_wide<R>(s, rb, ct, mask, parity);
     return;
@@ -1793,6 +1926,7 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if (nb != 1 && ((mask >> nb) & 1))
         reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+    if (two) reset_other(s, 1, (parity >> 1) & 1, b0, min(b0 + R, (N + 1) >> 1), rb == 0);
   }
   // swap alone covers j >= i + 2 only: a tile left of that diagonal is empty
   if (mask == 1 && q1 <= b0 + 3) return;
@@ -1804,15 +1938,21 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
       for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc[it], dr);
       L.w[c - cbase] = (uint16_t)wv[it];
       L.t[c - cbase] = tv[it];
+      if constexpr (TWO) F.two.h[c - cbase] = hv[it];
     }
   }
   if (tid < R + 8) {
     L.rw[tid] = rwv;
     L.rt[tid] = rtv;
     L.rxy[tid] = rxv;
+    if constexpr (TWO) F.two.rg[tid] = rgv;
+  }
+  if constexpr (TWO) {
+    for (int q = tid; q < C + R; q += kBlock) F.two.dg[q] = MLP_KEY_NONE;
+    if (tid < 4) F.two.dgx[tid] = MLP_KEY_NONE;
   }
   __syncthreads();
-  if (R == kBlock / kWave && __popc(mask & 0x1d) >= 2) {
+  if (!TWO && R == kBlock / kWave && __popc(mask & 0x1d) >= 2) {
     // Several neighborhoods on a four-row tile: wave w takes the w-th
     // neighborhood of the mask over all rows, so the neighborhoods run side
     // by side instead of one after another (the latency of a one-tour launch).
@@ -1823,16 +1963,32 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
         if (q == 0) nb = b;
         --q;
       }
-    if (nb == 0) tile_swap<R, C, R>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
+    if (nb == 0) tile_swap<R, C, R, false>(s, b0, q0, q1, cbase, L, F.two, parity, 0, lane, pk);
     else if (nb == 2) tile_oropt<R, C, R, 1>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
     else if (nb == 3) tile_oropt<R, C, R, 2>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
     else if (nb == 4) tile_oropt<R, C, R, 3>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
     return;
   }
-  if (mask & 1) tile_swap<R, C, RW>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if (two) tile_swap<R, C, RW, TWO>(s, b0, q0, q1, cbase, L, F.two, parity, wave, lane, pk);
+  else if (mask & 1) tile_swap<R, C, RW, false>(s, b0, q0, q1, cbase, L, F.two, parity, wave, lane, pk);
   if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
   if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
   if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if constexpr (TWO) {
+    if (two) {
+      // one key per diagonal of the tile into the 2-opt row buffer
+      __syncthreads();
+      const int rows2 = (N + 1) >> 1, dlo = q0 - b0 - R, mid0 = N - 1 - rows2;
+      int64_t* out = row_buf(s, 1, (parity >> 1) & 1);
+      for (int q = tid; q < C + R; q += kBlock) {
+        const int64_t key = F.two.dg[q];
+        const int D = dlo + q;
+        if (key != MLP_KEY_NONE) fold_key(out + (D <= rows2 ? D - 1 : N - 2 - D), key);
+      }
+      if (tid < 4 && F.two.dgx[tid] != MLP_KEY_NONE)
+        fold_key(out + (N - 2 - (mid0 + tid)), F.two.dgx[tid]);
+    }
+  }
 }
 
 __device__ __forceinline__ void sweep_2opt_wide(const SlotDev& s, int rb, int ct, int parity) {
@@ -2031,9 +2187,9 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
   return find_inline<SweepSeg>(inl, nsegs, item);
 }
 
-template <int R, int C>
+template <int R, int C, bool TWO>
 __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds<R, C>& L, bool dvnd) {
+                           SweepLds<R, C, TWO>& L, bool dvnd) {
   const SlotDev& s = table[sg.slot];
   if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
   // bit nb of par: the buffer this sweep writes (the one not holding the last result)
@@ -2051,7 +2207,7 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
   }
-  if (sg.kind == 0) sweep_fixed<R, C>(s, rb, ct, sg.mask, par, L.t);
+  if (sg.kind == 0) sweep_fixed<R, C, TWO>(s, rb, ct, sg.mask, par, L.f);
   else sweep_2opt<C>(s, rb, ct, par, L.d);
 }
 
@@ -2062,17 +2218,17 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
 // depends on every tile being resident at once, and many-tour launches,
 // which gain from the occupancy.  The large shapes are held to three
 // workgroups per CU by their LDS tiles anyway.
-template <int R, int C>
+template <int R, int C, bool TWO>
 __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter,
                                                   int skip_inactive) {
-  __shared__ SweepLds<R, C> L;
+  __shared__ SweepLds<R, C, TWO> L;
   if (!counter) {
     const int item = blockIdx.x;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, TWO>(table, sg, item, L, skip_inactive != 0);
     return;
   }
   __shared__ int s_item;
@@ -2083,21 +2239,23 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     __syncthreads();
     if (item >= total) break;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, TWO>(table, sg, item, L, skip_inactive != 0);
     __syncthreads();
   }
 }
 
-template __global__ void k_sweep<4, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*, int);
-template __global__ void k_sweep<8, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*, int);
-template __global__ void k_sweep<16, 1024>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                           int, int*, int);
-template __global__ void k_sweep<8, 256>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*, int);
-template __global__ void k_sweep<4, 512>(const SlotDev*, SweepSegsInline, const SweepSeg*, int,
-                                         int, int*, int);
+#define MLP_SWEEP_INST(R, C, TWO)                                                     \
+  template __global__ void k_sweep<R, C, TWO>(const SlotDev*, SweepSegsInline, const SweepSeg*, \
+                                              int, int, int*, int);
+MLP_SWEEP_INST(4, 256, false)
+MLP_SWEEP_INST(8, 512, false)
+MLP_SWEEP_INST(16, 1024, false)
+MLP_SWEEP_INST(8, 256, false)
+MLP_SWEEP_INST(4, 512, false)
+// shapes of eight or more rows also come with 2-opt evaluated in the fixed tiles
+MLP_SWEEP_INST(8, 512, true)
+MLP_SWEEP_INST(16, 1024, true)
+MLP_SWEEP_INST(8, 256, true)
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -2144,15 +2302,25 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid, int skip_inactive) {
-#define MLP_SWEEP_LAUNCH(R, C)                                                                 \
-  hipLaunchKernelGGL((k_sweep<R, C>), dim3(grid), dim3(kBlock), 0, st, table, inl, segs_dev,   \
-                     nsegs, total, counter, skip_inactive)
-  switch (tile) {
-    case MLP_TILE_4x256: MLP_SWEEP_LAUNCH(4, 256); break;
-    case MLP_TILE_8x512: MLP_SWEEP_LAUNCH(8, 512); break;
-    case MLP_TILE_8x256: MLP_SWEEP_LAUNCH(8, 256); break;
-    case MLP_TILE_4x512: MLP_SWEEP_LAUNCH(4, 512); break;
-    default: MLP_SWEEP_LAUNCH(16, 1024); break;
+  // MLP_TILE_TWO: fixed-row tiles with 2-opt (mask bit 1), the shapes' TWO variants
+#define MLP_SWEEP_LAUNCH(R, C, TWO)                                                            \
+  hipLaunchKernelGGL((k_sweep<R, C, TWO>), dim3(grid), dim3(kBlock), 0, st, table, inl,        \
+                     segs_dev, nsegs, total, counter, skip_inactive)
+  switch (tile & 0xff) {
+    case MLP_TILE_4x256: MLP_SWEEP_LAUNCH(4, 256, false); break;
+    case MLP_TILE_8x512:
+      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(8, 512, true);
+      else MLP_SWEEP_LAUNCH(8, 512, false);
+      break;
+    case MLP_TILE_8x256:
+      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(8, 256, true);
+      else MLP_SWEEP_LAUNCH(8, 256, false);
+      break;
+    case MLP_TILE_4x512: MLP_SWEEP_LAUNCH(4, 512, false); break;
+    default:
+      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(16, 1024, true);
+      else MLP_SWEEP_LAUNCH(16, 1024, false);
+      break;
   }
 #undef MLP_SWEEP_LAUNCH
   return hipGetLastError();
@@ -2302,6 +2470,14 @@ hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* 
 hipError_t mlp_launch_clear_rows(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
                                  const SweepSeg* segs_dev, int nsegs) {
   hipLaunchKernelGGL(k_clear_rows, dim3(nsegs), dim3(kBlock), 0, st, table, inl, segs_dev, nsegs);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_probe(hipStream_t st, const SlotDev* table, int slot, int kind, int grid,
+                            int64_t* out) {
+  SweepSegsInline inl{};
+  inl.s[0].slot = slot;
+  hipLaunchKernelGGL(k_probe, dim3(grid), dim3(kBlock), 0, st, table, inl, kind, out);
   return hipGetLastError();
 }
 

@@ -62,6 +62,8 @@ hipError_t mlp_launch_weights(hipStream_t st, const int32_t* xs, const int32_t* 
                               int round, int32_t* out);
 hipError_t mlp_launch_clear_rows(hipStream_t st, const SlotDev* table, const SweepSegsInline& inl,
                                  const SweepSeg* segs_dev, int nsegs);
+hipError_t mlp_launch_probe(hipStream_t st, const SlotDev* table, int slot, int kind, int grid,
+                            int64_t* out);
 hipError_t mlp_launch_fill_i64(hipStream_t st, int64_t* p, int64_t count, int64_t v);
 hipError_t mlp_launch_set_slot(hipStream_t st, SlotDev* table, int id, const SlotDev& d);
 hipError_t mlp_launch_slot_init(hipStream_t st, SlotDev* table, int id, const SlotDev& d,

@@ -522,10 +522,12 @@ int mlp_device_info(mlp_ctx* ctx, int32_t* cu_count, int64_t* hbm_bytes, char* n
 
 // Instance validation shared by instance creation and mlp_check_job.  The
 // coordinates may be any int32 values whose span is at most 32767 in x and in
-// y: they are translated to [0, span] (distances depend on differences only,
-// so the reference's raw-coordinate distances, mlproblem.cpp:153-166, are
-// unchanged), which keeps dx^2 + dy^2 within int32 and the device's packed
-// 16-bit coordinates exact.  *maxd receives a bound of every distance.
+// y; an axis whose values leave [0, 32767] is translated to [0, span]
+// (distances depend on differences only, so the reference's raw-coordinate
+// distances, mlproblem.cpp:153-166, are unchanged), which keeps dx^2 + dy^2
+// within int32 and the device's packed 16-bit coordinates exact.  *xmin_out
+// and *ymin_out receive the translation (0 for an axis inside [0, 32767]),
+// *maxd a bound of every distance.
 static int check_instance(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, int32_t n,
                           int32_t round, int32_t* xmin_out, int32_t* ymin_out, int* maxd) {
   if (!xs || !ys || n < 2) return fail(ctx, MLP_EINVAL, "need at least 2 clients");
@@ -549,8 +551,8 @@ static int check_instance(mlp_ctx* ctx, const int32_t* xs, const int32_t* ys, in
   if (N * (N + 32) * (uint64_t)md >= (1ull << 43))
     return fail(ctx, MLP_ERANGE, "n = %d with a coordinate span of %d exceeds the 64-bit move keys",
                 n, md);
-  if (xmin_out) *xmin_out = (int32_t)xmin;
-  if (ymin_out) *ymin_out = (int32_t)ymin;
+  if (xmin_out) *xmin_out = (xmin < 0 || xmax > 32767) ? (int32_t)xmin : 0;
+  if (ymin_out) *ymin_out = (ymin < 0 || ymax > 32767) ? (int32_t)ymin : 0;
   if (maxd) *maxd = md;
   return MLP_OK;
 }
@@ -1033,10 +1035,13 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     if (tile < 0) tile = auto_tile(ctx, slots, count, nb_mask);
     int TR, TC;
     mlp_tile_shape(tile, &TR, &TC);
+    // with swap in the launch, tiles of eight or more rows evaluate 2-opt
+    // too (sharing swap's distance products) instead of 2-opt diagonal tiles
+    const bool two = (nb_mask & 3u) == 3u && TR >= 8;
     std::vector<SweepSeg> segs;
     segs.reserve((size_t)count * 2);
     int total = 0;
-    const uint32_t fixed = nb_mask & 0x1du;
+    const uint32_t fixed = (nb_mask & 0x1du) | (two ? 2u : 0u);
     for (int q = 0; q < count; ++q) {
       Slot& s = ctx->slots[slots[q]];
       int par = 0;
@@ -1050,13 +1055,15 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
         int rmax = 0;
         for (int nb : {0, 2, 3, 4})
           if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
+        // 2-opt rows i = 1 .. N-3 and diagonal rows 0 .. ceil(N/2) - 1
+        if (two) rmax = std::max({rmax, s.N - 3, rows_of(1, 0, s.N)});
         if (rmax > 0) {
           const int ncols = (s.N - 2 + TC - 1) / TC;
           segs.push_back(SweepSeg{slots[q], 0, (int)fixed, total, ncols, par});
           total += (rmax + TR - 1) / TR * ncols;
         }
       }
-      if (nb_mask & 2u) {
+      if ((nb_mask & 2u) && !two) {
         const int r = rows_of(1, 0, s.N);
         if (r > 0 && s.N >= 4) {
           const int ncols = (s.N - 3 + TC - 1) / TC;
@@ -1077,10 +1084,13 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
       if (st) return st;
     }
     a.grid = total;
+    if (two) a.rowmap |= MLP_TILE_TWO;
     if (ctx->user_capture)  // replays fold into the same buffers: clear them in the graph
       HCK(mlp_launch_clear_rows(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs));
     if (persistent) {
-      a.grid = std::min(total, ctx->cu_count * kTileWgPerCu[tile]);
+      // the 16 x 1024 tiles with 2-opt hold more LDS: two per CU
+      const int per_cu = (two && tile == MLP_TILE_16x1024) ? 2 : kTileWgPerCu[tile];
+      a.grid = std::min(total, ctx->cu_count * per_cu);
       if (round_counter) {
         a.counter = round_counter;
       } else {
@@ -1376,6 +1386,19 @@ int mlp_eval_each(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t nb
     st = eval_impl(ctx, slots + q, 1, nb_mask, rowmap, 0, 0, nullptr);
     if (st) return st;
   }
+  return MLP_OK;
+}
+
+int mlp_launch_probe(mlp_ctx* ctx, int32_t slot, int32_t kind, int32_t grid, int32_t reps) {
+  NO_CAPTURE(ctx);
+  if (!ctx) return MLP_EINVAL;
+  if (!slot_ok(ctx, slot)) return fail(ctx, MLP_EINVAL, "slot %d does not exist", slot);
+  if (kind < 0 || kind > 3 || grid < 1 || grid > 65535 || reps < 0)
+    return fail(ctx, MLP_EINVAL, "bad probe arguments");
+  int st = ensure_i64(ctx, 2 * 8192);
+  if (st) return st;
+  for (int r = 0; r < reps; ++r)
+    HCK(mlp_launch_probe(ctx->stream, ctx->d_table, slot, kind, grid, ctx->d_i64));
   return MLP_OK;
 }
 

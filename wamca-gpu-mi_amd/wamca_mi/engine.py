@@ -328,3 +328,8 @@ class Engine:
     def copy_probe(self, nbytes, reps=1):
         """Device-to-device copies of nbytes (counter calibration)."""
         self._ck(self._L.mlp_copy_probe(self.h, int(nbytes), int(reps)))
+
+    def launch_probe(self, slot, kind, grid, reps=1):
+        """Fixed-cost probe launches (kind 0 empty .. 3 with a tile's loads
+        and atomics) with a sweep's launch shape."""
+        self._ck(self._L.mlp_launch_probe(self.h, slot, kind, grid, reps))
