@@ -236,6 +236,26 @@ int mlp_timer_stop(mlp_ctx* ctx, int32_t timer);
 int mlp_timer_elapsed(mlp_ctx* ctx, int32_t timer, float* ms);
 int mlp_timer_destroy(mlp_ctx* ctx, int32_t timer);
 
+/* mlp_eval for one tour held in device memory, with no allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  inst is an instance made beforehand; tour (N = n + 1
+ * ints), cost (int64), is and js (int32) are device arrays of the caller.
+ * The rows of every neighborhood in nb_mask (ids 0..4, in id order,
+ * rows_of(nb, rowmap, N) each) land in cost / is / js as mlp_get_rows returns
+ * them (INT64_MAX for a row without a move).  The slot's working arrays live
+ * in a file-scope device array of 4 MiB that each GPU holds until the process
+ * ends (tours up to MLP_STATIC_MAX_N positions; MLP_ERANGE beyond); the call
+ * only enqueues kernels on the context stream (no allocation, copy, memset or
+ * synchronisation), so it can be captured into a graph.  Every call on one
+ * GPU shares that GPU's array and keeps the last call's contents: calls that
+ * overlap on one GPU (two contexts, or two streams) race.  The tour is not
+ * checked (it is in device memory): a tour that is not [0, perm, 0] gives
+ * unspecified rows, but every access stays inside the instance. */
+#define MLP_STATIC_MAX_N 8192
+int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb_mask,
+                    int32_t rowmap, int64_t* cost, int32_t* is, int32_t* js);
+
 /* The reference's ADS buffer (mlads.h:38-74) for the slot's current tour, built
  * on the device: 128-byte header, coords, [client|weight], T, C[N][N] (uint32
  * with the reference's wrap-around).  bytes = 128 + 4*N*(N+3). */

@@ -1222,6 +1222,35 @@ __global__ void k_rows_moves64(const SlotDev* __restrict__ table, int id, int nb
     out[r] = move64_of(keys[r], nb, rowmap, s.N, r);
 }
 
+// Rows of the last evaluation of every neighborhood in mask (ids 0..4, in
+// id order, rows_of(nb) entries each) as cost (INT64_MAX without a move),
+// i and j: the records mlp_get_rows returns, written to device arrays.
+__global__ void k_rows_out(const SlotDev* __restrict__ table, int id, int mask, int rowmap,
+                           int total, int64_t* cost, int32_t* oi, int32_t* oj) {
+  const SlotDev& s = table[id];
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < total; e += gridDim.x * kBlock) {
+    int nb = 0, r = e;
+    for (; nb < MLP_NB_COUNT; ++nb) {
+      if (!((mask >> nb) & 1)) continue;
+      const int R = max(rows_of(nb, rowmap, s.N), 0);
+      if (r < R) break;
+      r -= R;
+    }
+    if (nb >= MLP_NB_COUNT) continue;
+    const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1);
+    const int64_t key = s.rows[nb][(par ? s.rowcap : 0) + r];
+    int i = 0, j = 0;
+    int64_t c = INT64_MAX;
+    if (key != MLP_KEY_NONE) {
+      c = key >> MLP_KEY_SHIFT;
+      move_ij(nb, rowmap, s.N, r, (int)(key & MLP_KEY_MASK), i, j);
+    }
+    cost[e] = c;
+    oi[e] = i;
+    oj[e] = j;
+  }
+}
+
 // TOTAL entries row * B + t of nb (ADS row map).
 __global__ void k_total_moves64(const int64_t* __restrict__ tk, int64_t M, int B, int nb, int N,
                                 uint64_t* out) {
@@ -2430,6 +2459,14 @@ hipError_t mlp_launch_rows_moves64(hipStream_t st, const SlotDev* table, int id,
   const int grid = std::max(1, std::min((R + kBlock - 1) / kBlock, 1024));
   hipLaunchKernelGGL(k_rows_moves64, dim3(grid), dim3(kBlock), 0, st, table, id, nb, rowmap, R,
                      out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_rows_out(hipStream_t st, const SlotDev* table, int id, int mask, int rowmap,
+                              int total, int64_t* cost, int32_t* i, int32_t* j) {
+  const int grid = std::max(1, std::min((total + kBlock - 1) / kBlock, 1024));
+  hipLaunchKernelGGL(k_rows_out, dim3(grid), dim3(kBlock), 0, st, table, id, mask, rowmap, total,
+                     cost, i, j);
   return hipGetLastError();
 }
 

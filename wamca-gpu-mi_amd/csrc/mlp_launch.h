@@ -51,6 +51,8 @@ hipError_t mlp_launch_ads(hipStream_t st, const SlotDev* table, int id, int N, u
 // MLMove64 words of the rows of the last evaluation of nb / of TOTAL entries.
 hipError_t mlp_launch_rows_moves64(hipStream_t st, const SlotDev* table, int id, int nb,
                                   int rowmap, int R, uint64_t* out);
+hipError_t mlp_launch_rows_out(hipStream_t st, const SlotDev* table, int id, int mask, int rowmap,
+                              int total, int64_t* cost, int32_t* i, int32_t* j);
 hipError_t mlp_launch_total_moves64(hipStream_t st, const int64_t* tk, int64_t M, int B, int nb,
                                    int N, uint64_t* out);
 hipError_t mlp_launch_dist_table(hipStream_t st, int s0, int count, int round, int32_t* out);

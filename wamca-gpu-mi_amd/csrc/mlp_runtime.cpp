@@ -16,6 +16,7 @@
 #include <string>
 #include <unordered_map>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "mlp_gpu.h"
@@ -135,6 +136,37 @@ SlotLayout slot_layout(int N, int nbs) {
 
 }  // namespace
 
+// Working arrays of mlp_eval_static (one slot's ADS prefix arrays, info words
+// and row buffers, behind its slot-table entry): a file-scope device array of
+// a fixed size, so that entry point allocates nothing.  Every GPU holds its
+// own copy until the process ends; its device address is looked up once per
+// device (mlp_ctx_create) and only that address is handed to kernels.
+#define MLP_STATIC_SCRATCH_BYTES (4 << 20)
+static __device__ __align__(256) char scratch[MLP_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_set[kMaxDevices] = {};
+
+// The calling thread's current device must be `device`.
+void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_set[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;  // retried by the next context on this device
+    }
+    g_scratch_ptr[device] = ptr;
+    g_scratch_set[device] = true;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 struct mlp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -191,6 +223,7 @@ struct mlp_ctx {
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
   int graph_uses = 0;
+  void* scratch = nullptr;  // device address of this GPU's copy of the static scratch array
   // caller-owned device buffers (mlp_dev_alloc) and event-pair timers
   std::vector<void*> dev_bufs;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timers;
@@ -423,6 +456,7 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
     delete ctx;
     return MLP_EDEVICE;
   }
+  ctx->scratch = scratch_address(device);  // device is current
   ctx->cu_count = prop.multiProcessorCount;
   ctx->hbm = (int64_t)prop.totalGlobalMem;
   snprintf(ctx->name, sizeof ctx->name, "%s", prop.gcnArchName);
@@ -2019,6 +2053,113 @@ int mlp_timer_destroy(mlp_ctx* ctx, int32_t timer) {
   hipEventDestroy(t.first);
   hipEventDestroy(t.second);
   t = {nullptr, nullptr};
+  return MLP_OK;
+}
+
+int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb_mask,
+                    int32_t rowmap, int64_t* cost, int32_t* is, int32_t* js) {
+  if (!ctx) return MLP_EINVAL;
+  if (inst < 0 || inst >= (int)ctx->insts.size() || !ctx->insts[inst].alive)
+    return fail(ctx, MLP_EINVAL, "instance %d does not exist", inst);
+  if (!tour || !cost || !is || !js) return fail(ctx, MLP_EINVAL, "NULL device array");
+  if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
+  if (nb_mask == 0 || (nb_mask & ~0x1fu))
+    return fail(ctx, MLP_EINVAL, "neighborhood mask must be a non-empty subset of 0x1f");
+  if (!ctx->scratch)
+    return fail(ctx, MLP_EDEVICE, "no static scratch address on device %d", ctx->device);
+  const Instance& in = ctx->insts[inst];
+  const int N = in.n + 1;
+  if (N > MLP_STATIC_MAX_N)
+    return fail(ctx, MLP_ERANGE, "N = %d exceeds MLP_STATIC_MAX_N = %d", N, MLP_STATIC_MAX_N);
+  // the scratch holds the slot-table entry, then the slot's block up to its
+  // merge arrays (no merge runs here)
+  const SlotLayout L = slot_layout(N, MLP_NB_COUNT);
+  const size_t head = align_up(sizeof(SlotDev), 256);
+  if (head + L.ms > (size_t)MLP_STATIC_SCRATCH_BYTES)
+    return fail(ctx, MLP_ERANGE, "N = %d does not fit the static scratch", N);
+  char* base = (char*)ctx->scratch;
+  SlotDev* table = (SlotDev*)base;
+  char* b = base + head;
+  SlotDev d{};
+  d.N = N;
+  d.n = in.n;
+  d.round = in.round;
+  d.maxd = in.maxd;
+  d.rowcap = row_cap(N);
+  d.xs = in.xs;
+  d.ys = in.ys;
+  d.tour = const_cast<int32_t*>(tour);  // read only: prep and the sweeps never write it
+  d.xy = (int32_t*)(b + L.xy);
+  d.w = (int32_t*)(b + L.w);
+  d.t32 = (int32_t*)(b + L.t32);
+  d.q32 = (int32_t*)(b + L.q32);
+  d.g32 = (int32_t*)(b + L.g32);
+  d.h32 = (int32_t*)(b + L.h32);
+  d.t64 = (int64_t*)(b + L.t64);
+  d.q64 = (int64_t*)(b + L.q64);
+  d.info = (int64_t*)(b + L.info);
+  d.nbs = MLP_NB_COUNT;
+  SlotInitRows rows{};
+  rows.nbs = d.nbs;
+  for (int nb = 0; nb < d.nbs; ++nb) {
+    d.rows[nb] = (int64_t*)(b + L.rows[nb]);
+    d.kept_cost[nb] = (int64_t*)(b + L.kc[nb]);
+    d.kept_ij[nb] = (int32_t*)(b + L.kij[nb]);
+    rows.start[nb] = (int64_t)(L.rows[nb] / 8);
+    rows.count[nb] = 2 * (int64_t)d.rowcap;
+  }
+  d.mscratch = nullptr;
+  // the block is cleared by a kernel (info zero, row buffers KEY_NONE) and
+  // the table entry written with it; then the ADS of the tour
+  HCK(mlp_launch_slot_init(ctx->stream, table, 0, d, (int64_t*)b, (int64_t)(L.ms / 8), rows));
+  SlotListInline one{};
+  one.id[0] = 0;
+  HCK(mlp_launch_prep(ctx->stream, table, one, nullptr, 1));
+  // every neighborhood writes buffer 1 (the first tile records it)
+  int total_rows = 0;
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
+    if (nb_mask & (1u << nb)) total_rows += std::max(0, rows_of(nb, rowmap, N));
+  if (rowmap == 0) {
+    int tile = ctx->tile_override;
+    if (tile < 0) tile = auto_tile_moves(ctx, tour_moves(N, nb_mask), 1, nb_mask);
+    int TR, TC;
+    mlp_tile_shape(tile, &TR, &TC);
+    SweepSegsInline segs{};
+    int nsegs = 0, total = 0;
+    const bool two = (nb_mask & 3u) == 3u && TR >= 8;
+    const uint32_t fixed = (nb_mask & 0x1du) | (two ? 2u : 0u);
+    int rmax = 0;
+    for (int nb : {0, 2, 3, 4})
+      if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, N));
+    if (two) rmax = std::max({rmax, N - 3, rows_of(1, 0, N)});
+    if (fixed && rmax > 0) {
+      const int ncols = (N - 2 + TC - 1) / TC;
+      segs.s[nsegs++] = SweepSeg{0, 0, (int)fixed, total, ncols, 0x1f};
+      total += (rmax + TR - 1) / TR * ncols;
+    }
+    if (two) tile |= MLP_TILE_TWO;
+    if ((nb_mask & 2u) && !two && rows_of(1, 0, N) > 0 && N >= 4) {
+      const int ncols = (N - 3 + TC - 1) / TC;
+      segs.s[nsegs++] = SweepSeg{0, 1, 2, total, ncols, 0x1f};
+      total += (rows_of(1, 0, N) + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * ncols;
+    }
+    if (total > 0)
+      HCK(mlp_launch_sweep(ctx->stream, tile, table, segs, nullptr, nsegs, total, nullptr, total, 0));
+  } else {
+    EvalSegsInline segs{};
+    int nsegs = 0, total = 0;
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      const int R = rows_of(nb, 1, N);
+      if (!(nb_mask & (1u << nb)) || R <= 0) continue;
+      segs.s[nsegs++] = EvalSeg{0, nb, R, total, 1};
+      total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
+    }
+    if (total > 0)
+      HCK(mlp_launch_eval(ctx->stream, table, segs, nullptr, nsegs, total, 1, nullptr, total, 0));
+  }
+  if (total_rows > 0)
+    HCK(mlp_launch_rows_out(ctx->stream, table, 0, (int)nb_mask, rowmap, total_rows, cost, is,
+                            js));
   return MLP_OK;
 }
 

@@ -69,6 +69,8 @@ _SIGS = {
     "mlp_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mlp_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "mlp_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "mlp_eval_static": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_int32,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
     "mlp_pool_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlp_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "mlp_server_destroy": (C.c_int, [C.c_void_p]),
