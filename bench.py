@@ -273,6 +273,23 @@ def dvnd_legs(eng, args):
             "ratio": sv["solves_per_s"] / pl["solves_per_s"],
             "identical_results": sv["latency"] == pl["latency"]}
     res["rate_sweep_100_1000"] = sweep
+    # the serving configuration's own stream: Poisson 50 .. 2000 jobs/s, n
+    # log-uniform in [50, 2000] with 1% clustered n = 18000 jobs (at least one
+    # per rate); about one second of arrivals per rate (60 .. 300 jobs)
+    sweep = {}
+    for rate in (50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0):
+        cnt = int(min(300, max(60, rate)))
+        jobs, meta = WL.stream_jobs_with_big(cnt, 14, lo=50, hi=2000, big_frac=0.01)
+        arr = WL.poisson_arrivals(rate, cnt, 42)
+        sv = serve.run_served(eng, jobs, arr)
+        pl = serve.run_plain(eng, jobs, arr)
+        sweep[str(int(rate))] = {
+            "jobs": cnt, "n18000_jobs": sum(m["n"] == 18000 for m in meta),
+            "served": {k: sv[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+            "plain": {k: pl[k] for k in ("solves_per_s", "p50_s", "p99_s", "makespan_s")},
+            "ratio": sv["solves_per_s"] / pl["solves_per_s"],
+            "identical_results": sv["latency"] == pl["latency"]}
+    res["rate_sweep_50_2000_big"] = sweep
     return res
 
 

@@ -110,7 +110,8 @@ struct SweepSeg {
 #define MLP_TILE_4x512 4
 #define MLP_TILE_COUNT 5
 // Launch flag beside the shape: the fixed-row tiles also evaluate 2-opt
-// (shapes of eight or more rows; the launch then has no 2-opt diagonal tiles).
+// (shapes of eight or more rows, 16 x 1024 then run as 12 x 1024; the launch
+// has no 2-opt diagonal tiles).
 #define MLP_TILE_TWO 0x100
 
 #define MLP_SEGS_INLINE 32

@@ -789,18 +789,21 @@ __device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
 // conflict bitmap.
 constexpr int kGmemSortCap = 8192;
 
+// The merge's small shared variables (counts, the long-move queue, the
+// prefix rebuild's scan carries).
+struct MergeShared {
+  int m, kept, lng;
+  int longq[kLongMoves];
+  PrepLds PL;
+};
+
+// Merge + apply of one slot by one workgroup; lds holds the working arrays
+// (mlp_merge_lds_bytes(pcap_l, ncap_l), or the sort buffer and bitmap of a
+// GMEM merge).
 template <bool GMEM>
-__global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
-                                                        SlotListInline list,
-                                                        const int32_t* list_dev, int nb,
-                                                        int rowmap, int pcap_l, int ncap_l,
-                                                        int skip_inactive, int tail) {
-  extern __shared__ int64_t lds[];
-  __shared__ int s_m, s_kept, s_long;
-  __shared__ int s_longq[kLongMoves];
-  __shared__ PrepLds PL;
-  const int id = list_at(list, list_dev, blockIdx.x);
-  const SlotDev& s = table[id];
+__device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap, int pcap_l,
+                                           int ncap_l, int skip_inactive, int tail, int64_t* lds,
+                                           MergeShared& S) {
   if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
   const int N = s.N, tid = threadIdx.x;
   const int R = rows_of(nb, rowmap, N);
@@ -822,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   uint16_t* ctxL = (uint16_t*)((uint32_t*)(xyc + ncap) + (ncap + 32) / 32 + 1);  // ctx of each improving row
   int64_t* sk = keys;  // the sorted keys (greedy, apply)
 
-  if (tid == 0) s_m = 0;
+  if (tid == 0) S.m = 0;
   for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
   __syncthreads();
   // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
@@ -846,13 +849,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
       xyc[c] = xq;
     }
     if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
-      const int at = atomicAdd(&s_m, 1);
+      const int at = atomicAdd(&S.m, 1);
       keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
       ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
     }
   }
   __syncthreads();
-  const int M = s_m;
+  const int M = S.m;
   if (M == 0) {
     if (tid == 0) {
       s.info[MLP_INFO_KEPT0 + nb] = 0;
@@ -908,19 +911,31 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
       default: break;
     }
 #undef MLP_GREEDY
-    if (tid == 0) s_kept = kept;
+    if (tid == 0) S.kept = kept;
   }
   __syncthreads();
-  const int K = s_kept;
+  const int K = S.kept;
   // 4. apply the kept moves, rebuild the prefix arrays
-  apply_kept(s, nb, K, [&](int q) { return sk[q]; }, told, tnew, s_long, s_longq, true);
-  prep_wg(s, tnew, PL, xyc);
+  apply_kept(s, nb, K, [&](int q) { return sk[q]; }, told, tnew, S.lng, S.longq, true);
+  prep_wg(s, tnew, S.PL, xyc);
   if (tid == 0) {
     s.info[MLP_INFO_KEPT0 + nb] = K;
     s.info[MLP_INFO_ROUND_MOVES] += K;
     s.info[MLP_INFO_TOTAL_MOVES] += K;
     if (tail) round_tail(s, tail - 1);
   }
+}
+
+template <bool GMEM>
+__global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
+                                                        SlotListInline list,
+                                                        const int32_t* list_dev, int nb,
+                                                        int rowmap, int pcap_l, int ncap_l,
+                                                        int skip_inactive, int tail) {
+  extern __shared__ int64_t lds[];
+  __shared__ MergeShared S;
+  const int id = list_at(list, list_dev, blockIdx.x);
+  merge_slot<GMEM>(table[id], nb, rowmap, pcap_l, ncap_l, skip_inactive, tail, lds, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -2283,7 +2298,7 @@ MLP_SWEEP_INST(8, 256, false)
 MLP_SWEEP_INST(4, 512, false)
 // shapes of eight or more rows also come with 2-opt evaluated in the fixed tiles
 MLP_SWEEP_INST(8, 512, true)
-MLP_SWEEP_INST(16, 1024, true)
+MLP_SWEEP_INST(12, 1024, true)
 MLP_SWEEP_INST(8, 256, true)
 
 // ===========================================================================
@@ -2347,7 +2362,7 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
       break;
     case MLP_TILE_4x512: MLP_SWEEP_LAUNCH(4, 512, false); break;
     default:
-      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(16, 1024, true);
+      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(12, 1024, true);
       else MLP_SWEEP_LAUNCH(16, 1024, false);
       break;
   }
@@ -2358,8 +2373,12 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
 void mlp_tile_shape(int tile, int* rows, int* cols) {
   static const int kRows[MLP_TILE_COUNT] = {4, 8, 16, 8, 4};
   static const int kCols[MLP_TILE_COUNT] = {256, 512, 1024, 256, 512};
-  *rows = kRows[tile];
-  *cols = kCols[tile];
+  const int t = tile & 0xff;
+  *rows = kRows[t];
+  *cols = kCols[t];
+  // with 2-opt in the tile the 16 x 1024 shape runs 12 rows (its LDS then
+  // still fits three workgroups per CU)
+  if ((tile & MLP_TILE_TWO) && t == MLP_TILE_16x1024) *rows = 12;
 }
 
 // keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16

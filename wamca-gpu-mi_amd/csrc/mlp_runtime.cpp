@@ -1072,6 +1072,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     // with swap in the launch, tiles of eight or more rows evaluate 2-opt
     // too (sharing swap's distance products) instead of 2-opt diagonal tiles
     const bool two = (nb_mask & 3u) == 3u && TR >= 8;
+    if (two) mlp_tile_shape(tile | MLP_TILE_TWO, &TR, &TC);
     std::vector<SweepSeg> segs;
     segs.reserve((size_t)count * 2);
     int total = 0;
@@ -1122,9 +1123,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     if (ctx->user_capture)  // replays fold into the same buffers: clear them in the graph
       HCK(mlp_launch_clear_rows(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs));
     if (persistent) {
-      // the 16 x 1024 tiles with 2-opt hold more LDS: two per CU
-      const int per_cu = (two && tile == MLP_TILE_16x1024) ? 2 : kTileWgPerCu[tile];
-      a.grid = std::min(total, ctx->cu_count * per_cu);
+      a.grid = std::min(total, ctx->cu_count * kTileWgPerCu[tile]);
       if (round_counter) {
         a.counter = round_counter;
       } else {
@@ -2127,6 +2126,7 @@ int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb
     SweepSegsInline segs{};
     int nsegs = 0, total = 0;
     const bool two = (nb_mask & 3u) == 3u && TR >= 8;
+    if (two) mlp_tile_shape(tile | MLP_TILE_TWO, &TR, &TC);
     const uint32_t fixed = (nb_mask & 0x1du) | (two ? 2u : 0u);
     int rmax = 0;
     for (int nb : {0, 2, 3, 4})

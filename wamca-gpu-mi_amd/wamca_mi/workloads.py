@@ -50,6 +50,20 @@ def dvnd_jobs(count: int, seed: int, lo: int = 50, hi: int = 2000, big_frac: flo
     return jobs, meta
 
 
+def stream_jobs_with_big(count: int, seed: int, lo: int = 50, hi: int = 2000,
+                         big_frac: float = 0.01, big_n: int = 18000, rnd: int = 1):
+    """``dvnd_jobs(count, seed, lo, hi, big_frac)``, with at least one job of
+    n = big_n: if the draws made none, job count // 2 becomes one (instance
+    seed as dvnd_jobs would give it)."""
+    jobs, meta = dvnd_jobs(count, seed, lo, hi, big_frac, big_n, rnd)
+    if not any(m["n"] == big_n for m in meta):
+        q = count // 2
+        iseed = seed * 100003 + q
+        jobs[q] = big_job(iseed, big_n, rnd)
+        meta[q] = {"n": big_n, "kind": "clustered", "seed": iseed}
+    return jobs, meta
+
+
 def big_job(iseed: int, n: int = 18000, rnd: int = 1):
     """A large stream job: clustered instance of seed iseed and the
     selection-based random-insertion start tour of the same seed."""
