@@ -98,6 +98,49 @@ def parity_check(eng, slots, rows, count):
     return bad
 
 
+def odc_leg(eng, slots, xs, ys, tours, steps, check=64):
+    """The same one-tour sweep under the ODC row map (the default of the
+    bestNeighbor and MLKernel surfaces): fixed-row tiles folding each move into
+    its ODC row, autotuned over the shapes of at most eight rows; the rows of
+    the first `check` tours are compared with the oracle's ODC rows."""
+    import numpy as np
+    from oracle import oracle as O
+    N = len(tours[0]) - 1
+    per_tour = sum(moves_per_nb(N).values())
+    tune = {}
+    for t in (0, 1, 3, 4):
+        eng.set_tile(t)
+        eng.eval_each(slots, 0x1F, 1)
+        eng.region_begin()
+        eng.eval_each(slots, 0x1F, 1)
+        tune[TILES[t]] = eng.region_end() * 1e3 / len(slots)
+    tile = min(tune, key=tune.get)
+    eng.set_tile([k for k, v in TILES.items() if v == tile][0])
+    eng.eval_each(slots, 0x1F, 1)
+    eng.sync()
+    eng.region_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.eval_each(slots, 0x1F, 1)
+    region_ms = eng.region_end()
+    dt = time.perf_counter() - t0
+    eng.set_tile(-1)
+    ref, _ = O.eval_batch_xy(0x1F, 1, xs, ys, 1, tours[:check])
+    none = np.iinfo(np.int64).max
+    bad = 0
+    for q in range(check):
+        for nb in range(5):
+            g = eng.rows(slots[q], nb)
+            c = ref[nb]["cost"][q]
+            has = c != none
+            bad += int(((g["cost"] != c) | (has & ((g["i"] != ref[nb]["i"][q]) |
+                                                   (g["j"] != ref[nb]["j"][q])))).sum())
+    launches = steps * len(slots)
+    return {"moves_per_s": launches * per_tour / dt, "rowmap": "ODC", "kernel": tile,
+            "avg_launch_us": region_ms * 1e3 / launches, "autotune_us_per_launch": tune,
+            "parity": {"tours": check, "mismatched_rows": bad}}
+
+
 def batched_cfg2(eng):
     """256 random-insertion tours (seeds 1..256) of one clustered n=784
     instance (20 clusters, sigma 40, centres in [0,2000]^2, seed 2), all five
@@ -230,6 +273,20 @@ def dvnd_legs(eng, args):
         "identical_results": sv["latency"] == pl["latency"],
         "oracle_matches": (sum(int(a == r["latency"] and b == r["rounds"]) for a, b, r in
                                zip(sv["latency"], sv["rounds"], fixtures["batched_64"]))
+                           if "batched_64" in fixtures else None)}
+    # the same jobs with every DVND round as one persistent launch
+    eng.dvnd_fused(True)
+    serve.run_served(eng, jobs[:8], [0.0] * 8)  # warm-up
+    sf = serve.run_served(eng, jobs, [0.0] * 64, rounds_per_step=4)
+    eng.dvnd_fused(False)
+    res["batched_64_fused"] = {
+        "jobs": 64, "form": "one persistent launch per DVND round",
+        "served_solves_per_s": sf["solves_per_s"], "plain_solves_per_s": pl["solves_per_s"],
+        "ratio": sf["solves_per_s"] / pl["solves_per_s"],
+        "served_makespan_s": sf["makespan_s"],
+        "identical_results": sf["latency"] == pl["latency"],
+        "oracle_matches": (sum(int(a == r["latency"] and b == r["rounds"]) for a, b, r in
+                               zip(sf["latency"], sf["rounds"], fixtures["batched_64"]))
                            if "batched_64" in fixtures else None)}
     # n log-uniform in [100, 1000]: all jobs at t = 0 (capacity) and a Poisson
     # stream; n in [50, 2000] with a fraction of clustered n = 18000 jobs
@@ -525,6 +582,7 @@ def main():
         "device": info,
     }
 
+    out["odc_one_tour"] = odc_leg(eng, slots, xs, ys, tours, args.steps)
     if not args.no_batched:
         # batched form: every tour in one launch, largest tiles
         eng.set_tile(2)

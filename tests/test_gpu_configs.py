@@ -128,21 +128,26 @@ def _check_dvnd(recs, tours, lat, rounds=None, what=""):
             assert int(rounds[q]) == rec["rounds"], (what, q, rec["n"])
 
 
-@pytest.mark.parametrize("chunk,graph,persistent", [(4, True, False), (3, True, True),
-                                                    (1, False, False)])
-def test_batched_64_uneven_instances(eng, chunk, graph, persistent):
+@pytest.mark.parametrize("chunk,graph,persistent,fused", [(4, True, False, False),
+                                                          (3, True, True, False),
+                                                          (1, False, False, False),
+                                                          (1, True, False, True),
+                                                          (4, True, False, True)])
+def test_batched_64_uneven_instances(eng, chunk, graph, persistent, fused):
     recs = _fixtures()["batched_64"]
     jobs, meta = WL.dvnd_jobs(64, 3)
     insts = [eng.instance(xs, ys, 1) for xs, ys, _ in jobs]
     S = [eng.slot(i) for i in insts]
     try:
         eng.dvnd_config(chunk, graph, persistent)
+        eng.dvnd_fused(fused)
         for q, (_, _, t0) in enumerate(jobs):
             assert _sha(t0) == recs[q]["start_sha256"]
         eng.set_tours(S, [j[2] for j in jobs])
         rounds = eng.dvnd(S, 0)
-        _check_dvnd(recs, eng.get_tours(S), eng.latency(S), rounds, f"chunk={chunk}")
+        _check_dvnd(recs, eng.get_tours(S), eng.latency(S), rounds, f"chunk={chunk} fused={fused}")
     finally:
+        eng.dvnd_fused(False)
         eng.dvnd_config(4, True, False)
         for s in S:
             eng.destroy_slot(s)

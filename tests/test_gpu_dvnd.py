@@ -196,3 +196,22 @@ def test_cached_round_graph_follows_neighborhood_count(eng):
         eng.dvnd_neighborhoods(5)
         eng.set_neighborhoods(5)
         eng.dvnd_config(4, True, False)
+
+
+@pytest.mark.parametrize("chunk,graph", [(1, False), (4, True), (3, True)])
+def test_fused_rounds_match_oracle(eng, chunk, graph):
+    """Each DVND round as one persistent launch (a queue of every slot's eval
+    tiles and merges with per-slot dependencies): final tours, latencies and
+    round counts equal the oracle's for every slot of an uneven batch."""
+    jobs = _jobs([8, 40, 97, 180, 333, 611, 1200], 950 + chunk)
+    eng.dvnd_config(chunk, graph, False)
+    eng.dvnd_fused(True)
+    try:
+        out = _run(eng, jobs, 0)
+    finally:
+        eng.dvnd_fused(False)
+        eng.dvnd_config(4, True, False)
+    for (xs, ys, W, t0), (tour, L, rounds) in zip(jobs, out):
+        tref, Lref, rref, _ = O.dvnd(W, t0, 0)
+        assert L == Lref and rounds == rref
+        assert np.array_equal(tour, tref)

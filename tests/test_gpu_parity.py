@@ -142,22 +142,23 @@ def test_clustered_and_tiny(eng):
         _check_rows(eng, s, W, t, nb, 0)
 
 
-def test_persistent_and_batched_equal_single(eng):
+@pytest.mark.parametrize("rowmap", [0, 1])
+def test_persistent_and_batched_equal_single(eng, rowmap):
     xs, ys = O.instance_uniform(300, 11)
     W = O.weights(xs, ys, 1)
     inst = eng.instance(xs, ys, 1)
     S = eng.slots(inst, 24)
     tours = [O.perm_tour(500 + q, 300) for q in range(24)]
     eng.set_tours(S, tours)
-    eng.eval(S, 0x1F, 0, persistent=True)
+    eng.eval(S, 0x1F, rowmap, persistent=True)
     batched = {(s, nb): eng.rows(s, nb)["cost"] for s in S for nb in range(5)}
     for q, s in enumerate(S):
-        eng.eval([s], 0x1F, 0)
+        eng.eval([s], 0x1F, rowmap)
         for nb in range(5):
             assert np.array_equal(eng.rows(s, nb)["cost"], batched[(s, nb)])
     for q in (0, 7, 23):
         for nb in range(5):
-            _check_rows(eng, S[q], W, tours[q], nb, 0)
+            _check_rows(eng, S[q], W, tours[q], nb, rowmap)
 
 
 def test_ads_materialized(eng):
@@ -312,7 +313,8 @@ def test_tile_shapes_and_row_buffers(eng, tile, n, span):
     s = eng.slot(inst)
     eng.set_tile(tile)
     try:
-        for rep, (rowmap, tseed) in enumerate([(0, 1), (0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (0, 3)]):
+        for rep, (rowmap, tseed) in enumerate([(0, 1), (0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (0, 3),
+                                               (1, 4), (1, 4), (1, 5)]):
             t = O.perm_tour(tseed + n, n)
             eng.set_tour(s, t)
             eng.eval([s], 0x1F, rowmap)
@@ -321,6 +323,9 @@ def test_tile_shapes_and_row_buffers(eng, tile, n, span):
             # a neighborhood subset leaves the others' buffers alone
             eng.eval([s], 0b00101, rowmap)
             for nb in (0, 2):
+                _check_rows(eng, s, W, t, nb, rowmap)
+            eng.eval([s], 0b11100, rowmap)
+            for nb in (2, 3, 4):
                 _check_rows(eng, s, W, t, nb, rowmap)
     finally:
         eng.set_tile(-1)

@@ -161,3 +161,17 @@ def test_blocked_job_does_not_hold_small_jobs(eng):
             assert res[q]["latency"] == Lref and np.array_equal(res[q]["tour"], tref), q
     finally:
         srv.close()
+
+
+def test_served_fused_rounds_equal_alone(eng):
+    """Served jobs whose rounds run as one persistent launch each: every
+    result equals the oracle's DVND of the job alone."""
+    ns = [60, 200, 90, 350, 140, 75, 260, 120, 500, 55]
+    jobs, refs = _jobs(ns, 9500)
+    arrivals = [0.0, 0.0, 0.001, 0.002, 0.002, 0.004, 0.006, 0.006, 0.008, 0.01]
+    eng.dvnd_fused(True)
+    try:
+        out = serve.run_served(eng, jobs, arrivals)
+    finally:
+        eng.dvnd_fused(False)
+    _check(out, refs)

@@ -113,6 +113,9 @@ struct SweepSeg {
 // (shapes of eight or more rows, 16 x 1024 then run as 12 x 1024; the launch
 // has no 2-opt diagonal tiles).
 #define MLP_TILE_TWO 0x100
+// ... or the ODC row map for swap and or-opt (shapes of at most eight rows;
+// 16 x 1024 then runs as 8 x 512).
+#define MLP_TILE_ODC 0x200
 
 #define MLP_SEGS_INLINE 32
 struct EvalSegsInline {
@@ -121,6 +124,28 @@ struct EvalSegsInline {
 struct SweepSegsInline {
   SweepSeg s[MLP_SEGS_INLINE];
 };
+
+// A fused DVND launch: `rounds` rounds of {eval nb, merge nb} for nb = 0..4
+// over `count` slots as one persistent grid draining a work queue.  Items are
+// numbered round-major, then by phase (eval swap, merge swap, eval 2-opt, ...,
+// merge or-opt-3), then by slot (and tile); an item only waits for items of
+// its own slot with smaller numbers (the merge of the previous phase, the
+// tiles of its own eval phase), so the queue always drains.
+struct FusedPlan {
+  int32_t count, per_round, rounds, tail, pcap, ncap;
+  int32_t phase[11];          // first item of each phase within a round; [10] = per_round
+  const int32_t* slots;       // [count]
+  const int32_t* prefix;      // [5][count + 1]: eval items of nb before slot q
+  const int32_t* tiles;       // [5][count]: eval tiles of nb for slot q
+  const int32_t* ncols;       // [count][2]: column tiles of the fixed-row / 2-opt tiles
+  int32_t group;              // tiles per eval item (one wait, one signal)
+  int32_t* sync;              // [4 + 2 count]: work, exits, timeout flag, -, done[count], merged[count]
+};
+#define MLP_FUSED_R 8
+#define MLP_FUSED_C 512
+#ifndef MLP_FUSED_G
+#define MLP_FUSED_G 8
+#endif
 
 #define MLP_SLOTS_INLINE 128
 // 64-bit word ranges of a slot block holding row buffers (set to KEY_NONE

@@ -212,6 +212,24 @@ __device__ __forceinline__ int64_t wave_argmin(int32_t bd, int32_t bc) {
                         : (int64_t)m * (int64_t)(1 << MLP_KEY_SHIFT) + (int64_t)c;
 }
 
+// Wave-wide maximum (DPP, as wave_min_i32).
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT32_MIN, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// (delta, index) with the minimum delta and, among equal ones, the highest
+// index (lanes without a move hold index -1); (INT32_MAX, -1) when none.
+__device__ __forceinline__ void wave_argmin_hi(int32_t bd, int32_t bc, int32_t& m, int32_t& c) {
+  m = wave_min_i32(bc >= 0 ? bd : INT32_MAX);
+  c = wave_max_i32((bc >= 0 && bd == m) ? bc : -1);
+}
+
 __device__ __forceinline__ int64_t wave_min64(int64_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -1517,10 +1535,21 @@ struct TwoLds {
 
 struct NoTwoLds {};
 
-template <int R, int C, bool TWO>
+// ODC row map inside a fixed-row tile: per-diagonal minimum keys of or-opt-k
+// (k = 1..3) at slot (j - i) - (q0 - b0 - R - 1).
+template <int R, int C>
+struct OdcLds {
+  int64_t dg[3][C + R];
+};
+
+// Tile kinds: plain, with 2-opt (TWO), ODC row map (ODC).
+constexpr int kPlain = 0, kTwo = 1, kOdc = 2;
+
+template <int R, int C, int X>
 struct FixedLds {
   TileLds<R, C> t;
-  std::conditional_t<TWO, TwoLds<R, C>, NoTwoLds> two;
+  std::conditional_t<X == kTwo, TwoLds<R, C>, NoTwoLds> two;
+  std::conditional_t<X == kOdc, OdcLds<R, C>, NoTwoLds> odc;
 };
 
 template <int C>
@@ -1528,9 +1557,9 @@ struct DiagLds {
   uint16_t e[MLP_2OPT_ROWS][2][C + 2];
 };
 
-template <int R, int C, bool TWO>
+template <int R, int C, int X>
 union SweepLds {
-  FixedLds<R, C, TWO> f;
+  FixedLds<R, C, X> f;
   DiagLds<C> d;
 };
 
@@ -1566,11 +1595,12 @@ __device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int l) {
 
 // Reset of the buffer holding the previous result (par ^ 1) by the tiles of
 // column 0; the first tile also clears both tails beyond this row map.
-__device__ __forceinline__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first) {
+__device__ __forceinline__ void reset_other(const SlotDev& s, int nb, int par, int r0, int r1, bool first,
+                                            int rowmap = 0) {
   int64_t* o = row_buf(s, nb, par ^ 1);
   for (int r = r0 + (int)threadIdx.x; r < r1; r += kBlock) o[r] = MLP_KEY_NONE;
   if (first) {
-    for (int r = rows_of(nb, 0, s.N) + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
+    for (int r = rows_of(nb, rowmap, s.N) + (int)threadIdx.x; r < s.rowcap; r += kBlock) {
       o[r] = MLP_KEY_NONE;
       s.rows[nb][(par ? s.rowcap : 0) + r] = MLP_KEY_NONE;
     }
@@ -1637,7 +1667,7 @@ __device__ __forceinline__ int64_t packed_row_key(int32_t bk, int jb0, int off) 
 // minimum in LDS; the tile flushes one key per diagonal afterwards.  Needs
 // the int32 2-opt path and T[N-1] < 2^24 (the j T[i] and i T[j] products in
 // 24-bit multiplies).
-template <int R, int C, int RW, bool TWO, typename TL>
+template <int R, int C, int RW, bool TWO, typename TL, bool ODC = false>
 __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
                           TL& T2, int parity, int wave, int lane, bool pk) {
   const int N = s.N;
@@ -1645,6 +1675,7 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
   const int rv = min(RW, N - 4 - b0 - r0);  // rows with i <= N-4
   const int rv2 = TWO ? min(RW, N - 3 - b0 - r0) : rv;  // 2-opt rows: i <= N-3
   if (rv2 <= 0) return;
+  if constexpr (ODC) pk = false;
   int32_t bd[RW], bc[RW];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
@@ -1660,6 +1691,20 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
       myT = (uint32_t)L.rt[o];
       myA = (uint32_t)T2.rg[o] - (uint32_t)i * myT;
     }
+  }
+  // ODC row map (mlkswap.cu:764-768): ADS row ar = b0 + r0 + r is the
+  // first part of ODC row ar (index j - 3, lowest j first on ties) while
+  // ar < rowsO, and the second part of ODC row N - 4 - ar (index N - 2 - j,
+  // highest j first) while N - 4 - ar < rowsO
+  const int rowsO = ((N + 1) >> 1) - 1;
+  int32_t be[RW], bf[RW];
+  bool up[RW], dn[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    be[r] = INT32_MAX;
+    bf[r] = -1;
+    up[r] = b0 + r0 + r < rowsO;
+    dn[r] = N - 4 - (b0 + r0 + r) < rowsO;
   }
   uint32_t Cs[RW], A2[RW], T2i[RW];
 #pragma unroll
@@ -1713,7 +1758,15 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
         if (FR || r < rv) {
           const uint32_t d = Cs[r] + cs + X + umul(a1 - 1u, c[r + 2]) +
                              umul(nj, V[r + 1][x - 1]);
-          if (PK) {
+          if constexpr (ODC) {
+            const bool ok = full || (j < jhi && j >= b0 + r0 + r + 3);
+            if (up[r]) take_min_if(bd[r], bc[r], d, j, ok);
+            if (dn[r]) {  // <= keeps the later (higher) column of equal deltas
+              const bool le = ok && (int32_t)d <= be[r];
+              bf[r] = le ? j : bf[r];
+              be[r] = le ? (int32_t)d : be[r];
+            }
+          } else if (PK) {
             const int32_t k = pack_key(d, it);
             if (full) bd[r] = min(bd[r], k);
             else bd[r] = (j < jhi && j >= b0 + r0 + r + 3) ? min(bd[r], k) : bd[r];
@@ -1740,6 +1793,28 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
     }
   };
   int64_t* out = row_buf(s, 0, parity & 1) + b0 + r0;
+  if constexpr (ODC) {
+    if (rv2 == RW) cols(Flag<true>{}, Flag<false>{});
+    else cols(Flag<false>{}, Flag<false>{});
+    int64_t* ob = row_buf(s, 0, parity & 1);
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int ar = b0 + r0 + r;
+      if (r < rv && up[r]) {
+        const int32_t m = wave_min_i32(bc[r] >= 0 ? bd[r] : INT32_MAX);
+        const int32_t c = wave_min_i32((bc[r] >= 0 && bd[r] == m) ? bc[r] : INT32_MAX);
+        if (lane == 0 && c != INT32_MAX)
+          fold_key(ob + ar, (int64_t)m * (int64_t)(1 << MLP_KEY_SHIFT) + (c - 3));
+      }
+      if (r < rv && dn[r]) {
+        int32_t m, c;
+        wave_argmin_hi(be[r], bf[r], m, c);
+        if (lane == 0 && c >= 0)
+          fold_key(ob + (N - 4 - ar), (int64_t)m * (int64_t)(1 << MLP_KEY_SHIFT) + (N - 2 - c));
+      }
+    }
+    return;
+  }
   if (pk) {
     if (rv2 == RW) cols(Flag<true>{}, Flag<true>{});
     else cols(Flag<false>{}, Flag<true>{});
@@ -1762,9 +1837,14 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
 // Or-opt-K rows of a tile, each wave owning RW rows.  Row r (by = b0+r):
 // forward i = by+1 for j >= by+2, backward i = by+2 for j <= by+1, j in
 // [1, N-K-1]; row-local index j - 1.
-template <int R, int C, int RW, int K>
+// ODC: the ODC row map's rows are diagonals (mlkoropt.cu:696-723): a forward
+// move (j > i) belongs to row j - i - 1 with index j - 2, a backward one with
+// i - j >= 2 to row N - K - 1 - (i - j) with index N - K - 1 - i (backward
+// moves by one position are not in that row map); each move's key is folded
+// into the tile's per-diagonal minimum dg[(j - i) - (q0 - b0 - R - 1)] in LDS.
+template <int R, int C, int RW, int K, bool ODC = false>
 __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
-                           int parity, int wave, int lane, bool pk) {
+                           int parity, int wave, int lane, bool pk, int64_t* dg = nullptr) {
   const int N = s.N;
   const float dr = round_dr(s.round);
   const int r0 = wave * RW;
@@ -1806,10 +1886,21 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
   }
   const uint16_t(*V)[C + 4] = L.v + r0;
   const int jhi = min(q1, N - K);
+  const int slo = q0 - b0 - R - 1;
   // PK: packed keys in bd (bc unused)
-  auto upd = [&](auto pkc, int r, uint32_t d, int it, int j, bool ok) {
-    if constexpr (decltype(pkc)::value) bd[r] = ok ? min(bd[r], pack_key(d, it)) : bd[r];
-    else take_min_if(bd[r], bc[r], d, j - 1, ok);
+  auto upd = [&](auto pkc, int r, uint32_t d, int it, int j, bool ok, bool fwd) {
+    if constexpr (ODC) {
+      const int i = fwd ? bb + r + 1 : bb + r + 2;
+      const int S = j - i;
+      if (ok && S != -1)
+        atomicMin((long long*)&dg[S - slo],
+                  (long long)((int64_t)(int32_t)d * (int64_t)(1 << MLP_KEY_SHIFT) +
+                              (S >= 1 ? j - 2 : N - K - 1 - i)));
+    } else if constexpr (decltype(pkc)::value) {
+      bd[r] = ok ? min(bd[r], pack_key(d, it)) : bd[r];
+    } else {
+      take_min_if(bd[r], bc[r], d, j - 1, ok);
+    }
   };
   auto cols = [&](auto fr, auto pkc) {
     constexpr bool FR = decltype(fr)::value;
@@ -1829,7 +1920,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
           if (FR || r < rv) {
             uint32_t d = RF[r] + CF + umul(nj - K, V[r + K][x + K]) + umul(nj, V[r + 1][x + K - 1]);
             if (K > 1) d += smul(jj, nSF[r]);
-            upd(pkc, r, d, it, j, true);
+            upd(pkc, r, d, it, j, true, true);
           }
         }
       } else if (allB) {
@@ -1839,7 +1930,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
           if (FR || r < rv) {
             uint32_t d = RB[r] + CB + umul(nj, V[r + 2][x - 1]) + umul(nj - K, V[r + K + 1][x]);
             if (K > 1) d += smul(jj, nSB[r]);
-            upd(pkc, r, d, it, j, true);
+            upd(pkc, r, d, it, j, true, false);
           }
         }
       } else {
@@ -1855,13 +1946,18 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
               dF += smul(jj, nSF[r]);
               dB += smul(jj, nSB[r]);
             }
-            upd(pkc, r, fwd ? dF : dB, it, j, j < jhi);
+            upd(pkc, r, fwd ? dF : dB, it, j, j < jhi, fwd);
           }
         }
       }
     }
   };
   int64_t* out = row_buf(s, K + 1, (parity >> (K + 1)) & 1) + bb;
+  if constexpr (ODC) {  // the tile flushes the diagonals afterwards
+    if (rv == RW) cols(Flag<true>{}, Flag<false>{});
+    else cols(Flag<false>{}, Flag<false>{});
+    return;
+  }
   if (pk) {
     if (rv == RW) cols(Flag<true>{}, Flag<true>{});
     else cols(Flag<false>{}, Flag<true>{});
@@ -1883,7 +1979,7 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
 
 // int64 fallback of a fixed-i tile (column tile 0 only): one wave per
 // (neighborhood, row) over the whole row.
-template <int R>
+template <int R, int RM = 0>
 __device__ __forceinline__ void sweep_fixed_wide(const SlotDev& s, int rb, int ct, int mask, int parity) {
   const int N = s.N, wave = wave_id(), lane = threadIdx.x % kWave;
   const int b0 = rb * R;
@@ -1891,14 +1987,14 @@ __device__ __forceinline__ void sweep_fixed_wide(const SlotDev& s, int rb, int c
 #pragma unroll
   for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
     if (nb != 1 && ((mask >> nb) & 1))
-      reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+      reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, RM, N)), rb == 0, RM);
   for (int sg = wave; sg < 4 * R; sg += kBlock / kWave) {
     const int g = sg / R, by = b0 + sg % R;
     const int nb = g == 0 ? 0 : g + 1;
-    if (!((mask >> nb) & 1) || by >= rows_of(nb, 0, N)) continue;
+    if (!((mask >> nb) & 1) || by >= rows_of(nb, RM, N)) continue;
     int64_t* out = row_buf(s, nb, (parity >> nb) & 1) + by;
-    if (nb == 0) eval_row<0, false>(s, nb, 0, by, lane, out);
-    else eval_row<2, false>(s, nb, 0, by, lane, out);
+    if (nb == 0) eval_row<0, false>(s, nb, RM, by, lane, out);
+    else eval_row<2, false>(s, nb, RM, by, lane, out);
   }
 }
 
@@ -1920,9 +2016,12 @@ __device__ __forceinline__ void two_rows_wide(const SlotDev& s, int rb, int ct, 
   }
 }
 
-template <int R, int C, bool TWO>
+template <int R, int C, int X>
 __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
-                            FixedLds<R, C, TWO>& F) {
+                            FixedLds<R, C, X>& F) {
+  constexpr bool TWO = X == kTwo;
+  constexpr bool ODC = X == kOdc;  // ODC row map for swap and or-opt
+  constexpr int RM = ODC ? 1 : 0;
   constexpr int RW = R / (kBlock / kWave);
   constexpr int FI = (C + 4 + kBlock - 1) / kBlock;  // fill iterations per thread
   TileLds<R, C>& L = F.t;
@@ -1962,14 +2061,14 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   if (TWO && (mask & 2) && !two)
     two_rows_wide<R>(s, rb, ct, parity, (pflags & MLP_P32_2OPT) != 0);
   if (!p32) {
-    sweep_fixed_wide<R>(s, rb, ct, mask, parity);
+    sweep_fixed_wide<R, RM>(s, rb, ct, mask, parity);
     return;
   }
   if (ct == 0) {
 #pragma unroll
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if (nb != 1 && ((mask >> nb) & 1))
-        reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, 0, N)), rb == 0);
+        reset_other(s, nb, (parity >> nb) & 1, b0, min(b0 + R, rows_of(nb, RM, N)), rb == 0, RM);
     if (two) reset_other(s, 1, (parity >> 1) & 1, b0, min(b0 + R, (N + 1) >> 1), rb == 0);
   }
   // swap alone covers j >= i + 2 only: a tile left of that diagonal is empty
@@ -1995,6 +2094,18 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     for (int q = tid; q < C + R; q += kBlock) F.two.dg[q] = MLP_KEY_NONE;
     if (tid < 4) F.two.dgx[tid] = MLP_KEY_NONE;
   }
+  if constexpr (ODC) {
+    for (int q = tid; q < 3 * (C + R); q += kBlock) F.odc.dg[q / (C + R)][q % (C + R)] = MLP_KEY_NONE;
+  }
+  // the or-opt diagonals of an ODC tile (unused otherwise)
+  int64_t* dg1 = nullptr;
+  int64_t* dg2 = nullptr;
+  int64_t* dg3 = nullptr;
+  if constexpr (ODC) {
+    dg1 = F.odc.dg[0];
+    dg2 = F.odc.dg[1];
+    dg3 = F.odc.dg[2];
+  }
   __syncthreads();
   if (!TWO && R == kBlock / kWave && __popc(mask & 0x1d) >= 2) {
     // Several neighborhoods on a four-row tile: wave w takes the w-th
@@ -2007,17 +2118,35 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
         if (q == 0) nb = b;
         --q;
       }
-    if (nb == 0) tile_swap<R, C, R, false>(s, b0, q0, q1, cbase, L, F.two, parity, 0, lane, pk);
-    else if (nb == 2) tile_oropt<R, C, R, 1>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
-    else if (nb == 3) tile_oropt<R, C, R, 2>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
-    else if (nb == 4) tile_oropt<R, C, R, 3>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk);
-    return;
+    if (nb == 0)
+      tile_swap<R, C, R, false, decltype(F.two), ODC>(s, b0, q0, q1, cbase, L, F.two, parity, 0,
+                                                      lane, pk);
+    else if (nb == 2) tile_oropt<R, C, R, 1, ODC>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk, dg1);
+    else if (nb == 3) tile_oropt<R, C, R, 2, ODC>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk, dg2);
+    else if (nb == 4) tile_oropt<R, C, R, 3, ODC>(s, b0, q0, q1, cbase, L, parity, 0, lane, pk, dg3);
+    if constexpr (!ODC) return;
+  } else {
+    if (two) tile_swap<R, C, RW, TWO>(s, b0, q0, q1, cbase, L, F.two, parity, wave, lane, pk);
+    else if (mask & 1)
+      tile_swap<R, C, RW, false, decltype(F.two), ODC>(s, b0, q0, q1, cbase, L, F.two, parity,
+                                                       wave, lane, pk);
+    if (mask & 4) tile_oropt<R, C, RW, 1, ODC>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk, dg1);
+    if (mask & 8) tile_oropt<R, C, RW, 2, ODC>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk, dg2);
+    if (mask & 16) tile_oropt<R, C, RW, 3, ODC>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk, dg3);
   }
-  if (two) tile_swap<R, C, RW, TWO>(s, b0, q0, q1, cbase, L, F.two, parity, wave, lane, pk);
-  else if (mask & 1) tile_swap<R, C, RW, false>(s, b0, q0, q1, cbase, L, F.two, parity, wave, lane, pk);
-  if (mask & 4) tile_oropt<R, C, RW, 1>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
-  if (mask & 8) tile_oropt<R, C, RW, 2>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
-  if (mask & 16) tile_oropt<R, C, RW, 3>(s, b0, q0, q1, cbase, L, parity, wave, lane, pk);
+  if constexpr (ODC) {
+    // one key per or-opt diagonal of the tile into its ODC row
+    __syncthreads();
+    const int slo = q0 - b0 - R - 1;
+    for (int q = tid; q < 3 * (C + R); q += kBlock) {
+      const int k = q / (C + R) + 1, sl = q % (C + R);
+      if (!((mask >> (k + 1)) & 1)) continue;
+      const int64_t key = F.odc.dg[k - 1][sl];
+      if (key == MLP_KEY_NONE) continue;
+      const int S = slo + sl;  // j - i
+      fold_key(row_buf(s, k + 1, (parity >> (k + 1)) & 1) + (S >= 1 ? S - 1 : N - k - 1 + S), key);
+    }
+  }
   if constexpr (TWO) {
     if (two) {
       // one key per diagonal of the tile into the 2-opt row buffer
@@ -2231,9 +2360,9 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
   return find_inline<SweepSeg>(inl, nsegs, item);
 }
 
-template <int R, int C, bool TWO>
+template <int R, int C, int X>
 __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds<R, C, TWO>& L, bool dvnd) {
+                           SweepLds<R, C, X>& L, bool dvnd) {
   const SlotDev& s = table[sg.slot];
   if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
   // bit nb of par: the buffer this sweep writes (the one not holding the last result)
@@ -2251,7 +2380,7 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
   }
-  if (sg.kind == 0) sweep_fixed<R, C, TWO>(s, rb, ct, sg.mask, par, L.f);
+  if (sg.kind == 0) sweep_fixed<R, C, X>(s, rb, ct, sg.mask, par, L.f);
   else sweep_2opt<C>(s, rb, ct, par, L.d);
 }
 
@@ -2262,17 +2391,17 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
 // depends on every tile being resident at once, and many-tour launches,
 // which gain from the occupancy.  The large shapes are held to three
 // workgroups per CU by their LDS tiles anyway.
-template <int R, int C, bool TWO>
+template <int R, int C, int X>
 __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter,
                                                   int skip_inactive) {
-  __shared__ SweepLds<R, C, TWO> L;
+  __shared__ SweepLds<R, C, X> L;
   if (!counter) {
     const int item = blockIdx.x;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C, TWO>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, X>(table, sg, item, L, skip_inactive != 0);
     return;
   }
   __shared__ int s_item;
@@ -2283,23 +2412,159 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     __syncthreads();
     if (item >= total) break;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C, TWO>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, X>(table, sg, item, L, skip_inactive != 0);
     __syncthreads();
   }
 }
 
-#define MLP_SWEEP_INST(R, C, TWO)                                                     \
-  template __global__ void k_sweep<R, C, TWO>(const SlotDev*, SweepSegsInline, const SweepSeg*, \
-                                              int, int, int*, int);
-MLP_SWEEP_INST(4, 256, false)
-MLP_SWEEP_INST(8, 512, false)
-MLP_SWEEP_INST(16, 1024, false)
-MLP_SWEEP_INST(8, 256, false)
-MLP_SWEEP_INST(4, 512, false)
+// ---------------------------------------------------------------------------
+// DVND rounds as one persistent launch (FusedPlan): each workgroup takes the
+// next item of the queue, waits until that item's own dependencies are done
+// (its slot's previous merge; for a merge, all tiles of its eval phase), runs
+// it, and publishes its completion.  Slots advance independently inside the
+// launch, so short tours do not wait for the longest tour of the batch at
+// every neighborhood.  Waits poll with a sleep and give up after about
+// 2^22 polls (the flag sync[2] is then set and reported by the host), so the
+// grid always exits.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ int sync_load(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0 waits until *p >= target, then acquires at agent scope; the
+// barrier orders every thread's later reads after that acquire.  One thread
+// fences for the workgroup: an agent-scope fence is a cache invalidation (and
+// a release a cache write-back) of the whole XCD, and the workgroup's waves
+// share one CU's caches.
+__device__ __forceinline__ void wait_at_least(const int32_t* p, int target, int32_t* timeout) {
+  if (threadIdx.x == 0) {
+    int polls = 0;
+    while (sync_load(p) < target) {
+      __builtin_amdgcn_s_sleep(8);
+      if (++polls > (1 << 22)) {
+        atomicExch(timeout, 1);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);  // the invalidation has completed
+  }
+  __syncthreads();
+}
+
+// Every wave waits for its own stores and atomics to complete, the barrier
+// gathers the workgroup, and thread 0 releases at agent scope and counts the
+// item.
+__device__ __forceinline__ void signal_done(int32_t* p) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);  // the write-back has completed
+    __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+}  // namespace
+
+template <int R, int C>
+__global__ __launch_bounds__(kBlock) void k_dvnd_fused(const SlotDev* __restrict__ table,
+                                                       FusedPlan P) {
+  extern __shared__ int64_t dyn[];
+  __shared__ MergeShared MS;
+  __shared__ int s_item;
+  SweepLds<R, C, kPlain>& L = *reinterpret_cast<SweepLds<R, C, kPlain>*>(dyn);
+  int32_t* work = P.sync;
+  int32_t* exits = P.sync + 1;
+  int32_t* timeout = P.sync + 2;
+  int32_t* done = P.sync + 4;
+  int32_t* merged = done + P.count;
+  const int total = P.per_round * P.rounds;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(work, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    const int rr = item / P.per_round, x = item - rr * P.per_round;
+    int ph = 0;
+    while (ph < 9 && x >= P.phase[ph + 1]) ++ph;
+    const int nb = ph >> 1;
+    if (ph & 1) {  // merge + apply of slot q
+      const int q = x - P.phase[ph];
+      int all = 0, upto = 0;
+      for (int b = 0; b < MLP_NB_COUNT; ++b) {
+        const int t = P.prefix[b * (P.count + 1) + q + 1] - P.prefix[b * (P.count + 1) + q];
+        all += t;
+        if (b <= nb) upto += t;
+      }
+      wait_at_least(merged + q, rr * MLP_NB_COUNT + nb, timeout);
+      wait_at_least(done + q, rr * all + upto, timeout);
+      merge_slot<false>(table[P.slots[q]], nb, 0, P.pcap, P.ncap, 1,
+                        nb == MLP_NB_COUNT - 1 ? P.tail : 0, dyn, MS);
+      signal_done(merged + q);
+    } else {  // up to P.group tiles of the evaluation of nb for slot q
+      const int e = x - P.phase[ph];
+      const int32_t* pre = P.prefix + nb * (P.count + 1);
+      int lo = 0, hi = P.count - 1;
+      while (lo < hi) {  // the slot whose tiles hold tile e
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      const int q = lo;
+      wait_at_least(merged + q, rr * MLP_NB_COUNT + nb, timeout);
+      SweepSeg sg;
+      sg.slot = P.slots[q];
+      sg.kind = nb == 1 ? 1 : 0;
+      sg.mask = 1 << nb;
+      sg.blk0 = 0;
+      sg.ncols = P.ncols[2 * q + (nb == 1 ? 1 : 0)];
+      sg.parity = 0;
+      const int g = e - pre[q];
+      const int t1 = min((g + 1) * P.group, P.tiles[nb * P.count + q]);
+      for (int t = g * P.group; t < t1; ++t) {
+        sweep_item<R, C, kPlain>(table, sg, t, L, true);
+        __syncthreads();  // the next tile reuses the LDS
+      }
+      signal_done(done + q);
+    }
+    __syncthreads();
+  }
+  // the last workgroup out clears the counters for the next launch
+  if (threadIdx.x == 0) s_item = atomicAdd(exits, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (s_item) {
+    __threadfence();
+    for (int q = threadIdx.x; q < 2 * P.count; q += kBlock) done[q] = 0;
+    if (threadIdx.x == 0) {
+      *work = 0;
+      *exits = 0;
+    }
+  }
+}
+
+template __global__ void k_dvnd_fused<MLP_FUSED_R, MLP_FUSED_C>(const SlotDev*, FusedPlan);
+
+#define MLP_SWEEP_INST(R, C, X)                                                       \
+  template __global__ void k_sweep<R, C, X>(const SlotDev*, SweepSegsInline, const SweepSeg*,   \
+                                            int, int, int*, int);
+MLP_SWEEP_INST(4, 256, kPlain)
+MLP_SWEEP_INST(8, 512, kPlain)
+MLP_SWEEP_INST(16, 1024, kPlain)
+MLP_SWEEP_INST(8, 256, kPlain)
+MLP_SWEEP_INST(4, 512, kPlain)
 // shapes of eight or more rows also come with 2-opt evaluated in the fixed tiles
-MLP_SWEEP_INST(8, 512, true)
-MLP_SWEEP_INST(12, 1024, true)
-MLP_SWEEP_INST(8, 256, true)
+MLP_SWEEP_INST(8, 512, kTwo)
+MLP_SWEEP_INST(12, 1024, kTwo)
+MLP_SWEEP_INST(8, 256, kTwo)
+// the ODC row map on the shapes of at most eight rows
+MLP_SWEEP_INST(4, 256, kOdc)
+MLP_SWEEP_INST(8, 512, kOdc)
+MLP_SWEEP_INST(8, 256, kOdc)
+MLP_SWEEP_INST(4, 512, kOdc)
 
 // ===========================================================================
 // host-side launch wrappers (called by the C ABI runtime in mlp_runtime.cpp)
@@ -2346,27 +2611,37 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid, int skip_inactive) {
-  // MLP_TILE_TWO: fixed-row tiles with 2-opt (mask bit 1), the shapes' TWO variants
-#define MLP_SWEEP_LAUNCH(R, C, TWO)                                                            \
-  hipLaunchKernelGGL((k_sweep<R, C, TWO>), dim3(grid), dim3(kBlock), 0, st, table, inl,        \
+  // MLP_TILE_TWO: fixed-row tiles with 2-opt (mask bit 1), the shapes' TWO
+  // variants; MLP_TILE_ODC: the ODC row map (16 x 1024 runs as 8 x 512)
+#define MLP_SWEEP_LAUNCH(R, C, X)                                                              \
+  hipLaunchKernelGGL((k_sweep<R, C, X>), dim3(grid), dim3(kBlock), 0, st, table, inl,          \
                      segs_dev, nsegs, total, counter, skip_inactive)
+  const bool two = (tile & MLP_TILE_TWO) != 0, odc = (tile & MLP_TILE_ODC) != 0;
   switch (tile & 0xff) {
-    case MLP_TILE_4x256: MLP_SWEEP_LAUNCH(4, 256, false); break;
+    case MLP_TILE_4x256:
+      if (odc) MLP_SWEEP_LAUNCH(4, 256, kOdc);
+      else MLP_SWEEP_LAUNCH(4, 256, kPlain);
+      break;
     case MLP_TILE_8x512:
-      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(8, 512, true);
-      else MLP_SWEEP_LAUNCH(8, 512, false);
+      if (two) MLP_SWEEP_LAUNCH(8, 512, kTwo);
+      else if (odc) MLP_SWEEP_LAUNCH(8, 512, kOdc);
+      else MLP_SWEEP_LAUNCH(8, 512, kPlain);
       break;
     case MLP_TILE_8x256:
-      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(8, 256, true);
-      else MLP_SWEEP_LAUNCH(8, 256, false);
+      if (two) MLP_SWEEP_LAUNCH(8, 256, kTwo);
+      else if (odc) MLP_SWEEP_LAUNCH(8, 256, kOdc);
+      else MLP_SWEEP_LAUNCH(8, 256, kPlain);
       break;
-    case MLP_TILE_4x512: MLP_SWEEP_LAUNCH(4, 512, false); break;
+    case MLP_TILE_4x512:
+      if (odc) MLP_SWEEP_LAUNCH(4, 512, kOdc);
+      else MLP_SWEEP_LAUNCH(4, 512, kPlain);
+      break;
     default:
-      if (tile & MLP_TILE_TWO) MLP_SWEEP_LAUNCH(12, 1024, true);
-      else MLP_SWEEP_LAUNCH(16, 1024, false);
+      if (two) MLP_SWEEP_LAUNCH(12, 1024, kTwo);
+      else if (odc) MLP_SWEEP_LAUNCH(8, 512, kOdc);
+      else MLP_SWEEP_LAUNCH(16, 1024, kPlain);
       break;
   }
-#undef MLP_SWEEP_LAUNCH
   return hipGetLastError();
 }
 
@@ -2379,6 +2654,10 @@ void mlp_tile_shape(int tile, int* rows, int* cols) {
   // with 2-opt in the tile the 16 x 1024 shape runs 12 rows (its LDS then
   // still fits three workgroups per CU)
   if ((tile & MLP_TILE_TWO) && t == MLP_TILE_16x1024) *rows = 12;
+  if ((tile & MLP_TILE_ODC) && t == MLP_TILE_16x1024) {
+    *rows = 8;
+    *cols = 512;
+  }
 }
 
 // keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16
@@ -2399,6 +2678,23 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds) {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGmemMergeLds);
   if (e != hipSuccess) (void)hipGetLastError();  // do not leave it for the next launch check
   return e;
+}
+
+size_t mlp_fused_lds_bytes(int pcap, int ncap) {
+  return std::max(sizeof(SweepLds<MLP_FUSED_R, MLP_FUSED_C, kPlain>), mlp_merge_lds_bytes(pcap, ncap));
+}
+
+hipError_t mlp_fused_setup(size_t max_dyn_lds) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_dvnd_fused<MLP_FUSED_R, MLP_FUSED_C>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+
+hipError_t mlp_launch_fused(hipStream_t st, const SlotDev* table, const FusedPlan& plan, int grid) {
+  hipLaunchKernelGGL((k_dvnd_fused<MLP_FUSED_R, MLP_FUSED_C>), dim3(grid), dim3(kBlock),
+                     mlp_fused_lds_bytes(plan.pcap, plan.ncap), st, table, plan);
+  return hipGetLastError();
 }
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,

@@ -30,6 +30,10 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds);
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
                                   int pcap, int ncap, int skip_inactive, int gmem, int tail);
+// Fused DVND rounds (FusedPlan): dynamic LDS of a launch, its limit, the launch.
+size_t mlp_fused_lds_bytes(int pcap, int ncap);
+hipError_t mlp_fused_setup(size_t max_dyn_lds);
+hipError_t mlp_launch_fused(hipStream_t st, const SlotDev* table, const FusedPlan& plan, int grid);
 // TOTAL mode: entries rows * B of nb, their sort keys, the merge + apply.
 size_t mlp_total_merge_lds_bytes(int N);
 size_t mlp_total_static_lds();

@@ -209,6 +209,9 @@ struct mlp_ctx {
   int dvnd_chunk = 4;
   int dvnd_graph = 1;
   int dvnd_persistent = 0;
+  int dvnd_fused = 0;         // a round is one persistent launch (k_dvnd_fused)
+  int32_t* d_sync = nullptr;  // its queue and dependency counters (zero between launches)
+  int sync_cap = 0;           // slots the counters cover
   bool capturing = false;     // the stream is being captured into a graph
   bool user_capture = false;  // ... by mlp_graph_begin
   std::vector<hipGraphExec_t> user_graphs;
@@ -469,6 +472,7 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
   if (lds > 160 * 1024) lds = 160 * 1024;
   ctx->max_merge_lds = lds > 8192 ? lds - 8192 : lds;
   if (mlp_merge_setup(ctx->max_merge_lds) != hipSuccess) ctx->max_merge_lds = 64 * 1024 - 8192;
+  (void)mlp_fused_setup(ctx->max_merge_lds);
   ctx->max_total_lds = ctx->max_merge_lds;
   if (mlp_total_setup(ctx->max_total_lds) != hipSuccess) ctx->max_total_lds = 64 * 1024 - 8192;
   (void)hipGetLastError();
@@ -495,6 +499,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   if (ctx->d_i64) hipFree(ctx->d_i64);
   if (ctx->h_i64) hipHostFree(ctx->h_i64);
   if (ctx->d_tstage) hipFree(ctx->d_tstage);
+  if (ctx->d_sync) hipFree(ctx->d_sync);
   if (ctx->h_lists) hipHostFree(ctx->h_lists);
   hipEventDestroy(ctx->ev0);
   hipEventDestroy(ctx->ev1);
@@ -831,6 +836,7 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
       const size_t cap = std::max<size_t>(total, 2 * ctx->d_tstage_cap + 4096);
       HCK(hipStreamSynchronize(ctx->stream));
       if (ctx->d_tstage) hipFree(ctx->d_tstage);
+  if (ctx->d_sync) hipFree(ctx->d_sync);
       ctx->d_tstage = nullptr;
       ctx->d_tstage_cap = 0;
       HCK(hipMalloc((void**)&ctx->d_tstage, cap * 4));
@@ -1061,17 +1067,21 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
   EvalArgs a{};
   a.skip = skip;
   int st = MLP_OK;
-  // ADS row map: the five tiled neighborhoods go to k_sweep; or-opt-k for
-  // k > 3 (ids 5..14) to the one-wave-per-row k_eval below
-  if (rowmap == 0 && (nb_mask & 0x1fu)) {
-    // ADS row map: fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles
+  // the five tiled neighborhoods go to k_sweep; or-opt-k for k > 3
+  // (ids 5..14) to the one-wave-per-row k_eval below
+  if (nb_mask & 0x1fu) {
+    // fixed-i tiles (swap + or-opt) and 2-opt diagonal tiles; under the ODC
+    // row map the fixed tiles fold each move into its ODC row (tiles of at
+    // most eight rows: 16 x 1024 runs as 8 x 512)
+    const bool odc = rowmap == 1;
     int tile = ctx->tile_override;
     if (tile < 0) tile = auto_tile(ctx, slots, count, nb_mask);
+    if (odc && tile == MLP_TILE_16x1024) tile = MLP_TILE_8x512;
     int TR, TC;
     mlp_tile_shape(tile, &TR, &TC);
     // with swap in the launch, tiles of eight or more rows evaluate 2-opt
     // too (sharing swap's distance products) instead of 2-opt diagonal tiles
-    const bool two = (nb_mask & 3u) == 3u && TR >= 8;
+    const bool two = !odc && (nb_mask & 3u) == 3u && TR >= 8;
     if (two) mlp_tile_shape(tile | MLP_TILE_TWO, &TR, &TC);
     std::vector<SweepSeg> segs;
     segs.reserve((size_t)count * 2);
@@ -1089,7 +1099,8 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
       if (fixed) {
         int rmax = 0;
         for (int nb : {0, 2, 3, 4})
-          if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, s.N));
+          if (fixed & (1u << nb))  // tiles of column 0 also reset the ODC rows
+            rmax = std::max({rmax, rows_of(nb, 0, s.N), odc ? rows_of(nb, 1, s.N) : 0});
         // 2-opt rows i = 1 .. N-3 and diagonal rows 0 .. ceil(N/2) - 1
         if (two) rmax = std::max({rmax, s.N - 3, rows_of(1, 0, s.N)});
         if (rmax > 0) {
@@ -1120,6 +1131,7 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     }
     a.grid = total;
     if (two) a.rowmap |= MLP_TILE_TWO;
+    if (odc) a.rowmap |= MLP_TILE_ODC;
     if (ctx->user_capture)  // replays fold into the same buffers: clear them in the graph
       HCK(mlp_launch_clear_rows(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs));
     if (persistent) {
@@ -1764,23 +1776,120 @@ int enqueue_round(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int
   return MLP_OK;
 }
 
+// The fused round kernel applies: ADS row map, the five neighborhoods, every
+// slot's merge in LDS, tours of at least 8 positions.
+bool fused_ok(const mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int* pcap,
+              int* ncap) {
+  if (!ctx->dvnd_fused || rowmap != 0 || ctx->dvnd_nbs != MLP_NB_COUNT || ctx->merge_force_global)
+    return false;
+  int p = 1, n = 1;
+  for (int q = 0; q < count; ++q) {
+    const int N = ctx->slots[slots[q]].N;
+    if (N < 8) return false;
+    p = std::max(p, merge_pcap(N));
+    n = std::max(n, N);
+  }
+  if (mlp_merge_lds_bytes(p, n) > ctx->max_merge_lds) return false;
+  *pcap = p;
+  *ncap = n;
+  return true;
+}
+
+// One fused launch of `rounds` DVND rounds over the slots.
+int enqueue_fused(mlp_ctx* ctx, const int32_t* slots, int count, int max_rounds, int rounds,
+                  int pcap, int ncap) {
+  constexpr int R = MLP_FUSED_R, C = MLP_FUSED_C;
+  // MLP_FUSED_G tiles per item: a workgroup takes them with one work-counter
+  // atomic, one dependency wait and one completion signal
+  constexpr int G = MLP_FUSED_G;
+  std::vector<int32_t> prefix((size_t)MLP_NB_COUNT * (count + 1)), ncols((size_t)2 * count),
+      ntiles((size_t)MLP_NB_COUNT * count);
+  FusedPlan P{};
+  P.count = count;
+  P.group = G;
+  P.rounds = rounds;
+  P.tail = max_rounds + 1;
+  P.pcap = pcap;
+  P.ncap = ncap;
+  int at = 0;
+  for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+    P.phase[2 * nb] = at;
+    int32_t* pre = prefix.data() + (size_t)nb * (count + 1);
+    pre[0] = 0;
+    for (int q = 0; q < count; ++q) {
+      const int N = ctx->slots[slots[q]].N;
+      const int nc = (N - 2 + C - 1) / C, nc2 = (N - 3 + C - 1) / C;
+      ncols[2 * (size_t)q] = nc;
+      ncols[2 * (size_t)q + 1] = nc2;
+      const int rows = rows_of(nb, 0, N);
+      int tiles = 0;
+      if (rows > 0) tiles = nb == 1 ? (rows + MLP_2OPT_ROWS - 1) / MLP_2OPT_ROWS * nc2
+                                    : (rows + R - 1) / R * nc;
+      ntiles[(size_t)nb * count + q] = tiles;
+      pre[q + 1] = pre[q] + (tiles + G - 1) / G;
+    }
+    at += pre[count];
+    P.phase[2 * nb + 1] = at;
+    at += count;
+  }
+  P.phase[10] = at;
+  P.per_round = at;
+  int st = device_list(ctx, slots, count, &P.slots);
+  if (st == MLP_OK) st = device_list(ctx, prefix.data(), (int)prefix.size(), &P.prefix);
+  if (st == MLP_OK) st = device_list(ctx, ncols.data(), (int)ncols.size(), &P.ncols);
+  if (st == MLP_OK) st = device_list(ctx, ntiles.data(), (int)ntiles.size(), &P.tiles);
+  if (st) return st;
+  if (count > ctx->sync_cap) {  // counters start at zero; each launch leaves them at zero
+    if (ctx->capturing) return fail(ctx, MLP_ESTATE, "fused round counters grown during a capture");
+    HCK(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_sync) HCK(hipFree(ctx->d_sync));
+    ctx->d_sync = nullptr;
+    ctx->sync_cap = 0;
+    const int cap = std::max(count, 2 * ctx->sync_cap + 64);
+    HCK(hipMalloc((void**)&ctx->d_sync, sizeof(int32_t) * (4 + 2 * (size_t)cap)));
+    HCK(hipMemset(ctx->d_sync, 0, sizeof(int32_t) * (4 + 2 * (size_t)cap)));
+    ctx->sync_cap = cap;
+  }
+  P.sync = ctx->d_sync;
+  const size_t lds = mlp_fused_lds_bytes(pcap, ncap) + 4096;
+  const int per_cu = std::max(1, std::min(4, (int)((160 * 1024) / lds)));
+  const int grid = std::max(1, std::min(P.per_round * rounds, ctx->cu_count * per_cu));
+  HCK(mlp_launch_fused(ctx->stream, ctx->d_table, P, grid));
+  for (int q = 0; q < count; ++q) {
+    Slot& s = ctx->slots[slots[q]];
+    s.parity_stale = true;
+    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
+      s.eval_rowmap[nb] = 0;
+      s.merged[nb] = true;
+    }
+  }
+  return MLP_OK;
+}
+
 // `rounds` DVND rounds over `slots`, as one captured graph when enabled.  A
 // chunk is captured the second time the same (slots, settings) is seen, so
 // every device list it needs already exists (no copies inside a capture).
+int enqueue_rounds(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
+                   int rounds) {
+  int pcap = 0, ncap = 0;
+  const bool fused = fused_ok(ctx, slots, count, rowmap, &pcap, &ncap);
+  for (int r = 0; r < rounds; ++r) {
+    const int st = fused ? enqueue_fused(ctx, slots, count, max_rounds, 1, pcap, ncap)
+                         : enqueue_round(ctx, slots, count, rowmap, max_rounds);
+    if (st) return st;
+  }
+  return MLP_OK;
+}
+
 int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
               int rounds) {
-  if (!ctx->dvnd_graph || ctx->user_capture) {
-    for (int r = 0; r < rounds; ++r) {
-      int st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
-      if (st) return st;
-    }
-    return MLP_OK;
-  }
+  if (!ctx->dvnd_graph || ctx->user_capture)
+    return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   int tile = ctx->tile_override;  // (the per-neighborhood shapes follow from the slots)
   // the captured launches embed the slot table address, every slot's id and
   // size (segment tables, merge capacities) and the settings
-  const int32_t hdr[8] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count,
-                          ctx->dvnd_nbs, ctx->merge_force_global ? 1 : 0};
+  const int32_t hdr[9] = {rowmap, max_rounds, rounds, ctx->dvnd_persistent, tile, count,
+                          ctx->dvnd_nbs, ctx->merge_force_global ? 1 : 0, ctx->dvnd_fused};
   std::string key((const char*)hdr, sizeof hdr);
   key.append((const char*)&ctx->d_table, sizeof ctx->d_table);
   key.append((const char*)slots, 4 * (size_t)count);
@@ -1807,27 +1916,18 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
       ctx->graph_exec = nullptr;
     }
     ctx->graph_key = key;
-    for (int r = 0; r < rounds; ++r) {
-      int st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
-      if (st) return st;
-    }
-    return MLP_OK;
+    return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   }
   hipGraph_t g = nullptr;
   HCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
-  int st = MLP_OK;
-  for (int r = 0; r < rounds && st == MLP_OK; ++r) st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
+  int st = enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   ctx->capturing = false;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
   if (st || ec != hipSuccess) {  // nothing captured ran: run the chunk eagerly instead
     if (g) hipGraphDestroy(g);
     ctx->graph_key.clear();
-    for (int r = 0; r < rounds; ++r) {
-      st = enqueue_round(ctx, slots, count, rowmap, max_rounds);
-      if (st) return st;
-    }
-    return MLP_OK;
+    return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   }
   const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
@@ -1873,6 +1973,12 @@ int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* 
   int st = check_slots(ctx, slots, count);
   if (st) return st;
   if (count == 0) return MLP_OK;
+  if (ctx->d_sync) {  // a fused round whose dependency wait gave up
+    int32_t flag = 0;
+    HCK(hipMemcpyAsync(&flag, ctx->d_sync + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    if (flag) return fail(ctx, MLP_EDEVICE, "a fused DVND round timed out waiting for its queue");
+  }
   std::vector<int64_t> a((size_t)count), r((size_t)count);
   st = gather_info(ctx, slots, count, MLP_INFO_ACTIVE, a.data(), MLP_INFO_ROUNDS, r.data());
   if (st) return st;
@@ -1890,6 +1996,12 @@ int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t pers
   ctx->dvnd_graph = use_graph ? 1 : 0;
   ctx->dvnd_persistent = persistent ? 1 : 0;
   return MLP_OK;
+}
+
+int mlp_dvnd_fused(mlp_ctx* ctx, int32_t on) {
+  if (!ctx) return MLP_EINVAL;
+  ctx->dvnd_fused = on ? 1 : 0;
+  return drop_dvnd_graph(ctx);
 }
 
 int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
@@ -2118,19 +2230,22 @@ int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb
   int total_rows = 0;
   for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
     if (nb_mask & (1u << nb)) total_rows += std::max(0, rows_of(nb, rowmap, N));
-  if (rowmap == 0) {
+  {
+    const bool odc = rowmap == 1;  // fixed tiles folding into ODC rows
     int tile = ctx->tile_override;
     if (tile < 0) tile = auto_tile_moves(ctx, tour_moves(N, nb_mask), 1, nb_mask);
+    if (odc && tile == MLP_TILE_16x1024) tile = MLP_TILE_8x512;
     int TR, TC;
     mlp_tile_shape(tile, &TR, &TC);
     SweepSegsInline segs{};
     int nsegs = 0, total = 0;
-    const bool two = (nb_mask & 3u) == 3u && TR >= 8;
+    const bool two = !odc && (nb_mask & 3u) == 3u && TR >= 8;
     if (two) mlp_tile_shape(tile | MLP_TILE_TWO, &TR, &TC);
     const uint32_t fixed = (nb_mask & 0x1du) | (two ? 2u : 0u);
     int rmax = 0;
     for (int nb : {0, 2, 3, 4})
-      if (fixed & (1u << nb)) rmax = std::max(rmax, rows_of(nb, 0, N));
+      if (fixed & (1u << nb))
+        rmax = std::max({rmax, rows_of(nb, 0, N), odc ? rows_of(nb, 1, N) : 0});
     if (two) rmax = std::max({rmax, N - 3, rows_of(1, 0, N)});
     if (fixed && rmax > 0) {
       const int ncols = (N - 2 + TC - 1) / TC;
@@ -2138,6 +2253,7 @@ int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb
       total += (rmax + TR - 1) / TR * ncols;
     }
     if (two) tile |= MLP_TILE_TWO;
+    if (odc) tile |= MLP_TILE_ODC;
     if ((nb_mask & 2u) && !two && rows_of(1, 0, N) > 0 && N >= 4) {
       const int ncols = (N - 3 + TC - 1) / TC;
       segs.s[nsegs++] = SweepSeg{0, 1, 2, total, ncols, 0x1f};
@@ -2145,17 +2261,6 @@ int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb
     }
     if (total > 0)
       HCK(mlp_launch_sweep(ctx->stream, tile, table, segs, nullptr, nsegs, total, nullptr, total, 0));
-  } else {
-    EvalSegsInline segs{};
-    int nsegs = 0, total = 0;
-    for (int nb = 0; nb < MLP_NB_COUNT; ++nb) {
-      const int R = rows_of(nb, 1, N);
-      if (!(nb_mask & (1u << nb)) || R <= 0) continue;
-      segs.s[nsegs++] = EvalSeg{0, nb, R, total, 1};
-      total += (R + MLP_EVAL_ROWS_PER_WG - 1) / MLP_EVAL_ROWS_PER_WG;
-    }
-    if (total > 0)
-      HCK(mlp_launch_eval(ctx->stream, table, segs, nullptr, nsegs, total, 1, nullptr, total, 0));
   }
   if (total_rows > 0)
     HCK(mlp_launch_rows_out(ctx->stream, table, 0, (int)nb_mask, rowmap, total_rows, cost, is,

@@ -237,6 +237,11 @@ class Engine:
         """Rounds per convergence check, HIP-graph replay of a chunk, persistent grids."""
         self._ck(self._L.mlp_dvnd_config(self.h, chunk, int(use_graph), int(persistent)))
 
+    def dvnd_fused(self, on=True):
+        """Each DVND round as one persistent launch (a dependency-ordered
+        queue of every slot's evaluation tiles and merges)."""
+        self._ck(self._L.mlp_dvnd_fused(self.h, int(on)))
+
     def dvnd_start(self, slots):
         slots = np.ascontiguousarray(slots, np.int32)
         self._ck(self._L.mlp_dvnd_start(self.h, slots, len(slots)))
