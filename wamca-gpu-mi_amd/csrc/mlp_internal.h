@@ -29,7 +29,9 @@ enum {
   MLP_INFO_TOTAL_MOVES = 4,  // moves applied since the tour was set
   MLP_INFO_ACTIVE = 5,    // 1 while a DVND run on this slot continues
   MLP_INFO_ROUNDS = 6,    // DVND rounds run since the tour was set
+  MLP_INFO_MCOUNT = 7,    // improving rows gathered by a many-workgroup merge (zero between merges)
   MLP_INFO_KEPT0 = 8,     // kept-move count of nb (8 + nb)
+  MLP_INFO_MG_APPLY = 23, // moves the last many-workgroup merge applied (its rebuild runs if > 0)
   MLP_INFO_PARITY0 = 24,  // row buffer (0/1) holding the last eval of nb (24 + nb)
   MLP_INFO_WORDS = 40
 };
@@ -141,6 +143,9 @@ struct FusedPlan {
   int32_t group;              // tiles per eval item (one wait, one signal)
   int32_t* sync;              // [4 + 2 count]: work, exits, timeout flag, -, done[count], merged[count]
 };
+// positions per workgroup of the many-workgroup merge stages (kBlock threads
+// x 8 positions)
+#define MLP_MG_CHUNK 2048
 #define MLP_FUSED_R 8
 #define MLP_FUSED_C 512
 #ifndef MLP_FUSED_G

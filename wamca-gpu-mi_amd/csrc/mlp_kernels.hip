@@ -7,11 +7,19 @@
 //                   O(1) integer cost per move, wave argmin on (delta, index)
 //                   [kernelSwap / kernel2Opt / kernelOrOpt, mlkswap.cu:54-332,
 //                   mlk2opt.cu:41-267, mlkoropt.cu:48-366]
+//   k_sweep         the same rows over LDS-tiled distance blocks: fixed-row
+//                   tiles (swap, or-opt-1/2/3, optionally 2-opt) folding into
+//                   either row map, 2-opt diagonal tiles
 //   k_merge_apply   stable sort of the improving rows, greedy non-overlapping
-//                   subset, parallel application, ADS rebuild
+//                   subset, parallel application, ADS rebuild, one workgroup
+//                   per slot with its working arrays in LDS
 //                   [MLKernel::mergeGPU mlkernel.cu:254-384 + testSwap/test2Opt/
 //                   testOrOpt :73-170 + applyMove mlkswap.cu:1066, mlk2opt.cu:725,
 //                   mlkoropt.cu:886]
+//   k_mg_*          the same merge for tours beyond LDS, its O(N) steps over
+//                   many workgroups per slot
+//   k_dvnd_fused    a DVND round as one persistent launch
+//   k_total_*       TOTAL mode (per-thread-slot entries, sort, merge)
 //   k_ads_*         the reference's materialised ADS buffer (mlads.h:38-74)
 //
 // Integer arithmetic throughout; the only floating-point step is the distance
@@ -384,60 +392,64 @@ struct PrepLds {
   int64_t lsum[kBlock / kWave];
 };
 
+// One segment of kBlock * kPrepPer positions from `base` (thread t owns
+// kPrepPer consecutive positions): w, T and Q (offset by carryT, carryQ,
+// the prefixes before the segment) and the 2-opt constants G, H; WRITE
+// false only sums.  segT / segQ receive the segment's sums of w and p w, lacc
+// gains the thread's share of sum_p T[p].  Ends with every thread past its
+// reads of P.wt.
 // xyc: packed coordinates indexed by client (the merge's copy of the old
 // tour's xy, so the new tour needs no gather from the instance arrays), or
 // nullptr to read the instance arrays.
-__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
-                        const int32_t* xyc = nullptr) {
+template <bool WRITE>
+__device__ __forceinline__ void prep_segment(const SlotDev& s, const int32_t* tour, const int32_t* xyc,
+                                             int base, int64_t carryT, int64_t carryQ, PrepLds& P,
+                                             int64_t& segT, int64_t& segQ, int64_t& lacc) {
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
-  int64_t lacc = 0;  // this thread's share of L = sum_p T[p]
-  if (tid == 0) {
-    P.carry[0] = 0;
-    P.carry[1] = 0;
+  const int p0 = base + tid * kPrepPer;
+  // xy of positions p0 - 1 .. p0 + kPrepPer (clamped into the tour)
+  int xy[kPrepPer + 2];
+#pragma unroll
+  for (int e = 0; e < kPrepPer + 2; ++e) {
+    // client ids clamped into the instance: a tour read from caller memory
+    // never indexes outside the coordinate arrays
+    const int c = min(max(tour[min(max(p0 - 1 + e, 0), N - 1)], 0), s.n - 1);
+    xy[e] = xyc ? xyc[c] : (ldg(s.xs + c) << 16) | ldg(s.ys + c);
   }
-  for (int base = 0; base < N; base += kBlock * kPrepPer) {
-    const int p0 = base + tid * kPrepPer;
-    // xy of positions p0 - 1 .. p0 + kPrepPer (clamped into the tour)
-    int xy[kPrepPer + 2];
+  int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
 #pragma unroll
-    for (int e = 0; e < kPrepPer + 2; ++e) {
-      // client ids clamped into the instance: a tour read from caller memory
-      // never indexes outside the coordinate arrays
-      const int c = min(max(tour[min(max(p0 - 1 + e, 0), N - 1)], 0), s.n - 1);
-      xy[e] = xyc ? xyc[c] : (ldg(s.xs + c) << 16) | ldg(s.ys + c);
-    }
-    int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
+  for (int e = 0; e <= kPrepPer; ++e) {
+    const int p = p0 + e;
+    w[e] = (p >= 1 && p < N) ? dist_xy(xy[e], xy[e + 1], dr) : 0;
+  }
+  int64_t lt[kPrepPer], lq[kPrepPer], at = 0, aq = 0;
 #pragma unroll
-    for (int e = 0; e <= kPrepPer; ++e) {
-      const int p = p0 + e;
-      w[e] = (p >= 1 && p < N) ? dist_xy(xy[e], xy[e + 1], dr) : 0;
-    }
-    int64_t lt[kPrepPer], lq[kPrepPer], at = 0, aq = 0;
+  for (int e = 0; e < kPrepPer; ++e) {
+    at += w[e];
+    aq += (int64_t)(p0 + e) * w[e];
+    lt[e] = at;
+    lq[e] = aq;
+  }
+  const int64_t it = wave_scan_incl(at, lane), iq = wave_scan_incl(aq, lane);
+  if (lane == kWave - 1) {
+    P.wt[wave][0] = it;
+    P.wt[wave][1] = iq;
+  }
+  __syncthreads();
+  int64_t offT = carryT + it - at, offQ = carryQ + iq - aq;
+  segT = 0;
+  segQ = 0;
 #pragma unroll
-    for (int e = 0; e < kPrepPer; ++e) {
-      at += w[e];
-      aq += (int64_t)(p0 + e) * w[e];
-      lt[e] = at;
-      lq[e] = aq;
+  for (int q = 0; q < kBlock / kWave; ++q) {
+    if (q < wave) {
+      offT += P.wt[q][0];
+      offQ += P.wt[q][1];
     }
-    const int64_t it = wave_scan_incl(at, lane), iq = wave_scan_incl(aq, lane);
-    if (lane == kWave - 1) {
-      P.wt[wave][0] = it;
-      P.wt[wave][1] = iq;
-    }
-    __syncthreads();
-    int64_t offT = P.carry[0] + it - at, offQ = P.carry[1] + iq - aq;
-    int64_t segT = 0, segQ = 0;
-#pragma unroll
-    for (int q = 0; q < kBlock / kWave; ++q) {
-      if (q < wave) {
-        offT += P.wt[q][0];
-        offQ += P.wt[q][1];
-      }
-      segT += P.wt[q][0];
-      segQ += P.wt[q][1];
-    }
+    segT += P.wt[q][0];
+    segQ += P.wt[q][1];
+  }
+  if (WRITE) {
 #pragma unroll
     for (int e = 0; e < kPrepPer; ++e) {
       const int p = p0 + e;
@@ -456,10 +468,45 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
         lacc += T;
       }
     }
-    __syncthreads();  // every thread has read carry and wt
+  }
+  __syncthreads();  // every thread has read wt
+}
+
+// The tour's T[N-1] and int32-path flags.
+__device__ __forceinline__ void prep_flags(const SlotDev& s, int64_t Ttot) {
+  // Moves computed mod 2^32 are exact while |delta| < 2^31.  Node-wise:
+  // a 2-opt reversal moves each reversed node's arrival by at most
+  // Ttot + 2 maxd, so |delta| <= N (Ttot + 6 maxd) + 3 Ttot; a swap changes
+  // 4 edges (|delta| <= 4 N maxd); an or-opt-k moves k nodes by at most
+  // Ttot each and every other node by at most (k + 4) maxd, so
+  // |delta| <= 3 Ttot + 7 N maxd for k <= 3.
+  const int64_t N = s.N, md = s.maxd;
+  const int64_t b2 = N * (Ttot + 6 * md) + 3 * Ttot;
+  const int64_t bso = 3 * Ttot + 8 * N * md;
+  s.info[MLP_INFO_TTOT] = Ttot;
+  s.info[MLP_INFO_PATH32] = (b2 < 2147483647LL ? MLP_P32_2OPT : 0) |
+                            (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0) |
+                            (bso < (1LL << 26) ? MLP_PK_SWAP_OROPT : 0) |
+                            (Ttot < (1LL << 24) ? MLP_T24 : 0);
+}
+
+// The whole tour's prefix arrays and info words by one workgroup.
+__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
+                        const int32_t* xyc = nullptr) {
+  const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  int64_t lacc = 0;  // this thread's share of L = sum_p T[p]
+  if (tid == 0) {
+    P.carry[0] = 0;
+    P.carry[1] = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < N; base += kBlock * kPrepPer) {
+    const int64_t cT = P.carry[0], cQ = P.carry[1];
+    int64_t segT, segQ;
+    prep_segment<true>(s, tour, xyc, base, cT, cQ, P, segT, segQ, lacc);
     if (tid == 0) {
-      P.carry[0] += segT;
-      P.carry[1] += segQ;
+      P.carry[0] = cT + segT;
+      P.carry[1] = cQ + segQ;
     }
     __syncthreads();
   }
@@ -469,22 +516,8 @@ __device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
   if (tid == 0) {
     int64_t L = 0;
     for (int q = 0; q < kBlock / kWave; ++q) L += P.lsum[q];
-    const int64_t Ttot = P.carry[0];  // T[N-1]: positions past N-1 add nothing
-    // Moves computed mod 2^32 are exact while |delta| < 2^31.  Node-wise:
-    // a 2-opt reversal moves each reversed node's arrival by at most
-    // Ttot + 2 maxd, so |delta| <= N (Ttot + 6 maxd) + 3 Ttot; a swap changes
-    // 4 edges (|delta| <= 4 N maxd); an or-opt-k moves k nodes by at most
-    // Ttot each and every other node by at most (k + 4) maxd, so
-    // |delta| <= 3 Ttot + 7 N maxd for k <= 3.
-    const int64_t md = s.maxd;
-    const int64_t b2 = (int64_t)N * (Ttot + 6 * md) + 3 * Ttot;
-    const int64_t bso = 3 * Ttot + 8 * (int64_t)N * md;
     s.info[MLP_INFO_LATENCY] = L;
-    s.info[MLP_INFO_TTOT] = Ttot;
-    s.info[MLP_INFO_PATH32] = (b2 < 2147483647LL ? MLP_P32_2OPT : 0) |
-                              (bso < 2147483647LL ? MLP_P32_SWAP_OROPT : 0) |
-                              (bso < (1LL << 26) ? MLP_PK_SWAP_OROPT : 0) |
-                              (Ttot < (1LL << 24) ? MLP_T24 : 0);
+    prep_flags(s, P.carry[0]);  // T[N-1]: positions past N-1 add nothing
   }
   __syncthreads();
 }
@@ -596,8 +629,16 @@ __global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ tab
 
 namespace {
 
-// Interval bitmaps of positions claimed by kept moves.
-__device__ __forceinline__ bool bits_any(const uint32_t* bits, int a, int b) {
+// Positions claimed by kept moves: one bit per position (claim_words0(n)
+// words), then one summary bit per word of those, set when the word holds a
+// claimed position, so that a test over a long interval reads one summary bit
+// per 32 words between its two edge words.
+__host__ __device__ constexpr int claim_words0(int n) { return (n + 32) / 32 + 1; }
+__host__ __device__ constexpr int claim_words(int n) {
+  return claim_words0(n) + claim_words0(n) / 32 + 1;
+}
+
+__device__ __forceinline__ bool range_any(const uint32_t* bits, int a, int b) {
   const int wa = a >> 5, wb = b >> 5;
   for (int w = wa; w <= wb; ++w) {
     uint32_t m = 0xffffffffu;
@@ -608,7 +649,7 @@ __device__ __forceinline__ bool bits_any(const uint32_t* bits, int a, int b) {
   return false;
 }
 
-__device__ __forceinline__ void bits_set(uint32_t* bits, int a, int b) {
+__device__ __forceinline__ void range_set(uint32_t* bits, int a, int b) {
   const int wa = a >> 5, wb = b >> 5;
   for (int w = wa; w <= wb; ++w) {
     uint32_t m = 0xffffffffu;
@@ -616,6 +657,20 @@ __device__ __forceinline__ void bits_set(uint32_t* bits, int a, int b) {
     if (w == wb) m &= 0xffffffffu >> (31 - (b & 31));
     atomicOr(&bits[w], m);
   }
+}
+
+// Is any position of [a, b] claimed?  (n: the tour's positions)
+__device__ __forceinline__ bool claims_any(const uint32_t* bits, int n, int a, int b) {
+  const int wa = a >> 5, wb = b >> 5;
+  if (wb - wa <= 2) return range_any(bits, a, b);
+  if ((bits[wa] & (0xffffffffu << (a & 31))) | (bits[wb] & (0xffffffffu >> (31 - (b & 31)))))
+    return true;
+  return range_any(bits + claim_words0(n), wa + 1, wb - 1);  // the whole words between
+}
+
+__device__ __forceinline__ void claims_set(uint32_t* bits, int n, int a, int b) {
+  range_set(bits, a, b);
+  range_set(bits + claim_words0(n), a >> 5, b >> 5);
 }
 
 // Positions a kept move rewrites, and their new sources (mlkswap.cu:1066-1106,
@@ -687,7 +742,7 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
     bool pre = valid;
     if (valid) {
       if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
-      else pre = !bits_any(bits, a, b);
+      else pre = !claims_any(bits, s.N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
     // conflicts with leaves the batch (the predicates are symmetric), so the
@@ -702,10 +757,10 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
     }
     if ((kept >> lane) & 1ull) {
       if (NB == 0) {
-        bits_set(bits, i - 1, i + 1);
-        bits_set(bits, j - 1, j + 1);
+        claims_set(bits, s.N, i - 1, i + 1);
+        claims_set(bits, s.N, j - 1, j + 1);
       } else {
-        bits_set(bits, a, b);
+        claims_set(bits, s.N, a, b);
       }
       const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
       keys[q] = ((int64_t)i << 32) | (uint32_t)j;  // q <= t: already consumed
@@ -729,7 +784,7 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
 template <typename IJ>
 __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ ij, int32_t* told,
                                            int32_t* tnew, int& s_long, int* s_longq,
-                                           bool loaded = false) {
+                                           bool loaded = false, bool copy_back = true) {
   const int N = s.N, tid = threadIdx.x, k = nb_k(nb);
   if (!loaded)
     for (int p = tid; p < N; p += kBlock) {
@@ -756,7 +811,8 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
     apply_span(nb, k, (int)(v >> 32), (int)(uint32_t)v, told, tnew, tid, kBlock);
   }
   __syncthreads();
-  for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
+  if (copy_back)
+    for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
 }
 
 }  // namespace
@@ -805,7 +861,7 @@ __device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
 // A merge whose working arrays are in global memory still sorts up to
 // kGmemSortCap improving rows in LDS; the rest of its dynamic LDS holds the
 // conflict bitmap.
-constexpr int kGmemSortCap = 8192;
+constexpr int kGmemSortCap = 16384;
 
 // The merge's small shared variables (counts, the long-move queue, the
 // prefix rebuild's scan carries).
@@ -815,77 +871,19 @@ struct MergeShared {
   PrepLds PL;
 };
 
-// Merge + apply of one slot by one workgroup; lds holds the working arrays
-// (mlp_merge_lds_bytes(pcap_l, ncap_l), or the sort buffer and bitmap of a
-// GMEM merge).
+// 2.-3. of a merge: sort of the M improving (cost, row) keys == stable sort
+// by cost of the row list, then the greedy filter on wave 0.  Returns the
+// kept count; sk points at the sorted keys, whose first entries then hold the
+// kept moves as (i << 32 | j).
 template <bool GMEM>
-__device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap, int pcap_l,
-                                           int ncap_l, int skip_inactive, int tail, int64_t* lds,
-                                           MergeShared& S) {
-  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
-  const int N = s.N, tid = threadIdx.x;
-  const int R = rows_of(nb, rowmap, N);
-  // working arrays in LDS (capacities of the launch), or in the slot's global
-  // scratch laid out for its own size
-  int pcap = pcap_l, ncap = ncap_l;
-  if (GMEM) {
-    pcap = 1;
-    while (pcap < s.rowcap) pcap <<= 1;
-    ncap = N;
-  }
-  int64_t* keys = GMEM ? (int64_t*)s.mscratch : lds;
-  int32_t* told = (int32_t*)(keys + pcap);
-  int32_t* tnew = told + ncap;
-  int32_t* xyc = tnew + ncap;  // packed coordinates by client
-  const int nwords = (N + 32) / 32 + 1;
-  // the conflict bitmap: in the LDS layout, or after the LDS sort buffer
-  uint32_t* bits = GMEM ? (uint32_t*)(lds + kGmemSortCap) : (uint32_t*)(xyc + ncap);
-  uint16_t* ctxL = (uint16_t*)((uint32_t*)(xyc + ncap) + (ncap + 32) / 32 + 1);  // ctx of each improving row
-  int64_t* sk = keys;  // the sorted keys (greedy, apply)
-
-  if (tid == 0) S.m = 0;
-  for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
-  __syncthreads();
-  // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
-  // wrote the buffer opposite info[PARITY0 + nb]; this merge commits it.
-  // The tour (for the apply step) and its coordinates (for the prefix
-  // rebuild, by client) are copied in the same loop, so their loads go out
-  // with the row keys'.
-  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
-  const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
-  if (skip_inactive) {
-    __syncthreads();  // every thread has read the old parity
-    if (tid == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
-  }
-  for (int q = tid; q < max(R, N); q += kBlock) {
-    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
-    const int c = q < N ? s.tour[q] : 0;
-    const int xq = q < N ? s.xy[q] : 0;
-    if (q < N) {
-      told[q] = c;
-      tnew[q] = c;
-      xyc[c] = xq;
-    }
-    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
-      const int at = atomicAdd(&S.m, 1);
-      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
-      ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
-    }
-  }
-  __syncthreads();
-  const int M = S.m;
-  if (M == 0) {
-    if (tid == 0) {
-      s.info[MLP_INFO_KEPT0 + nb] = 0;
-      if (tail) round_tail(s, tail - 1);
-    }
-    return;
-  }
-  // 2. sort by (cost, row) == stable sort by cost of the row list.  Up to
-  // one key per lane of a wave, keys in LDS: rank sort on wave 0, which also
-  // runs the greedy next (the keys are distinct; a key's rank is the number
-  // of smaller keys; every lane reads all keys before any lane writes).
-  // Otherwise a bitonic sort over the workgroup.
+__device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap, int N, int M,
+                                           int64_t* keys, int64_t* lds, const uint16_t* ctxL,
+                                           uint32_t* bits, MergeShared& S, int64_t*& sk) {
+  const int tid = threadIdx.x;
+  // Up to one key per lane of a wave, keys in LDS: rank sort on wave 0, which
+  // also runs the greedy next (the keys are distinct; a key's rank is the
+  // number of smaller keys; every lane reads all keys before any lane
+  // writes).  Otherwise a bitonic sort over the workgroup.
   if (!GMEM && M <= kWave) {
     if (tid < kWave) {
       const int64_t my = tid < M ? keys[tid] : 0;
@@ -932,7 +930,66 @@ __device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap,
     if (tid == 0) S.kept = kept;
   }
   __syncthreads();
-  const int K = S.kept;
+  return S.kept;
+}
+
+// Merge + apply of one slot by one workgroup; lds holds the working arrays
+// (mlp_merge_lds_bytes(pcap_l, ncap_l)).
+__device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap, int pcap_l,
+                                           int ncap_l, int skip_inactive, int tail, int64_t* lds,
+                                           MergeShared& S) {
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
+  const int N = s.N, tid = threadIdx.x;
+  const int R = rows_of(nb, rowmap, N);
+  const int pcap = pcap_l, ncap = ncap_l;  // capacities of the launch
+  int64_t* keys = lds;
+  int32_t* told = (int32_t*)(keys + pcap);
+  int32_t* tnew = told + ncap;
+  int32_t* xyc = tnew + ncap;  // packed coordinates by client
+  const int nwords = claim_words(N);
+  uint32_t* bits = (uint32_t*)(xyc + ncap);
+  uint16_t* ctxL = (uint16_t*)((uint32_t*)(xyc + ncap) + claim_words(ncap));  // ctx of each improving row
+  int64_t* sk = keys;  // the sorted keys (greedy, apply)
+
+  if (tid == 0) S.m = 0;
+  for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
+  __syncthreads();
+  // 1. improving rows -> (cost, row) keys.  Inside a DVND round the eval
+  // wrote the buffer opposite info[PARITY0 + nb]; this merge commits it.
+  // The tour (for the apply step) and its coordinates (for the prefix
+  // rebuild, by client) are copied in the same loop, so their loads go out
+  // with the row keys'.
+  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
+  const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
+  if (skip_inactive) {
+    __syncthreads();  // every thread has read the old parity
+    if (tid == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
+  }
+  for (int q = tid; q < max(R, N); q += kBlock) {
+    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
+    const int c = q < N ? s.tour[q] : 0;
+    const int xq = q < N ? s.xy[q] : 0;
+    if (q < N) {
+      told[q] = c;
+      tnew[q] = c;
+      xyc[c] = xq;
+    }
+    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
+      const int at = atomicAdd(&S.m, 1);
+      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
+      ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
+    }
+  }
+  __syncthreads();
+  const int M = S.m;
+  if (M == 0) {
+    if (tid == 0) {
+      s.info[MLP_INFO_KEPT0 + nb] = 0;
+      if (tail) round_tail(s, tail - 1);
+    }
+    return;
+  }
+  const int K = sort_greedy<false>(s, nb, rowmap, N, M, keys, lds, ctxL, bits, S, sk);
   // 4. apply the kept moves, rebuild the prefix arrays
   apply_kept(s, nb, K, [&](int q) { return sk[q]; }, told, tnew, S.lng, S.longq, true);
   prep_wg(s, tnew, S.PL, xyc);
@@ -944,7 +1001,6 @@ __device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap,
   }
 }
 
-template <bool GMEM>
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
@@ -953,7 +1009,187 @@ __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restric
   extern __shared__ int64_t lds[];
   __shared__ MergeShared S;
   const int id = list_at(list, list_dev, blockIdx.x);
-  merge_slot<GMEM>(table[id], nb, rowmap, pcap_l, ncap_l, skip_inactive, tail, lds, S);
+  merge_slot(table[id], nb, rowmap, pcap_l, ncap_l, skip_inactive, tail, lds, S);
+}
+
+// ---------------------------------------------------------------------------
+// Merge of tours whose working arrays exceed LDS, over many workgroups per
+// slot (grid.y = slot list, grid.x = MLP_MG_CHUNK-position segments):
+//   k_mg_compact  the improving rows as (cost, row) keys (gathered per
+//                 workgroup in LDS, then appended with one atomic), the tour
+//                 copied to told / tnew, the coordinates by client;
+//   k_mg_select   one workgroup per slot: sort, greedy, the kept moves'
+//                 spans written into tnew, the merge's and round's counters;
+//   k_mg_sums     per segment of the new tour: sums of w and p w;
+//   k_mg_rebuild  per segment: the prefix arrays with the preceding
+//                 segments' sums as carries, the tour, the latency (one
+//                 atomic add per segment), T[N-1] and the int32 flags.
+// The one-workgroup merge runs these steps one after another over the whole
+// tour; here the O(N) steps are spread over the GPU.
+static_assert(MLP_MG_CHUNK == kBlock * kPrepPer, "one segment per workgroup");
+
+struct MgArrays {
+  int64_t* keys;
+  int32_t* told;
+  int32_t* tnew;
+  int32_t* xyc;
+  uint16_t* ctxL;
+  int64_t* csum;  // [segments][2]
+};
+
+// The slot's global working arrays: the one-workgroup merge's layout
+// (keys[pcap] | told[N] | tnew[N] | xyc[N] | bitmap | ctx[pcap]), then the
+// segment sums.
+__device__ __forceinline__ MgArrays mg_arrays(const SlotDev& s) {
+  int pcap = 1;
+  while (pcap < s.rowcap) pcap <<= 1;
+  const int N = s.N;
+  MgArrays a;
+  a.keys = (int64_t*)s.mscratch;
+  a.told = (int32_t*)(a.keys + pcap);
+  a.tnew = a.told + N;
+  a.xyc = a.tnew + N;
+  a.ctxL = (uint16_t*)((uint32_t*)(a.xyc + N) + claim_words(N));
+  a.csum = (int64_t*)(((uintptr_t)(a.ctxL + pcap) + 7) & ~(uintptr_t)7);
+  return a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_compact(const SlotDev* __restrict__ table,
+                                                       SlotListInline list,
+                                                       const int32_t* list_dev, int nb, int rowmap,
+                                                       int skip_inactive) {
+  __shared__ int64_t lk[MLP_MG_CHUNK];
+  __shared__ int lm;
+  __shared__ int64_t lbase;
+  const SlotDev& s = table[list_at(list, list_dev, blockIdx.y)];
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) return;
+  const int N = s.N, tid = threadIdx.x, R = rows_of(nb, rowmap, N);
+  const int q0 = blockIdx.x * MLP_MG_CHUNK, q1 = min(q0 + MLP_MG_CHUNK, max(R, N));
+  if (q0 >= q1) return;
+  const MgArrays a = mg_arrays(s);
+  // the eval of a DVND round wrote the buffer opposite info[PARITY0 + nb]
+  // (k_mg_select commits it once every segment has read it)
+  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
+  const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
+  if (tid == 0) lm = 0;
+  __syncthreads();
+  for (int q = q0 + tid; q < q1; q += kBlock) {
+    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
+    if (q < N) {
+      const int c = s.tour[q];
+      a.told[q] = c;
+      a.tnew[q] = c;
+      a.xyc[c] = s.xy[q];
+    }
+    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
+      const int at = atomicAdd(&lm, 1);
+      lk[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
+      a.ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
+    }
+  }
+  __syncthreads();
+  const int m = lm;
+  if (m == 0) return;
+  if (tid == 0)
+    lbase = (int64_t)atomicAdd((unsigned long long*)&s.info[MLP_INFO_MCOUNT], (unsigned long long)m);
+  __syncthreads();
+  for (int t = tid; t < m; t += kBlock) a.keys[lbase + t] = lk[t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_select(const SlotDev* __restrict__ table,
+                                                      SlotListInline list, const int32_t* list_dev,
+                                                      int nb, int rowmap, int skip_inactive,
+                                                      int tail) {
+  extern __shared__ int64_t lds[];
+  __shared__ MergeShared S;
+  const SlotDev& s = table[list_at(list, list_dev, blockIdx.x)];
+  const int tid = threadIdx.x;
+  if (skip_inactive && s.info[MLP_INFO_ACTIVE] == 0) {
+    if (tid == 0) s.info[MLP_INFO_MG_APPLY] = 0;
+    return;
+  }
+  const int N = s.N;
+  const MgArrays a = mg_arrays(s);
+  const int M = (int)s.info[MLP_INFO_MCOUNT];
+  const int par = (int)(s.info[MLP_INFO_PARITY0 + nb] & 1) ^ (skip_inactive ? 1 : 0);
+  uint32_t* bits = (uint32_t*)(lds + kGmemSortCap);
+  const int nwords = claim_words(N);
+  for (int w = tid; w < nwords; w += kBlock) bits[w] = 0;
+  __syncthreads();  // every thread has read the count and the parity
+  if (tid == 0) {
+    s.info[MLP_INFO_MCOUNT] = 0;
+    if (skip_inactive) s.info[MLP_INFO_PARITY0 + nb] = par;
+  }
+  if (M == 0) {
+    if (tid == 0) {
+      s.info[MLP_INFO_KEPT0 + nb] = 0;
+      s.info[MLP_INFO_MG_APPLY] = 0;
+      if (tail) round_tail(s, tail - 1);
+    }
+    return;
+  }
+  int64_t* sk = a.keys;
+  const int K = sort_greedy<true>(s, nb, rowmap, N, M, a.keys, lds, a.ctxL, bits, S, sk);
+  // tnew becomes the new tour; k_mg_rebuild copies it back
+  apply_kept(s, nb, K, [&](int q) { return sk[q]; }, a.told, a.tnew, S.lng, S.longq, true, false);
+  if (tid == 0) {
+    s.info[MLP_INFO_KEPT0 + nb] = K;
+    s.info[MLP_INFO_ROUND_MOVES] += K;
+    s.info[MLP_INFO_TOTAL_MOVES] += K;
+    s.info[MLP_INFO_MG_APPLY] = K;
+    if (tail) round_tail(s, tail - 1);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_sums(const SlotDev* __restrict__ table,
+                                                    SlotListInline list, const int32_t* list_dev) {
+  __shared__ PrepLds P;
+  const SlotDev& s = table[list_at(list, list_dev, blockIdx.y)];
+  if (s.info[MLP_INFO_MG_APPLY] == 0) return;
+  const int base = blockIdx.x * MLP_MG_CHUNK;
+  if (base >= s.N) return;
+  const MgArrays a = mg_arrays(s);
+  int64_t segT, segQ, lacc = 0;
+  prep_segment<false>(s, a.tnew, a.xyc, base, 0, 0, P, segT, segQ, lacc);
+  if (threadIdx.x == 0) {
+    a.csum[2 * blockIdx.x] = segT;
+    a.csum[2 * blockIdx.x + 1] = segQ;
+    if (blockIdx.x == 0) s.info[MLP_INFO_LATENCY] = 0;  // k_mg_rebuild adds the segments
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_rebuild(const SlotDev* __restrict__ table,
+                                                       SlotListInline list,
+                                                       const int32_t* list_dev) {
+  __shared__ PrepLds P;
+  const SlotDev& s = table[list_at(list, list_dev, blockIdx.y)];
+  if (s.info[MLP_INFO_MG_APPLY] == 0) return;
+  const int N = s.N, base = blockIdx.x * MLP_MG_CHUNK;
+  if (base >= N) return;
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const MgArrays a = mg_arrays(s);
+  const int nseg = (N + MLP_MG_CHUNK - 1) / MLP_MG_CHUNK;
+  int64_t cT = 0, cQ = 0, Ttot = 0;
+  for (int c = 0; c < nseg; ++c) {
+    const int64_t t = a.csum[2 * c], q = a.csum[2 * c + 1];
+    if (c < (int)blockIdx.x) {
+      cT += t;
+      cQ += q;
+    }
+    Ttot += t;
+  }
+  int64_t segT, segQ, lacc = 0;
+  prep_segment<true>(s, a.tnew, a.xyc, base, cT, cQ, P, segT, segQ, lacc);
+  for (int p = base + tid; p < min(base + MLP_MG_CHUNK, N); p += kBlock) s.tour[p] = a.tnew[p];
+  const int64_t lw = wave_sum64(lacc);
+  if (lane == 0) P.lsum[wave] = lw;
+  __syncthreads();
+  if (tid == 0) {
+    int64_t L = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) L += P.lsum[q];
+    atomicAdd((unsigned long long*)&s.info[MLP_INFO_LATENCY], (unsigned long long)L);
+    if (blockIdx.x == 0) prep_flags(s, Ttot);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1075,7 +1311,7 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
     bool pre = valid;
     if (valid) {
       if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
-      else pre = !bits_any(bits, a, b);
+      else pre = !claims_any(bits, s.N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
     // conflicts with leaves the batch (the predicates are symmetric), so the
@@ -1090,10 +1326,10 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
     }
     if ((kept >> lane) & 1ull) {
       if (NB == 0) {
-        bits_set(bits, i - 1, i + 1);
-        bits_set(bits, j - 1, j + 1);
+        claims_set(bits, s.N, i - 1, i + 1);
+        claims_set(bits, s.N, j - 1, j + 1);
       } else {
-        bits_set(bits, a, b);
+        claims_set(bits, s.N, a, b);
       }
       const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
       s.kept_cost[NB][q] = T.cc[t];
@@ -1127,7 +1363,7 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
   const SlotDev& s = table[id];
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const int k = nb_k(nb);
-  const int nwords = (N + 32) / 32 + 1;
+  const int nwords = claim_words(N);
   uint32_t* bits = (uint32_t*)lds;
   int32_t* told = GMEM ? (int32_t*)s.mscratch : (int32_t*)(bits + nwords);
   int32_t* tnew = told + N;
@@ -1172,8 +1408,8 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
         ci[u] = i;
         cj[u] = j;
         if (nb == 0) pass = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
-        else if (nb == 1) pass = !bits_any(bits, i, j + 1);
-        else pass = !bits_any(bits, min(i, j), max(i, j) + k);
+        else if (nb == 1) pass = !claims_any(bits, N, i, j + 1);
+        else pass = !claims_any(bits, N, min(i, j), max(i, j) + k);
       } else {
         end = true;  // past the list or the first non-improving key
       }
@@ -2502,7 +2738,7 @@ __global__ __launch_bounds__(kBlock) void k_dvnd_fused(const SlotDev* __restrict
       }
       wait_at_least(merged + q, rr * MLP_NB_COUNT + nb, timeout);
       wait_at_least(done + q, rr * all + upto, timeout);
-      merge_slot<false>(table[P.slots[q]], nb, 0, P.pcap, P.ncap, 1,
+      merge_slot(table[P.slots[q]], nb, 0, P.pcap, P.ncap, 1,
                         nb == MLP_NB_COUNT - 1 ? P.tail : 0, dyn, MS);
       signal_done(merged + q);
     } else {  // up to P.group tiles of the evaluation of nb for slot q
@@ -2662,20 +2898,19 @@ void mlp_tile_shape(int tile, int* rows, int* cols) {
 
 // keys[pcap] int64 | told[ncap] | tnew[ncap] | bits | ctx[pcap] uint16
 size_t mlp_merge_lds_bytes(int pcap, int ncap) {
-  const int nwords = (ncap + 32) / 32 + 1;
-  return (size_t)pcap * 8 + (size_t)ncap * 12 + (size_t)nwords * 4 + (size_t)pcap * 2;
+  return (size_t)pcap * 8 + (size_t)ncap * 12 + (size_t)claim_words(ncap) * 4 + (size_t)pcap * 2;
 }
 
 // Dynamic LDS of a merge whose working arrays are in global memory: the
 // sort buffer and the conflict bitmap of the largest N.
-constexpr size_t kGmemMergeLds = kGmemSortCap * 8 + ((MLP_MAX_N + 32) / 32 + 1) * 4;
+constexpr size_t kGmemMergeLds = kGmemSortCap * 8 + claim_words(MLP_MAX_N) * 4;
 
 hipError_t mlp_merge_setup(size_t max_dyn_lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_merge_apply<false>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_merge_apply,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_dyn_lds);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_merge_apply<true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGmemMergeLds);
+    e = hipFuncSetAttribute((const void*)k_mg_select, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kGmemMergeLds);
   if (e != hipSuccess) (void)hipGetLastError();  // do not leave it for the next launch check
   return e;
 }
@@ -2699,19 +2934,27 @@ hipError_t mlp_launch_fused(hipStream_t st, const SlotDev* table, const FusedPla
 
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive, int gmem, int tail) {
-  if (gmem)
-    hipLaunchKernelGGL(k_merge_apply<true>, dim3(count), dim3(kBlock), kGmemMergeLds, st, table,
-                       inl, list_dev, nb, rowmap, pcap, ncap, skip_inactive, tail);
-  else
-    hipLaunchKernelGGL(k_merge_apply<false>, dim3(count), dim3(kBlock),
-                       mlp_merge_lds_bytes(pcap, ncap), st, table, inl, list_dev, nb, rowmap, pcap,
-                       ncap, skip_inactive, tail);
+                                  int pcap, int ncap, int skip_inactive, int tail) {
+  hipLaunchKernelGGL(k_merge_apply, dim3(count), dim3(kBlock), mlp_merge_lds_bytes(pcap, ncap), st,
+                     table, inl, list_dev, nb, rowmap, pcap, ncap, skip_inactive, tail);
+  return hipGetLastError();
+}
+
+hipError_t mlp_launch_merge_gmem(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                 const int32_t* list_dev, int count, int nb, int rowmap,
+                                 int skip_inactive, int tail, int max_n) {
+  const dim3 seg((max_n + MLP_MG_CHUNK - 1) / MLP_MG_CHUNK, count);
+  hipLaunchKernelGGL(k_mg_compact, seg, dim3(kBlock), 0, st, table, inl, list_dev, nb, rowmap,
+                     skip_inactive);
+  hipLaunchKernelGGL(k_mg_select, dim3(count), dim3(kBlock), kGmemMergeLds, st, table, inl,
+                     list_dev, nb, rowmap, skip_inactive, tail);
+  hipLaunchKernelGGL(k_mg_sums, seg, dim3(kBlock), 0, st, table, inl, list_dev);
+  hipLaunchKernelGGL(k_mg_rebuild, seg, dim3(kBlock), 0, st, table, inl, list_dev);
   return hipGetLastError();
 }
 
 size_t mlp_total_merge_lds_bytes(int N) {
-  return (size_t)((N + 32) / 32 + 1) * 4 + (size_t)N * 8;
+  return (size_t)claim_words(N) * 4 + (size_t)N * 8;
 }
 
 // The merge kernel's own static LDS (chunk arrays, prefix scratch) is below
@@ -2751,8 +2994,8 @@ hipError_t mlp_launch_total_merge(hipStream_t st, const SlotDev* table, int id, 
                                   int B, int64_t M, uint64_t bias, const uint64_t* ku,
                                   const int32_t* ve, int gmem) {
   if (gmem)
-    hipLaunchKernelGGL(k_total_merge<true>, dim3(1), dim3(kBlock),
-                       (size_t)((N + 32) / 32 + 1) * 4, st, table, id, nb, B, M, bias, ku, ve);
+    hipLaunchKernelGGL(k_total_merge<true>, dim3(1), dim3(kBlock), (size_t)claim_words(N) * 4,
+                       st, table, id, nb, B, M, bias, ku, ve);
   else
     hipLaunchKernelGGL(k_total_merge<false>, dim3(1), dim3(kBlock), mlp_total_merge_lds_bytes(N),
                        st, table, id, nb, B, M, bias, ku, ve);

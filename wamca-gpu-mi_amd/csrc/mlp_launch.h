@@ -29,7 +29,12 @@ hipError_t mlp_merge_setup(size_t max_dyn_lds);
 // tail > 0: the last merge of a DVND round also ends the round (max_rounds = tail - 1).
 hipError_t mlp_launch_merge_apply(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
                                   const int32_t* list_dev, int count, int nb, int rowmap,
-                                  int pcap, int ncap, int skip_inactive, int gmem, int tail);
+                                  int pcap, int ncap, int skip_inactive, int tail);
+// The merge of slots whose working arrays exceed LDS over many workgroups
+// per slot (four launches; max_n: the largest N among the slots).
+hipError_t mlp_launch_merge_gmem(hipStream_t st, const SlotDev* table, const SlotListInline& inl,
+                                 const int32_t* list_dev, int count, int nb, int rowmap,
+                                 int skip_inactive, int tail, int max_n);
 // Fused DVND rounds (FusedPlan): dynamic LDS of a launch, its limit, the launch.
 size_t mlp_fused_lds_bytes(int pcap, int ncap);
 hipError_t mlp_fused_setup(size_t max_dyn_lds);

@@ -98,6 +98,11 @@ int merge_pcap(int N) {
   return p;
 }
 size_t merge_bytes(int N) { return mlp_merge_lds_bytes(merge_pcap(N), N); }
+// The merge's working arrays in the slot's device memory: that layout, then
+// the many-workgroup merge's per-segment sums.
+size_t merge_gmem_bytes(int N) {
+  return merge_bytes(N) + 8 + 16 * (size_t)((N + MLP_MG_CHUNK - 1) / MLP_MG_CHUNK);
+}
 
 // Byte layout of one slot's device block.
 struct SlotLayout {
@@ -129,7 +134,7 @@ SlotLayout slot_layout(int N, int nbs) {
     L.kc[nb] = take(8 * r);
     L.kij[nb] = take(12 * r);
   }
-  L.ms = take(merge_bytes(N));  // merge working arrays in global memory
+  L.ms = take(merge_gmem_bytes(N));  // merge working arrays in global memory
   L.total = off;
   return L;
 }
@@ -1562,6 +1567,8 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
   for (int q = 0; q < count; ++q) ctx->slots[slots[q]].merged[nb] = true;
   if (ctx->user_capture)
     for (int q = 0; q < count; ++q) ctx->capture_ops.push_back({slots[q], 1, (uint32_t)nb, rowmap});
+  int gmem_n = 1;
+  for (int32_t id : in_gmem) gmem_n = std::max(gmem_n, ctx->slots[id].N);
   for (int g = 0; g < 2; ++g) {
     const std::vector<int32_t>& list = g ? in_gmem : in_lds;
     for (size_t q0 = 0; q0 < list.size(); q0 += 65535) {
@@ -1570,8 +1577,12 @@ static int merge_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t
       const int32_t* dev = nullptr;
       int st = slot_list(ctx, list.data() + q0, c, inl, &dev);
       if (st) return st;
-      HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
-                                 skip, g, tail));
+      if (g)  // working arrays in device memory: the O(N) steps over many workgroups
+        HCK(mlp_launch_merge_gmem(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, skip, tail,
+                                  gmem_n));
+      else
+        HCK(mlp_launch_merge_apply(ctx->stream, ctx->d_table, inl, dev, c, nb, rowmap, pcap, ncap,
+                                   skip, tail));
     }
   }
   return MLP_OK;

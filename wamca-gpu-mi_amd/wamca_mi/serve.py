@@ -158,11 +158,15 @@ def main():
     from .engine import Engine
     ap = argparse.ArgumentParser()
     ap.add_argument("--job", nargs=3, metavar=("KIND", "N", "SEED"), required=True)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each DVND round's kernels directly (no graph replay)")
     args = ap.parse_args()
     kind, n, seed = args.job[0], int(args.job[1]), int(args.job[2])
     xs, ys = I.clustered(n, seed) if kind == "clustered" else I.uniform(n, seed, 1000)
     tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, 1, seed)
     eng = Engine(0)
+    if args.no_graph:
+        eng.dvnd_config(4, False, False)
     t0 = time.perf_counter()
     out = run_plain(eng, [(xs, ys, tour)], [0.0])
     print(json.dumps({"kind": kind, "n": n, "seed": seed, "seconds": time.perf_counter() - t0,
