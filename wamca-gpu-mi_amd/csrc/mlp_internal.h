@@ -115,8 +115,10 @@ struct SweepSeg {
 // (shapes of eight or more rows, 16 x 1024 then run as 12 x 1024; the launch
 // has no 2-opt diagonal tiles).
 #define MLP_TILE_TWO 0x100
-// ... or the ODC row map for swap and or-opt (shapes of at most eight rows;
-// 16 x 1024 then runs as 8 x 512).
+// ... or the ODC row map for swap and or-opt (16 x 1024 then runs as
+// 8 x 512, and 8 x 256 as 16 x 128: an or-opt diagonal gets one key from
+// each tile it crosses, and all of a tour's tiles fold at once, so the row
+// count of the tile sets the contention on each ODC row).
 #define MLP_TILE_ODC 0x200
 
 #define MLP_SEGS_INLINE 32

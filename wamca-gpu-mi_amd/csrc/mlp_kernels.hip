@@ -1825,6 +1825,19 @@ __device__ __forceinline__ void fold_key(int64_t* p, int64_t key) {
   if (key != MLP_KEY_NONE) atomicMin((long long*)p, (long long)key);
 }
 
+// fold_key for rows that many tiles fold into (the diagonals of the ODC
+// or-opt rows and of 2-opt in fixed tiles, one key per tile the diagonal
+// crosses): the row's current value is read first and the atomic is issued
+// only by a key below it.  Values only decrease during the launch, so a stale
+// read costs at most an unneeded atomic.  It pays when tiles of a tour run
+// one after another (most keys then lose to the value already there); when
+// they all run at once every read still finds the reset value, and the read
+// only adds latency.
+__device__ __forceinline__ void fold_key_filtered(int64_t* p, int64_t key) {
+  if (key != MLP_KEY_NONE && key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMin((long long*)p, (long long)key);
+}
+
 __device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
@@ -2254,7 +2267,7 @@ __device__ __forceinline__ void two_rows_wide(const SlotDev& s, int rb, int ct, 
 
 template <int R, int C, int X>
 __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, int mask, int parity,
-                            FixedLds<R, C, X>& F) {
+                            FixedLds<R, C, X>& F, bool filtered) {
   constexpr bool TWO = X == kTwo;
   constexpr bool ODC = X == kOdc;  // ODC row map for swap and or-opt
   constexpr int RM = ODC ? 1 : 0;
@@ -2380,7 +2393,9 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
       const int64_t key = F.odc.dg[k - 1][sl];
       if (key == MLP_KEY_NONE) continue;
       const int S = slo + sl;  // j - i
-      fold_key(row_buf(s, k + 1, (parity >> (k + 1)) & 1) + (S >= 1 ? S - 1 : N - k - 1 + S), key);
+      int64_t* p = row_buf(s, k + 1, (parity >> (k + 1)) & 1) + (S >= 1 ? S - 1 : N - k - 1 + S);
+      if (filtered) fold_key_filtered(p, key);
+      else fold_key(p, key);
     }
   }
   if constexpr (TWO) {
@@ -2392,7 +2407,10 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
       for (int q = tid; q < C + R; q += kBlock) {
         const int64_t key = F.two.dg[q];
         const int D = dlo + q;
-        if (key != MLP_KEY_NONE) fold_key(out + (D <= rows2 ? D - 1 : N - 2 - D), key);
+        int64_t* p = out + (D <= rows2 ? D - 1 : N - 2 - D);
+        if (key == MLP_KEY_NONE) continue;
+        if (filtered) fold_key_filtered(p, key);
+        else fold_key(p, key);
       }
       if (tid < 4 && F.two.dgx[tid] != MLP_KEY_NONE)
         fold_key(out + (N - 2 - (mid0 + tid)), F.two.dgx[tid]);
@@ -2598,7 +2616,7 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
 
 template <int R, int C, int X>
 __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
-                           SweepLds<R, C, X>& L, bool dvnd) {
+                           SweepLds<R, C, X>& L, bool dvnd, bool filtered) {
   const SlotDev& s = table[sg.slot];
   if (dvnd && s.info[MLP_INFO_ACTIVE] == 0) return;
   // bit nb of par: the buffer this sweep writes (the one not holding the last result)
@@ -2616,7 +2634,7 @@ __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, co
     for (int nb = 0; nb < MLP_NB_COUNT; ++nb)
       if ((sg.mask >> nb) & 1) s.info[MLP_INFO_PARITY0 + nb] = (par >> nb) & 1;
   }
-  if (sg.kind == 0) sweep_fixed<R, C, X>(s, rb, ct, sg.mask, par, L.f);
+  if (sg.kind == 0) sweep_fixed<R, C, X>(s, rb, ct, sg.mask, par, L.f, filtered);
   else sweep_2opt<C>(s, rb, ct, par, L.d);
 }
 
@@ -2632,12 +2650,16 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     const SlotDev* __restrict__ table,
                                                   SweepSegsInline inl, const SweepSeg* segs_dev,
                                                   int nsegs, int total, int* counter,
-                                                  int skip_inactive) {
+                                                  int flags) {
+  // flags bit 0: a DVND launch (inactive slots skipped, parities read from
+  // info); bit 1: filtered folds of the diagonal rows (many more tiles than
+  // resident workgroups, so most tiles fold after others of their tour)
   __shared__ SweepLds<R, C, X> L;
+  const bool dvnd = (flags & 1) != 0, filtered = (flags & 2) != 0;
   if (!counter) {
     const int item = blockIdx.x;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C, X>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, X>(table, sg, item, L, dvnd, filtered);
     return;
   }
   __shared__ int s_item;
@@ -2648,7 +2670,7 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     __syncthreads();
     if (item >= total) break;
     const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
-    sweep_item<R, C, X>(table, sg, item, L, skip_inactive != 0);
+    sweep_item<R, C, X>(table, sg, item, L, dvnd, filtered);
     __syncthreads();
   }
 }
@@ -2762,7 +2784,7 @@ __global__ __launch_bounds__(kBlock) void k_dvnd_fused(const SlotDev* __restrict
       const int g = e - pre[q];
       const int t1 = min((g + 1) * P.group, P.tiles[nb * P.count + q]);
       for (int t = g * P.group; t < t1; ++t) {
-        sweep_item<R, C, kPlain>(table, sg, t, L, true);
+        sweep_item<R, C, kPlain>(table, sg, t, L, true, true);
         __syncthreads();  // the next tile reuses the LDS
       }
       signal_done(done + q);
@@ -2796,10 +2818,11 @@ MLP_SWEEP_INST(4, 512, kPlain)
 MLP_SWEEP_INST(8, 512, kTwo)
 MLP_SWEEP_INST(12, 1024, kTwo)
 MLP_SWEEP_INST(8, 256, kTwo)
-// the ODC row map on the shapes of at most eight rows
+// the ODC row map: 4 x 256, 8 x 512 (also for 16 x 1024), 4 x 512, and
+// 16 x 128 for 8 x 256 (its or-opt diagonals cross half as many tiles)
 MLP_SWEEP_INST(4, 256, kOdc)
 MLP_SWEEP_INST(8, 512, kOdc)
-MLP_SWEEP_INST(8, 256, kOdc)
+MLP_SWEEP_INST(16, 128, kOdc)
 MLP_SWEEP_INST(4, 512, kOdc)
 
 // ===========================================================================
@@ -2846,12 +2869,13 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
 
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
-                            int total, int* counter, int grid, int skip_inactive) {
+                            int total, int* counter, int grid, int flags) {
   // MLP_TILE_TWO: fixed-row tiles with 2-opt (mask bit 1), the shapes' TWO
-  // variants; MLP_TILE_ODC: the ODC row map (16 x 1024 runs as 8 x 512)
+  // variants; MLP_TILE_ODC: the ODC row map (16 x 1024 runs as 8 x 512,
+  // 8 x 256 as 16 x 128)
 #define MLP_SWEEP_LAUNCH(R, C, X)                                                              \
   hipLaunchKernelGGL((k_sweep<R, C, X>), dim3(grid), dim3(kBlock), 0, st, table, inl,          \
-                     segs_dev, nsegs, total, counter, skip_inactive)
+                     segs_dev, nsegs, total, counter, flags)
   const bool two = (tile & MLP_TILE_TWO) != 0, odc = (tile & MLP_TILE_ODC) != 0;
   switch (tile & 0xff) {
     case MLP_TILE_4x256:
@@ -2865,7 +2889,7 @@ hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
       break;
     case MLP_TILE_8x256:
       if (two) MLP_SWEEP_LAUNCH(8, 256, kTwo);
-      else if (odc) MLP_SWEEP_LAUNCH(8, 256, kOdc);
+      else if (odc) MLP_SWEEP_LAUNCH(16, 128, kOdc);
       else MLP_SWEEP_LAUNCH(8, 256, kPlain);
       break;
     case MLP_TILE_4x512:
@@ -2890,6 +2914,10 @@ void mlp_tile_shape(int tile, int* rows, int* cols) {
   // with 2-opt in the tile the 16 x 1024 shape runs 12 rows (its LDS then
   // still fits three workgroups per CU)
   if ((tile & MLP_TILE_TWO) && t == MLP_TILE_16x1024) *rows = 12;
+  if ((tile & MLP_TILE_ODC) && t == MLP_TILE_8x256) {
+    *rows = 16;
+    *cols = 128;
+  }
   if ((tile & MLP_TILE_ODC) && t == MLP_TILE_16x1024) {
     *rows = 8;
     *cols = 512;

@@ -20,9 +20,12 @@ hipError_t mlp_launch_round_end(hipStream_t st, const SlotDev* table, const Slot
 hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsInline& inl,
                            const EvalSeg* segs_dev, int nsegs, int total, int rowmap,
                            int* counter, int grid, int skip_inactive);
+// flags bit 0: DVND launch (inactive slots skipped); bit 1: filtered folds
+// of the diagonal rows (launches with many more tiles than resident
+// workgroups)
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
-                            int total, int* counter, int grid, int skip_inactive);
+                            int total, int* counter, int grid, int flags);
 void mlp_tile_shape(int tile, int* rows, int* cols);
 size_t mlp_merge_lds_bytes(int pcap, int ncap);
 hipError_t mlp_merge_setup(size_t max_dyn_lds);

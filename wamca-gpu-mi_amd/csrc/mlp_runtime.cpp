@@ -1080,10 +1080,17 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     // most eight rows: 16 x 1024 runs as 8 x 512)
     const bool odc = rowmap == 1;
     int tile = ctx->tile_override;
-    if (tile < 0) tile = auto_tile(ctx, slots, count, nb_mask);
+    if (tile < 0) {
+      tile = auto_tile(ctx, slots, count, nb_mask);
+      // the tuned table is the ADS map's; ODC tiles of tours from 512
+      // positions fold faster with 16 rows (16 x 128)
+      int maxn = 0;
+      for (int q = 0; q < count; ++q) maxn = std::max(maxn, ctx->slots[slots[q]].N);
+      if (odc && maxn >= 512 && (nb_mask & 0x1cu)) tile = MLP_TILE_8x256;
+    }
     if (odc && tile == MLP_TILE_16x1024) tile = MLP_TILE_8x512;
     int TR, TC;
-    mlp_tile_shape(tile, &TR, &TC);
+    mlp_tile_shape(tile | (odc ? MLP_TILE_ODC : 0), &TR, &TC);
     // with swap in the launch, tiles of eight or more rows evaluate 2-opt
     // too (sharing swap's distance products) instead of 2-opt diagonal tiles
     const bool two = !odc && (nb_mask & 3u) == 3u && TR >= 8;
@@ -1137,6 +1144,8 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
     a.grid = total;
     if (two) a.rowmap |= MLP_TILE_TWO;
     if (odc) a.rowmap |= MLP_TILE_ODC;
+    // tiles of a tour run mostly one after another: filtered diagonal folds
+    if (total > 2 * ctx->cu_count * kTileWgPerCu[tile]) a.skip |= 2;
     if (ctx->user_capture)  // replays fold into the same buffers: clear them in the graph
       HCK(mlp_launch_clear_rows(ctx->stream, ctx->d_table, a.sinl, a.sdev, a.nsegs));
     if (persistent) {
@@ -2244,10 +2253,13 @@ int mlp_eval_static(mlp_ctx* ctx, int32_t inst, const int32_t* tour, uint32_t nb
   {
     const bool odc = rowmap == 1;  // fixed tiles folding into ODC rows
     int tile = ctx->tile_override;
-    if (tile < 0) tile = auto_tile_moves(ctx, tour_moves(N, nb_mask), 1, nb_mask);
+    if (tile < 0) {
+      tile = auto_tile_moves(ctx, tour_moves(N, nb_mask), 1, nb_mask);
+      if (odc && N >= 512 && (nb_mask & 0x1cu)) tile = MLP_TILE_8x256;  // 16 x 128
+    }
     if (odc && tile == MLP_TILE_16x1024) tile = MLP_TILE_8x512;
     int TR, TC;
-    mlp_tile_shape(tile, &TR, &TC);
+    mlp_tile_shape(tile | (odc ? MLP_TILE_ODC : 0), &TR, &TC);
     SweepSegsInline segs{};
     int nsegs = 0, total = 0;
     const bool two = !odc && (nb_mask & 3u) == 3u && TR >= 8;
