@@ -128,6 +128,7 @@ def odc_leg(eng, slots, xs, ys, tours, steps, check=64):
     region_ms = eng.region_end()
     dt = time.perf_counter() - t0
     eng.set_tile(-1)
+    check = min(check, len(slots))
     ref, _ = O.eval_batch_xy(0x1F, 1, xs, ys, 1, tours[:check])
     none = np.iinfo(np.int64).max
     bad = 0
@@ -139,8 +140,10 @@ def odc_leg(eng, slots, xs, ys, tours, steps, check=64):
             bad += int(((g["cost"] != c) | (has & ((g["i"] != ref[nb]["i"][q]) |
                                                    (g["j"] != ref[nb]["j"][q])))).sum())
     launches = steps * len(slots)
+    avg_s = region_ms / 1e3 / launches
     return {"moves_per_s": launches * per_tour / dt, "rowmap": "ODC", "kernel": tile,
-            "avg_launch_us": region_ms * 1e3 / launches, "autotune_us_per_launch": tune,
+            "avg_launch_us": avg_s * 1e6, "autotune_us_per_launch": tune,
+            "traffic": traffic_per_launch(tile), "valu": valu_record(tile, per_tour / avg_s),
             "parity": {"tours": check, "mismatched_rows": bad}}
 
 
@@ -277,6 +280,31 @@ def dvnd_legs(eng, args):
         "oracle_matches": (sum(int(a == r["latency"] and b == r["rounds"]) for a, b, r in
                                zip(sv["latency"], sv["rounds"], fixtures["batched_64"]))
                            if "batched_64" in fixtures else None)}
+    # the same 64 jobs as one mlp_dvnd call over 64 slots, in both round forms
+    insts = [eng.instance(xs, ys, 1) for xs, ys, _ in jobs]
+    S = [eng.slot(i) for i in insts]
+    one_call = {}
+    for fused in (False, True, False, True):
+        eng.dvnd_fused(fused)
+        eng.set_tours(S, [j[2] for j in jobs])
+        eng.sync()
+        t = time.perf_counter()
+        eng.dvnd(S, 0)
+        dt = time.perf_counter() - t
+        lat = eng.latency(S)
+        key = "fused" if fused else "per_neighborhood"
+        one_call[key] = {"s": min(dt, one_call.get(key, {"s": dt})["s"]),
+                         "oracle_matches": (sum(int(int(a) == r["latency"]) for a, r in
+                                                zip(lat, fixtures["batched_64"]))
+                                            if "batched_64" in fixtures else None)}
+    eng.dvnd_fused(False)
+    for k in one_call:
+        one_call[k]["solves_per_s"] = 64 / one_call[k]["s"]
+    for x in S:
+        eng.destroy_slot(x)
+    for i in insts:
+        eng.destroy_instance(i)
+    res["batched_64"]["one_call"] = one_call
     # the same jobs with every DVND round as one persistent launch
     eng.dvnd_fused(True)
     serve.run_served(eng, jobs[:8], [0.0] * 8)  # warm-up
@@ -291,6 +319,8 @@ def dvnd_legs(eng, args):
         "oracle_matches": (sum(int(a == r["latency"] and b == r["rounds"]) for a, b, r in
                                zip(sf["latency"], sf["rounds"], fixtures["batched_64"]))
                            if "batched_64" in fixtures else None)}
+    if args.dvnd_legs == "batched":
+        return res
     # n log-uniform in [100, 1000]: all jobs at t = 0 (capacity) and a Poisson
     # stream; n in [50, 2000] with a fraction of clustered n = 18000 jobs
     for name, lo, hi, big, rate in (("burst_100_1000", 100, 1000, 0.0, 0.0),
@@ -353,28 +383,28 @@ def dvnd_legs(eng, args):
     return res
 
 
-def profile_figure(tile, key):
-    """A per-launch figure of this workload's tile shape from the committed
+def profile_figure(kernel, key):
+    """A per-launch figure of a kernel of this workload from the committed
     rocprofv3 summaries: HBM bytes per launch (FETCH_SIZE/WRITE_SIZE,
     calibrated against a copy kernel) or VALU lane-operations per move
     (SQ_INSTS_VALU x 64 / moves), if present."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        return t["kernels"][TILES[tile]][key]
+        return t["kernels"][kernel][key]
     except (OSError, ValueError, KeyError):
         return None
 
 
-def traffic_per_launch(tile):
-    return profile_figure(tile, "hbm_bytes_per_launch")
+def traffic_per_launch(kernel):
+    return profile_figure(kernel, "hbm_bytes_per_launch")
 
 
-def valu_record(tile, moves_per_s):
+def valu_record(kernel, moves_per_s):
     """VALU issue rate of the sweep (its binding resource): lane-operations
     per move from the counters x moves/s, against 256 CU x 4 SIMD x 16 lanes
     x 2.4 GHz."""
-    per_move = profile_figure(tile, "valu_lane_ops_per_move")
+    per_move = profile_figure(kernel, "valu_lane_ops_per_move")
     if per_move is None:
         return None
     ach = per_move * moves_per_s / 1e12
@@ -457,6 +487,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-dvnd", action="store_true")
     ap.add_argument("--no-total", action="store_true")
+    ap.add_argument("--dvnd-legs", choices=("all", "batched"), default="all",
+                    help="batched: only the n = 100 solve and the 64-instance batch in both "
+                         "forms (for trace runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch DVND rounds directly instead of replaying captured graphs "
+                         "(for kernel-trace runs)")
     ap.add_argument("--stream-jobs", type=int, default=300)
     # served vs plain compares capacity: the stream arrives faster than either
     # path solves (plain about 250 solves/s, served about 4000); the rate
@@ -482,6 +518,8 @@ def main():
     from wamca_mi import inputs as I
 
     eng = wamca_mi.Engine(0)
+    if args.no_graph:
+        eng.dvnd_config(4, False, False)
     if args.copy_bytes:
         eng.copy_probe(args.copy_bytes, 4)
         print(json.dumps({"copy_bytes": args.copy_bytes, "reps": 4}))
@@ -573,12 +611,12 @@ def main():
                    "n": n, "tours": T, "moves_per_tour": per_tour, "rowmap": "ADS",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "lds", "achieved": ach, "peak": PEAK_LDS_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_LDS_GBS, "traffic": traffic_per_launch(tile),
+                     "frac": ach / PEAK_LDS_GBS, "traffic": traffic_per_launch(TILES[tile]),
                      "kernel": TILES[tile], "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": lds_bytes,
                      "hbm_algorithmic_bytes_per_launch": hbm_alg,
                      "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS,
-                     "valu": valu_record(tile, per_tour / avg_launch_s)},
+                     "valu": valu_record(TILES[tile], per_tour / avg_launch_s)},
         "autotune_us_per_launch": tune,
         "parity_checked": parity["tours"] if parity else 0,
         "parity": parity,
@@ -604,7 +642,7 @@ def main():
                    "kernel": "k_sweep<12,1024,TWO>", "persistent": persistent,
                    "avg_launch_ms": bms,
                    "lds_frac": lds_bytes * T / (bms / 1e3) / 1e9 / PEAK_LDS_GBS,
-                   "valu": valu_record(2, T * per_tour / (bms / 1e3))}
+                   "valu": valu_record("k_sweep<12,1024,TWO>", T * per_tour / (bms / 1e3))}
             if best is None or rec["moves_per_s"] > best["moves_per_s"]:
                 best = rec
         out["batched"] = best
