@@ -36,6 +36,13 @@ TILES = {0: "k_sweep<4,256>", 1: "k_sweep<8,512>", 2: "k_sweep<16,1024>", 3: "k_
 # the kernels the ODC row map launches for each tile of the table
 ODC_TILES = {0: "k_sweep<4,256,ODC>", 1: "k_sweep<8,512,ODC>", 3: "k_sweep<16,128,ODC>",
              4: "k_sweep<4,512,ODC>"}
+# the kernels of ADS launches with swap and 2-opt (2-opt in the fixed tiles)
+TWO_TILES = {1: "k_sweep<8,512,TWO>", 2: "k_sweep<12,1024,TWO>", 3: "k_sweep<8,256,TWO>"}
+
+
+def ads_kernel(tile, mask):
+    """The kernel an ADS launch of this tile and neighborhood mask runs."""
+    return TWO_TILES[tile] if (mask & 3) == 3 and tile in TWO_TILES else TILES[tile]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_n1000_traffic.json")
 
 
@@ -189,8 +196,8 @@ def batched_cfg2(eng):
         eng.destroy_slot(s)
     eng.destroy_instance(inst)
     return {"workload": "n=784 clustered, 256 random-insertion tours, 5 neighborhoods, one launch",
-            "moves_per_launch": per_tour * T, "tile": TILES[tile],
-            "tile_us": {TILES[k]: v for k, v in enumerate(tile_us)},
+            "moves_per_launch": per_tour * T, "tile": ads_kernel(tile, 0x1F),
+            "tile_us": {ads_kernel(k, 0x1F): v for k, v in enumerate(tile_us)},
             "persistent": persistent, "grid_us": {str(k): v for k, v in grid_us.items()},
             "graph_replay_ms": ms, "moves_per_s": per_tour * T / dt,
             "lds_frac": sum(moves_per_nb(n + 1)[nb] * LDS_BYTES[nb] for nb in range(5)) * T
@@ -435,7 +442,8 @@ def probe_masks(eng, n, tours, tiles, masks):
                 eng.eval(sl, mask, 0)
                 us = eng.region_end() * 1e3
                 mv = sum(v for nb, v in mp.items() if (mask >> nb) & 1) * len(sl)
-                print(json.dumps({"mode": mode, "tile": tile, "kernel": TILES[tile], "mask": mask,
+                print(json.dumps({"mode": mode, "tile": tile, "kernel": ads_kernel(tile, mask),
+                                  "mask": mask,
                                   "tours": len(sl), "moves": mv, "us": round(us, 2),
                                   "gmoves_per_s": round(mv / us / 1e3, 1)}), flush=True)
     eng.set_tile(-1)
@@ -560,9 +568,9 @@ def main():
             eng.eval_each(probe, 0x1F, 0)
             eng.region_begin()
             eng.eval_each(probe, 0x1F, 0)
-            tune[TILES[t]] = eng.region_end() * 1e3 / len(probe)
+            tune[ads_kernel(t, 0x1F)] = eng.region_end() * 1e3 / len(probe)
         tile = min(tune, key=tune.get)
-        tile = [k for k, v in TILES.items() if v == tile][0]
+        tile = [k for k in TILES if ads_kernel(k, 0x1F) == tile][0]
     eng.set_tile(tile)
 
     for _ in range(args.warmup):
@@ -611,12 +619,13 @@ def main():
                    "n": n, "tours": T, "moves_per_tour": per_tour, "rowmap": "ADS",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "lds", "achieved": ach, "peak": PEAK_LDS_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_LDS_GBS, "traffic": traffic_per_launch(TILES[tile]),
-                     "kernel": TILES[tile], "avg_launch_us": avg_launch_s * 1e6,
+                     "frac": ach / PEAK_LDS_GBS,
+                     "traffic": traffic_per_launch(ads_kernel(tile, 0x1F)),
+                     "kernel": ads_kernel(tile, 0x1F), "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": lds_bytes,
                      "hbm_algorithmic_bytes_per_launch": hbm_alg,
                      "hbm_frac": hbm_alg / avg_launch_s / 1e9 / PEAK_HBM_GBS,
-                     "valu": valu_record(TILES[tile], per_tour / avg_launch_s)},
+                     "valu": valu_record(ads_kernel(tile, 0x1F), per_tour / avg_launch_s)},
         "autotune_us_per_launch": tune,
         "parity_checked": parity["tours"] if parity else 0,
         "parity": parity,
