@@ -629,49 +629,8 @@ __global__ __launch_bounds__(kBlock) void k_eval(const SlotDev* __restrict__ tab
 
 namespace {
 
-// Positions claimed by kept moves: one bit per position (claim_words0(n)
-// words), then one summary bit per word of those, set when the word holds a
-// claimed position, so that a test over a long interval reads one summary bit
-// per 32 words between its two edge words.
-__host__ __device__ constexpr int claim_words0(int n) { return (n + 32) / 32 + 1; }
-__host__ __device__ constexpr int claim_words(int n) {
-  return claim_words0(n) + claim_words0(n) / 32 + 1;
-}
-
-__device__ __forceinline__ bool range_any(const uint32_t* bits, int a, int b) {
-  const int wa = a >> 5, wb = b >> 5;
-  for (int w = wa; w <= wb; ++w) {
-    uint32_t m = 0xffffffffu;
-    if (w == wa) m &= 0xffffffffu << (a & 31);
-    if (w == wb) m &= 0xffffffffu >> (31 - (b & 31));
-    if (bits[w] & m) return true;
-  }
-  return false;
-}
-
-__device__ __forceinline__ void range_set(uint32_t* bits, int a, int b) {
-  const int wa = a >> 5, wb = b >> 5;
-  for (int w = wa; w <= wb; ++w) {
-    uint32_t m = 0xffffffffu;
-    if (w == wa) m &= 0xffffffffu << (a & 31);
-    if (w == wb) m &= 0xffffffffu >> (31 - (b & 31));
-    atomicOr(&bits[w], m);
-  }
-}
-
-// Is any position of [a, b] claimed?  (n: the tour's positions)
-__device__ __forceinline__ bool claims_any(const uint32_t* bits, int n, int a, int b) {
-  const int wa = a >> 5, wb = b >> 5;
-  if (wb - wa <= 2) return range_any(bits, a, b);
-  if ((bits[wa] & (0xffffffffu << (a & 31))) | (bits[wb] & (0xffffffffu >> (31 - (b & 31)))))
-    return true;
-  return range_any(bits + claim_words0(n), wa + 1, wb - 1);  // the whole words between
-}
-
-__device__ __forceinline__ void claims_set(uint32_t* bits, int n, int a, int b) {
-  range_set(bits, a, b);
-  range_set(bits + claim_words0(n), a >> 5, b >> 5);
-}
+#define MLP_CLAIM_OR(p, m) atomicOr((p), (m))
+#include "mlp_claims.h"
 
 // Positions a kept move rewrites, and their new sources (mlkswap.cu:1066-1106,
 // mlk2opt.cu:725-768, mlkoropt.cu:886-950 restated as a position map).
@@ -741,7 +700,7 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
     const int b = NB == 1 ? j + 1 : max(i, j) + K;
     bool pre = valid;
     if (valid) {
-      if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+      if (NB == 0) pre = !claimed(bits, s.N, i) && !claimed(bits, s.N, j);
       else pre = !claims_any(bits, s.N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
@@ -1310,7 +1269,7 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
     const int b = NB == 1 ? j + 1 : max(i, j) + K;
     bool pre = valid;
     if (valid) {
-      if (NB == 0) pre = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+      if (NB == 0) pre = !claimed(bits, s.N, i) && !claimed(bits, s.N, j);
       else pre = !claims_any(bits, s.N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
@@ -1407,7 +1366,7 @@ __global__ __launch_bounds__(kBlock) void k_total_merge(const SlotDev* __restric
         move_ij(nb, 0, N, row, ctx, i, j);
         ci[u] = i;
         cj[u] = j;
-        if (nb == 0) pass = (((bits[i >> 5] >> (i & 31)) | (bits[j >> 5] >> (j & 31))) & 1u) == 0;
+        if (nb == 0) pass = !claimed(bits, N, i) && !claimed(bits, N, j);
         else if (nb == 1) pass = !claims_any(bits, N, i, j + 1);
         else pass = !claims_any(bits, N, min(i, j), max(i, j) + k);
       } else {
