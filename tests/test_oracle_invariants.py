@@ -187,6 +187,31 @@ def test_int32_exactness_bounds(n, seed, extreme):
                 assert top * 20 >= bound, (nb, top, bound)
 
 
+@pytest.mark.parametrize("n,seed,extreme", [(40, 1, False), (90, 2, False), (60, 3, True),
+                                            (120, 4, True)])
+def test_2opt_per_move_bound(n, seed, extreme):
+    """The per-move 2-opt bound the 2-opt tiles use to take the int32 path
+    for a row whose moves all stay below 2^31 when the tour's bound fails:
+    reversing positions i..j moves each reversed node's arrival by at most
+    (T[j] - T[i]) + maxd and every later node's by at most 2 maxd, so
+    |delta| <= (j - i + 1)(T[j] - T[i] + maxd) + 2 (N - 1 - j) maxd;
+    checked against every move of random and adversarial tours."""
+    span = 3000
+    xs, ys = _extreme_instance(n, seed, span) if extreme else O.instance_uniform(n, seed, span)
+    W = O.weights(xs, ys, 1)
+    maxd = int(W.max())
+    tours = [O.perm_tour(seed + q, n) for q in range(3)]
+    if extreme:
+        tours.append(np.array([0] + list(range(1, n)) + [0], np.int32))
+    for t in tours:
+        N = len(t)
+        T = np.concatenate([[0], np.cumsum([W[t[p - 1], t[p]] for p in range(1, N)])]).astype(np.int64)
+        m = O.enumerate_moves(1, 0, W, t)
+        i, j = m["i"].astype(np.int64), m["j"].astype(np.int64)
+        bound = (j - i + 1) * (T[j] - T[i] + maxd) + 2 * (N - 1 - j) * maxd
+        assert (np.abs(m["ads"]) <= bound).all()
+
+
 @pytest.mark.parametrize("n,seed", [(9, 11), (16, 12), (30, 13), (41, 14)])
 @pytest.mark.parametrize("rowmap", [0, 1])
 def test_oropt_k_beyond_3(n, seed, rowmap):

@@ -2447,7 +2447,27 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
     }
   }
   __syncthreads();
+  // A tour above the int32 bound can still have rows whose moves all stay
+  // below 2^31: reversing positions i..j moves each reversed node's arrival
+  // by at most (T[j] - T[i]) + maxd and every later node's by at most
+  // 2 maxd, so |delta| <= (m + 1)(T[j] - T[i] + maxd) + 2 (N - 1 - j) maxd
+  // for j = i + m; such a row takes the int32 path (the bound is uniform
+  // over the wave, as the wave owns the row).
+  bool w32 = p32;
   if (!p32) {
+    const int64_t md = s.maxd;
+    bool ok = true;
+#pragma unroll
+    for (int part = 0; part < 2; ++part)
+      if (ihi[part] >= ilo[part]) {
+        const int64_t span = ldg(s.t64 + ihi[part] + mm[part]) - ldg(s.t64 + ilo[part]);
+        const int64_t b = (int64_t)(mm[part] + 1) * (span + md) +
+                          2 * (int64_t)(N - 1 - ilo[part] - mm[part]) * md;
+        ok = ok && b < 2147483647LL;
+      }
+    w32 = __builtin_amdgcn_readfirstlane((int)ok) != 0;
+  }
+  if (!w32) {
     // delta = (N-i)(e[x] - w[i]) + (N-j-1)(e[x+1] - w[j+1]) + 2(Q[j] - Q[i])
     //         - (i+j+1)(T[j] - T[i])   (the ADS form of mlk2opt.cu, 64-bit)
     // each lane keeps the smallest key delta * 2^20 + ctx of its moves
