@@ -215,3 +215,21 @@ def test_fused_rounds_match_oracle(eng, chunk, graph):
         tref, Lref, rref, _ = O.dvnd(W, t0, 0)
         assert L == Lref and rounds == rref
         assert np.array_equal(tour, tref)
+
+
+def test_fused_rounds_after_staging_growth(eng):
+    """Fused rounds on a few slots, then a larger batch whose tour upload grows
+    the context's staging buffers, then fused rounds again: every slot's
+    result equals the oracle's (the fused rounds' counters outlive the
+    upload buffers' growth)."""
+    eng.dvnd_fused(True)
+    try:
+        for ns, seed in (([50, 90, 130], 990), ([40 + 23 * q for q in range(24)], 991)):
+            jobs = _jobs(ns, seed)
+            out = _run(eng, jobs, 0)
+            for (xs, ys, W, t0), (tour, L, rounds) in zip(jobs, out):
+                tref, Lref, rref, _ = O.dvnd(W, t0, 0)
+                assert L == Lref and rounds == rref
+                assert np.array_equal(tour, tref)
+    finally:
+        eng.dvnd_fused(False)

@@ -62,7 +62,7 @@ def test_static_eval_matches_eval(eng):
             DevArray(eng, np.int32, cap_rows), DevArray(eng, np.int32, cap_rows))
     try:
         cases = [(1000, 1, 0x1F, 0, 11), (301, 0, 0x1F, 1, 12), (150, 1, 0x0B, 0, 13),
-                 (60, 1, 0x14, 1, 14)]
+                 (60, 1, 0x14, 1, 14), (700, 1, 0x1F, 1, 15)]
         for n, rnd, mask, rowmap, seed in cases:
             xs, ys = O.instance_clustered(n, seed) if seed % 2 else O.instance_uniform(n, seed)
             tour = O.perm_tour(seed, n)

@@ -892,6 +892,43 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
   return S.kept;
 }
 
+// Rows [q0, q1) of a merge: the tour and its coordinates by client copied
+// (told, tnew, xyc), and each improving row's (cost, row) key handed to
+// put(key) with its ctx recorded.  Each thread issues the loads of eight
+// rows (strided by kBlock) before it uses the first.
+template <typename Put>
+__device__ __forceinline__ void merge_rows(const SlotDev& s, const int64_t* rk, int R, int q0,
+                                           int q1, int32_t* told, int32_t* tnew, int32_t* xyc,
+                                           uint16_t* ctxL, Put put) {
+  constexpr int P = 8;
+  const int N = s.N, tid = threadIdx.x;
+  const int rend = min(R, q1), nend = min(N, q1);
+  for (int base = q0; base < q1; base += kBlock * P) {
+    int64_t key[P];
+    int c[P], xq[P];
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const int q = base + e * kBlock + tid;
+      key[e] = q < rend ? rk[q] : MLP_KEY_NONE;
+      c[e] = q < nend ? s.tour[q] : 0;
+      xq[e] = q < nend ? s.xy[q] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const int q = base + e * kBlock + tid;
+      if (q < nend) {
+        told[q] = c[e];
+        tnew[q] = c[e];
+        xyc[c[e]] = xq[e];
+      }
+      if (key[e] != MLP_KEY_NONE && (key[e] >> MLP_KEY_SHIFT) < 0) {
+        put((key[e] >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q);
+        ctxL[q] = (uint16_t)(key[e] & MLP_KEY_MASK);
+      }
+    }
+  }
+}
+
 // Merge + apply of one slot by one workgroup; lds holds the working arrays
 // (mlp_merge_lds_bytes(pcap_l, ncap_l)).
 __device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap, int pcap_l,
@@ -924,21 +961,8 @@ __device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap,
     __syncthreads();  // every thread has read the old parity
     if (tid == 0) s.info[MLP_INFO_PARITY0 + nb] = par;
   }
-  for (int q = tid; q < max(R, N); q += kBlock) {
-    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
-    const int c = q < N ? s.tour[q] : 0;
-    const int xq = q < N ? s.xy[q] : 0;
-    if (q < N) {
-      told[q] = c;
-      tnew[q] = c;
-      xyc[c] = xq;
-    }
-    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
-      const int at = atomicAdd(&S.m, 1);
-      keys[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
-      ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
-    }
-  }
+  merge_rows(s, rk, R, 0, max(R, N), told, tnew, xyc, ctxL,
+             [&](int64_t k) { keys[atomicAdd(&S.m, 1)] = k; });
   __syncthreads();
   const int M = S.m;
   if (M == 0) {
@@ -1032,20 +1056,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_compact(const SlotDev* __restrict
   const int64_t* rk = s.rows[nb] + (par ? s.rowcap : 0);
   if (tid == 0) lm = 0;
   __syncthreads();
-  for (int q = q0 + tid; q < q1; q += kBlock) {
-    const int64_t key = q < R ? rk[q] : MLP_KEY_NONE;
-    if (q < N) {
-      const int c = s.tour[q];
-      a.told[q] = c;
-      a.tnew[q] = c;
-      a.xyc[c] = s.xy[q];
-    }
-    if (key != MLP_KEY_NONE && (key >> MLP_KEY_SHIFT) < 0) {
-      const int at = atomicAdd(&lm, 1);
-      lk[at] = (key >> MLP_KEY_SHIFT) * (int64_t)(1 << MLP_KEY_SHIFT) + q;
-      a.ctxL[q] = (uint16_t)(key & MLP_KEY_MASK);
-    }
-  }
+  merge_rows(s, rk, R, q0, q1, a.told, a.tnew, a.xyc, a.ctxL,
+             [&](int64_t k) { lk[atomicAdd(&lm, 1)] = k; });
   __syncthreads();
   const int m = lm;
   if (m == 0) return;

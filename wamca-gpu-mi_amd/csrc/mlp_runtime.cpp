@@ -841,7 +841,6 @@ int mlp_set_tours(mlp_ctx* ctx, const int32_t* slots, int32_t count, const int32
       const size_t cap = std::max<size_t>(total, 2 * ctx->d_tstage_cap + 4096);
       HCK(hipStreamSynchronize(ctx->stream));
       if (ctx->d_tstage) hipFree(ctx->d_tstage);
-  if (ctx->d_sync) hipFree(ctx->d_sync);
       ctx->d_tstage = nullptr;
       ctx->d_tstage_cap = 0;
       HCK(hipMalloc((void**)&ctx->d_tstage, cap * 4));
@@ -1862,10 +1861,10 @@ int enqueue_fused(mlp_ctx* ctx, const int32_t* slots, int count, int max_rounds,
   if (count > ctx->sync_cap) {  // counters start at zero; each launch leaves them at zero
     if (ctx->capturing) return fail(ctx, MLP_ESTATE, "fused round counters grown during a capture");
     HCK(hipStreamSynchronize(ctx->stream));
+    const int cap = std::max(count, 2 * ctx->sync_cap + 64);
     if (ctx->d_sync) HCK(hipFree(ctx->d_sync));
     ctx->d_sync = nullptr;
     ctx->sync_cap = 0;
-    const int cap = std::max(count, 2 * ctx->sync_cap + 64);
     HCK(hipMalloc((void**)&ctx->d_sync, sizeof(int32_t) * (4 + 2 * (size_t)cap)));
     HCK(hipMemset(ctx->d_sync, 0, sizeof(int32_t) * (4 + 2 * (size_t)cap)));
     ctx->sync_cap = cap;

@@ -150,23 +150,46 @@ def run_plain(eng, jobs, arrivals, rnd=1, rowmap=ROWMAP_ADS):
 def main():
     """python -m wamca_mi.serve --job KIND N SEED: one DVND job alone on the GPU
     (KIND uniform or clustered; start tour from the randomized construction,
-    the selection-based variant above n = 4000)."""
+    the selection-based variant above n = 4000).
+    python -m wamca_mi.serve --batch COUNT SEED: the jobs of
+    workloads.dvnd_jobs(COUNT, SEED) as one mlp_dvnd call over COUNT slots
+    (--fused: each round as one persistent launch)."""
     import argparse
     import json
 
     from . import inputs as I
+    from . import workloads as WL
     from .engine import Engine
     ap = argparse.ArgumentParser()
-    ap.add_argument("--job", nargs=3, metavar=("KIND", "N", "SEED"), required=True)
+    g = ap.add_mutually_exclusive_group(required=True)
+    g.add_argument("--job", nargs=3, metavar=("KIND", "N", "SEED"))
+    g.add_argument("--batch", nargs=2, type=int, metavar=("COUNT", "SEED"))
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each DVND round's kernels directly (no graph replay)")
+    ap.add_argument("--fused", action="store_true")
     args = ap.parse_args()
-    kind, n, seed = args.job[0], int(args.job[1]), int(args.job[2])
-    xs, ys = I.clustered(n, seed) if kind == "clustered" else I.uniform(n, seed, 1000)
-    tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, 1, seed)
     eng = Engine(0)
     if args.no_graph:
         eng.dvnd_config(4, False, False)
+    eng.dvnd_fused(args.fused)
+    if args.batch:
+        count, seed = args.batch
+        jobs, meta = WL.dvnd_jobs(count, seed)
+        insts = [eng.instance(xs, ys, 1) for xs, ys, _ in jobs]
+        slots = [eng.slot(i) for i in insts]
+        eng.set_tours(slots, [j[2] for j in jobs])
+        eng.sync()
+        t0 = time.perf_counter()
+        rounds = eng.dvnd(slots, 0)
+        dt = time.perf_counter() - t0
+        print(json.dumps({"batch": count, "seed": seed, "seconds": dt,
+                          "max_n": max(m["n"] for m in meta), "max_rounds": int(max(rounds)),
+                          "latency_sum": int(sum(int(v) for v in eng.latency(slots)))}))
+        eng.close()
+        return
+    kind, n, seed = args.job[0], int(args.job[1]), int(args.job[2])
+    xs, ys = I.clustered(n, seed) if kind == "clustered" else I.uniform(n, seed, 1000)
+    tour = (I.random_tour_fast if n > 4000 else I.random_tour)(xs, ys, 1, seed)
     t0 = time.perf_counter()
     out = run_plain(eng, [(xs, ys, tour)], [0.0])
     print(json.dumps({"kind": kind, "n": n, "seed": seed, "seconds": time.perf_counter() - t0,
