@@ -450,22 +450,57 @@ __device__ __forceinline__ void prep_segment(const SlotDev& s, const int32_t* to
     segQ += P.wt[q][1];
   }
   if (WRITE) {
+    int32_t vt[kPrepPer], vq[kPrepPer], vg[kPrepPer], vh[kPrepPer];
+    int64_t T8[kPrepPer], Q8[kPrepPer];
 #pragma unroll
     for (int e = 0; e < kPrepPer; ++e) {
       const int p = p0 + e;
-      if (p < N) {
-        const int64_t T = offT + lt[e], Q = offQ + lq[e];
-        s.xy[p] = xy[e + 1];
-        s.w[p] = w[e];
-        s.t64[p] = T;
-        s.q64[p] = Q;
-        s.t32[p] = (int32_t)(uint32_t)(uint64_t)T;
-        s.q32[p] = (int32_t)(uint32_t)(uint64_t)Q;
-        const uint32_t Z =
-            2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
-        s.g32[p] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w[e] - Z);
-        s.h32[p] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)w[e + 1]);
-        lacc += T;
+      const int64_t T = offT + lt[e], Q = offQ + lq[e];
+      T8[e] = T;
+      Q8[e] = Q;
+      vt[e] = (int32_t)(uint32_t)(uint64_t)T;
+      vq[e] = (int32_t)(uint32_t)(uint64_t)Q;
+      const uint32_t Z = 2u * (uint32_t)(uint64_t)Q - (uint32_t)(2 * p + 1) * (uint32_t)(uint64_t)T;
+      vg[e] = (int32_t)(0u - (uint32_t)(N - p) * (uint32_t)w[e] - Z);
+      vh[e] = (int32_t)(Z - (uint32_t)(N - p - 1) * (uint32_t)w[e + 1]);
+      if (p < N) lacc += T;
+    }
+    if (p0 + kPrepPer <= N) {
+      // the thread's eight consecutive positions as 16-byte stores (p0 is a
+      // multiple of 8 and every array 256-byte aligned): a quarter of the
+      // store instructions of one word per position
+      auto put4 = [](int32_t* d, int a, int b, int c, int e) { *(int4*)d = make_int4(a, b, c, e); };
+      auto put2 = [](int64_t* d, int64_t a, int64_t b) {
+        *(longlong2*)d = make_longlong2((long long)a, (long long)b);
+      };
+#pragma unroll
+      for (int h = 0; h < kPrepPer; h += 4) {
+        put4(s.xy + p0 + h, xy[h + 1], xy[h + 2], xy[h + 3], xy[h + 4]);
+        put4(s.w + p0 + h, w[h], w[h + 1], w[h + 2], w[h + 3]);
+        put4(s.t32 + p0 + h, vt[h], vt[h + 1], vt[h + 2], vt[h + 3]);
+        put4(s.q32 + p0 + h, vq[h], vq[h + 1], vq[h + 2], vq[h + 3]);
+        put4(s.g32 + p0 + h, vg[h], vg[h + 1], vg[h + 2], vg[h + 3]);
+        put4(s.h32 + p0 + h, vh[h], vh[h + 1], vh[h + 2], vh[h + 3]);
+      }
+#pragma unroll
+      for (int h = 0; h < kPrepPer; h += 2) {
+        put2(s.t64 + p0 + h, T8[h], T8[h + 1]);
+        put2(s.q64 + p0 + h, Q8[h], Q8[h + 1]);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kPrepPer; ++e) {
+        const int p = p0 + e;
+        if (p < N) {
+          s.xy[p] = xy[e + 1];
+          s.w[p] = w[e];
+          s.t64[p] = T8[e];
+          s.q64[p] = Q8[e];
+          s.t32[p] = vt[e];
+          s.q32[p] = vq[e];
+          s.g32[p] = vg[e];
+          s.h32[p] = vh[e];
+        }
       }
     }
   }
