@@ -154,7 +154,7 @@ def odc_leg(eng, slots, xs, ys, tours, steps, check=64):
             "parity": {"tours": check, "mismatched_rows": bad}}
 
 
-def batched_cfg2(eng):
+def batched_n784(eng):
     """256 random-insertion tours (seeds 1..256) of one clustered n=784
     instance (20 clusters, sigma 40, centres in [0,2000]^2, seed 2), all five
     neighborhoods in one launch: the autotuner picks the tile shape, the faster
@@ -656,7 +656,7 @@ def main():
                 best = rec
         out["batched"] = best
     if not args.no_batched:
-        out["batched_cfg2"] = batched_cfg2(eng)
+        out["batched_n784_x256"] = batched_n784(eng)
     if not args.no_dvnd:
         out["dvnd"] = dvnd_legs(eng, args)
     if not args.no_total:

@@ -246,7 +246,7 @@ def test_merge_apply_steps(eng, n, seed, rowmap):
         tp = _check_step(eng, s, W, tp, nb, rowmap)
 
 
-def test_dvnd_cfg0(eng):
+def test_dvnd_n100_uniform(eng):
     """n = 100 uniform [0,1000]^2 seed 1, nint, random-insertion start (seed 1)."""
     xs, ys = O.instance_uniform(100, 1)
     W = O.weights(xs, ys, 1)
