@@ -408,13 +408,16 @@ __device__ __forceinline__ void prep_segment(const SlotDev& s, const int32_t* to
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int p0 = base + tid * kPrepPer;
-  // xy of positions p0 - 1 .. p0 + kPrepPer (clamped into the tour)
-  int xy[kPrepPer + 2];
+  // xy of positions p0 - 1 .. p0 + kPrepPer (clamped into the tour): every
+  // client id is loaded before the first coordinate gather
+  int cl[kPrepPer + 2], xy[kPrepPer + 2];
+#pragma unroll
+  for (int e = 0; e < kPrepPer + 2; ++e) cl[e] = tour[min(max(p0 - 1 + e, 0), N - 1)];
 #pragma unroll
   for (int e = 0; e < kPrepPer + 2; ++e) {
     // client ids clamped into the instance: a tour read from caller memory
     // never indexes outside the coordinate arrays
-    const int c = min(max(tour[min(max(p0 - 1 + e, 0), N - 1)], 0), s.n - 1);
+    const int c = min(max(cl[e], 0), s.n - 1);
     xy[e] = xyc ? xyc[c] : (ldg(s.xs + c) << 16) | ldg(s.ys + c);
   }
   int w[kPrepPer + 1];  // w[e]: edge into position p0 + e
@@ -525,9 +528,11 @@ __device__ __forceinline__ void prep_flags(const SlotDev& s, int64_t Ttot) {
                             (Ttot < (1LL << 24) ? MLP_T24 : 0);
 }
 
-// The whole tour's prefix arrays and info words by one workgroup.
-__device__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
-                        const int32_t* xyc = nullptr) {
+// The whole tour's prefix arrays and info words by one workgroup (inlined,
+// so that the tour and coordinate arrays keep their address space: LDS in a
+// merge, global memory in k_prep).
+__device__ __forceinline__ void prep_wg(const SlotDev& s, const int32_t* tour, PrepLds& P,
+                                        const int32_t* xyc = nullptr) {
   const int N = s.N, tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   int64_t lacc = 0;  // this thread's share of L = sum_p T[p]
   if (tid == 0) {
