@@ -2645,6 +2645,19 @@ __device__ __forceinline__ SweepSeg find_sweep_inline(const SweepSegsInline& inl
   return find_inline<SweepSeg>(inl, nsegs, item);
 }
 
+// A launch whose slots all have the same segments (flags bits 8..31: the
+// items per slot; bit 2: two segments per slot, the first starting at item
+// 0): the segment of an item by division instead of a binary search over
+// device memory.
+__device__ __forceinline__ SweepSeg find_sweep_dev(const SweepSeg* segs, int nsegs, int item,
+                                                   int flags) {
+  const int group = (int)((uint32_t)flags >> 8);
+  if (group == 0) return find_sweep(segs, nsegs, item);
+  const int g = item / group;
+  if (!(flags & 4)) return segs[g];
+  return segs[2 * g + (item - g * group >= segs[1].blk0 ? 1 : 0)];
+}
+
 template <int R, int C, int X>
 __device__ __forceinline__ void sweep_item(const SlotDev* __restrict__ table, const SweepSeg& sg, int item,
                            SweepLds<R, C, X>& L, bool dvnd, bool filtered) {
@@ -2684,12 +2697,14 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
                                                   int flags) {
   // flags bit 0: a DVND launch (inactive slots skipped, parities read from
   // info); bit 1: filtered folds of the diagonal rows (many more tiles than
-  // resident workgroups, so most tiles fold after others of their tour)
+  // resident workgroups, so most tiles fold after others of their tour);
+  // bits 2, 8..31: equal segments per slot (find_sweep_dev)
   __shared__ SweepLds<R, C, X> L;
   const bool dvnd = (flags & 1) != 0, filtered = (flags & 2) != 0;
   if (!counter) {
     const int item = blockIdx.x;
-    const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
+    const SweepSeg sg = segs_dev ? find_sweep_dev(segs_dev, nsegs, item, flags)
+                                  : find_sweep_inline(inl, nsegs, item);
     sweep_item<R, C, X>(table, sg, item, L, dvnd, filtered);
     return;
   }
@@ -2700,7 +2715,8 @@ __global__ __launch_bounds__(kBlock, (R * C <= 2048 ? 7 : 1)) void k_sweep(
     const int item = s_item;
     __syncthreads();
     if (item >= total) break;
-    const SweepSeg sg = segs_dev ? find_sweep(segs_dev, nsegs, item) : find_sweep_inline(inl, nsegs, item);
+    const SweepSeg sg = segs_dev ? find_sweep_dev(segs_dev, nsegs, item, flags)
+                                  : find_sweep_inline(inl, nsegs, item);
     sweep_item<R, C, X>(table, sg, item, L, dvnd, filtered);
     __syncthreads();
   }

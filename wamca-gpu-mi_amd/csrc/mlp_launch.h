@@ -22,7 +22,8 @@ hipError_t mlp_launch_eval(hipStream_t st, const SlotDev* table, const EvalSegsI
                            int* counter, int grid, int skip_inactive);
 // flags bit 0: DVND launch (inactive slots skipped); bit 1: filtered folds
 // of the diagonal rows (launches with many more tiles than resident
-// workgroups)
+// workgroups); bits 8..31 (with bit 2: two segments per slot): the items per
+// slot when every slot of a device segment list has the same segments
 hipError_t mlp_launch_sweep(hipStream_t st, int tile, const SlotDev* table,
                             const SweepSegsInline& inl, const SweepSeg* segs_dev, int nsegs,
                             int total, int* counter, int grid, int flags);

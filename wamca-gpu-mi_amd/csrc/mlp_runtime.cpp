@@ -1139,6 +1139,18 @@ static int eval_impl(mlp_ctx* ctx, const int32_t* slots, int32_t count, uint32_t
       st = device_list(ctx, (const int32_t*)segs.data(), a.nsegs * MLP_SWEEPSEG_INTS,
                        (const int32_t**)&a.sdev);
       if (st) return st;
+      // every slot with the same segments: the kernel finds an item's
+      // segment by division
+      const int sps = (a.nsegs == count) ? 1 : (a.nsegs == 2 * count ? 2 : 0);
+      if (sps) {
+        const int group = total / count, size0 = sps == 2 ? segs[1].blk0 : group;
+        bool eq = total == group * count && group < (1 << 23);
+        for (int k = 0; k < count && eq; ++k) {
+          eq = segs[(size_t)k * sps].blk0 == k * group;
+          if (sps == 2) eq = eq && segs[(size_t)k * 2 + 1].blk0 == k * group + size0;
+        }
+        if (eq) a.skip |= (sps == 2 ? 4 : 0) | (group << 8);
+      }
     }
     a.grid = total;
     if (two) a.rowmap |= MLP_TILE_TWO;
