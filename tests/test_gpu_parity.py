@@ -143,22 +143,34 @@ def test_clustered_and_tiny(eng):
 
 
 @pytest.mark.parametrize("rowmap", [0, 1])
-def test_persistent_and_batched_equal_single(eng, rowmap):
+@pytest.mark.parametrize("tile", [-1, 0, 2])
+@pytest.mark.parametrize("persistent", [True, False])
+def test_persistent_and_batched_equal_single(eng, rowmap, tile, persistent):
+    # 40 slots of one instance: more segments than a launch carries inline,
+    # all of one size (one segment per slot with 2-opt in the fixed-row tiles
+    # of 12x1024, two with 2-opt on its own tiles of 4x256 or under ODC)
     xs, ys = O.instance_uniform(300, 11)
     W = O.weights(xs, ys, 1)
     inst = eng.instance(xs, ys, 1)
-    S = eng.slots(inst, 24)
-    tours = [O.perm_tour(500 + q, 300) for q in range(24)]
+    S = eng.slots(inst, 40)
+    tours = [O.perm_tour(500 + q, 300) for q in range(40)]
     eng.set_tours(S, tours)
-    eng.eval(S, 0x1F, rowmap, persistent=True)
-    batched = {(s, nb): eng.rows(s, nb)["cost"] for s in S for nb in range(5)}
-    for q, s in enumerate(S):
-        eng.eval([s], 0x1F, rowmap)
-        for nb in range(5):
-            assert np.array_equal(eng.rows(s, nb)["cost"], batched[(s, nb)])
-    for q in (0, 7, 23):
+    eng.set_tile(tile)
+    try:
+        eng.eval(S, 0x1F, rowmap, persistent=persistent)
+        batched = {(s, nb): eng.rows(s, nb)["cost"] for s in S for nb in range(5)}
+        for q, s in enumerate(S):
+            eng.eval([s], 0x1F, rowmap)
+            for nb in range(5):
+                assert np.array_equal(eng.rows(s, nb)["cost"], batched[(s, nb)]), (q, nb)
+    finally:
+        eng.set_tile(-1)
+    for q in (0, 7, 39):
         for nb in range(5):
             _check_rows(eng, S[q], W, tours[q], nb, rowmap)
+    for s in S:
+        eng.destroy_slot(s)
+    eng.destroy_instance(inst)
 
 
 def test_ads_materialized(eng):

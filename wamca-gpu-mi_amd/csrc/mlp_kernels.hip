@@ -2292,6 +2292,10 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   const int q1 = min(q0 + C, N - 1);
   const int cbase = q0 - 1;                  // LDS column 0 <-> tour position q0 - 1
   const int cend = min(q1 + 3, N);
+  // Swap and 2-opt moves have j >= i + 1 >= b0 + 2 and read columns from
+  // j - 1 on: without or-opt the block starts at position b0 + 1 (the
+  // columns left of it are not filled and never used).
+  const int cfill = (mask & 0x1c) ? cbase : max(cbase, min(b0 + 1, cend - 1));
   // every global load of the tile is issued before the first wait
   const int64_t pflags = ldg(s.info + MLP_INFO_PATH32);
   const bool p32 = (pflags & MLP_P32_SWAP_OROPT) != 0;
@@ -2304,7 +2308,7 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   int xc[FI], wv[FI], tv[FI], hv[FI];
 #pragma unroll
   for (int it = 0; it < FI; ++it) {
-    const int c = min(cbase + tid + it * kBlock, cend - 1);
+    const int c = min(cfill + tid + it * kBlock, cend - 1);
     xc[it] = ldg(s.xy + c);
     wv[it] = ldg(s.w + c);
     tv[it] = ldg(s.t32 + c);
@@ -2335,7 +2339,7 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
   if (mask == 1 && q1 <= b0 + 3) return;
 #pragma unroll
   for (int it = 0; it < FI; ++it) {
-    const int c = cbase + tid + it * kBlock;
+    const int c = cfill + tid + it * kBlock;
     if (c < cend) {
 #pragma unroll
       for (int va = 0; va < R + 4; ++va) L.v[va][c - cbase] = (uint16_t)dist_xy(xa[va], xc[it], dr);
