@@ -720,9 +720,12 @@ __device__ __forceinline__ bool conflicts_nb(int i1, int j1, int i2, int j2) {
 // candidates that pass it are decided in order by a scalar loop over the
 // kept ones.  Kept moves go to keys[q]
 // as (i << 32 | j) and to the slot's kept arrays.  Returns their count.
+// Candidates from index `tail` on are read from gtail (a sort whose keys
+// exceed the LDS buffer); kept moves are written below `tail`.
 template <int NB>
 __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* keys,
-                           const uint16_t* ctxL, uint32_t* bits, int lane) {
+                           const uint16_t* ctxL, uint32_t* bits, int lane,
+                           const int64_t* gtail = nullptr, int tail = INT32_MAX) {
   constexpr int K = NB >= 2 ? NB - 1 : 0;
   int kept_total = 0;
   for (int base = 0; base < M; base += kWave) {
@@ -731,7 +734,7 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
     int i = 0, j = 0, row = 0;
     int64_t cost = 0;
     if (valid) {
-      const int64_t sk = keys[t];
+      const int64_t sk = t < tail ? keys[t] : ldg(gtail + t);
       cost = sk >> MLP_KEY_SHIFT;
       row = (int)(sk & MLP_KEY_MASK);
       move_ij(NB, rowmap, N, row, (int)ctxL[row], i, j);
@@ -816,45 +819,172 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
 
 }  // namespace
 
-// One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
-// tail != 0: this is the last merge of a DVND round, and it also ends the
-// round for each active slot (round_tail, max_rounds = tail - 1).
-// Bitonic sort of keys[0..M) over the workgroup (keys[M..P) padded with
-// KEY_NONE, P the next power of two).  Each phase has P/2 compare-swaps; a
-// thread takes up to four pairs at a time, loading all of them before
-// comparing.
-__device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
-  const int tid = threadIdx.x;
-  int P = 1;
-  while (P < M) P <<= 1;
-  for (int t = M + tid; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
-  __syncthreads();
-  const int half = P >> 1;
-  for (int kk = 2; kk <= P; kk <<= 1) {
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      const int lj = 31 - __clz(jj);
-      for (int p0 = tid; p0 < half; p0 += 4 * kBlock) {
-        int lo[4];
-        int64_t va[4], vb[4];
+// Bitonic sorts of 64-bit keys by the workgroup.  Stage kk (a power of two)
+// merges blocks of kk keys, element i ascending when (i & kk) == 0 (the
+// whole array flipped for a descending sort); its phase jj compares i with
+// i + jj for (i & jj) == 0.  Phases with jj >= 512 go pair by pair through
+// memory between barriers; the phases below 512 run on 512-key segments in
+// the registers of one wave, key base + 64 e + lane in v[e] of that lane:
+// jj of 64..256 pairs keys inside a lane, jj of 1..32 pairs lanes
+// (lane ^ jj).  The stages up to 512 then take one segment pass, and each
+// later stage one pass after its memory phases.
+constexpr int kSortSeg = 8 * kWave;
+
+__device__ __forceinline__ int64_t xor_lane64(int64_t v, int m) {
+  const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, m, kWave);
+  const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), m, kWave);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// The direction bit (i & kk) of key base + 64 e + lane: the lane's bit when
+// kk < 64 (the same for every e), otherwise a bit of base + 64 e (uniform
+// over the wave).
+template <int JJ>
+__device__ __forceinline__ void seg_phase(int64_t (&v)[8], int base, int lane, int kk, bool desc) {
+  if constexpr (JJ >= kWave) {
+    constexpr int D = JJ / kWave;  // kk > JJ >= 64
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int pr = min(p0 + u * kBlock, half - 1);
-          lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
-          va[u] = keys[lo[u]];
-          vb[u] = keys[lo[u] + jj];
-        }
+    for (int e = 0; e < 8; ++e) {
+      if (e & D) continue;
+      const bool asc = (((base + kWave * e) & kk) == 0) != desc;
+      const int64_t a = v[e], b = v[e + D];
+      // keys are distinct apart from KEY_NONE padding: (a > b) == asc is
+      // the swap test of either direction (branch-free selects)
+      const bool sw = (a > b) == asc;
+      v[e] = sw ? b : a;
+      v[e + D] = sw ? a : b;
+    }
+  } else {
+    const bool lower = (lane & JJ) == 0;
+    int64_t q[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const bool up = (lo[u] & kk) == 0;
-          if (p0 + u * kBlock < half && (up ? (va[u] > vb[u]) : (va[u] < vb[u]))) {
-            keys[lo[u]] = vb[u];
-            keys[lo[u] + jj] = va[u];
-          }
-        }
+    for (int e = 0; e < 8; ++e) q[e] = xor_lane64(v[e], JJ);
+    // the lower key of a pair keeps the minimum in an ascending block
+    if (kk < kWave) {
+      const bool tmin = lower == (((lane & kk) == 0) != desc);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = ((q[e] < v[e]) == tmin) ? q[e] : v[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const bool asc = (((base + kWave * e) & kk) == 0) != desc;
+        v[e] = ((q[e] < v[e]) == (lower == asc)) ? q[e] : v[e];
       }
-      __syncthreads();
     }
   }
+}
+
+// phases min(jj_hi, 256) .. 1 of stage kk on one segment (jj_hi uniform)
+__device__ __forceinline__ void seg_phases(int64_t (&v)[8], int base, int lane, int kk, int jj_hi,
+                                           bool desc) {
+  if (jj_hi >= 256) seg_phase<256>(v, base, lane, kk, desc);
+  if (jj_hi >= 128) seg_phase<128>(v, base, lane, kk, desc);
+  if (jj_hi >= 64) seg_phase<64>(v, base, lane, kk, desc);
+  if (jj_hi >= 32) seg_phase<32>(v, base, lane, kk, desc);
+  if (jj_hi >= 16) seg_phase<16>(v, base, lane, kk, desc);
+  if (jj_hi >= 8) seg_phase<8>(v, base, lane, kk, desc);
+  if (jj_hi >= 4) seg_phase<4>(v, base, lane, kk, desc);
+  if (jj_hi >= 2) seg_phase<2>(v, base, lane, kk, desc);
+  seg_phase<1>(v, base, lane, kk, desc);
+}
+
+// Stages kk0 .. kk1 (powers of two) on every segment of k[0..P), phases
+// below 512 only (those of a stage above 512 follow its memory phases).
+__device__ __forceinline__ void seg_pass(int64_t* k, int P, int kk0, int kk1, bool desc) {
+  const int lane = threadIdx.x % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // uniform segment bases
+  for (int base = wave * kSortSeg; base < P; base += (kBlock / kWave) * kSortSeg) {
+    int64_t v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = k[base + kWave * e + lane];
+    for (int kk = kk0; kk <= kk1; kk <<= 1) seg_phases(v, base, lane, kk, kk >> 1, desc);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) k[base + kWave * e + lane] = v[e];
+  }
+}
+
+// Phase jj (>= 512) of stage kk through memory: P/2 compare-swaps, up to four
+// pairs per thread loaded before any is compared.
+__device__ __forceinline__ void pair_phase(int64_t* k, int P, int kk, int jj, bool desc) {
+  const int half = P >> 1, lj = 31 - __clz(jj);
+  for (int p0 = threadIdx.x; p0 < half; p0 += 4 * kBlock) {
+    int lo[4];
+    int64_t va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int pr = min(p0 + u * kBlock, half - 1);
+      lo[u] = ((pr >> lj) << (lj + 1)) | (pr & (jj - 1));
+      va[u] = k[lo[u]];
+      vb[u] = k[lo[u] + jj];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool sw = (va[u] > vb[u]) == (((lo[u] & kk) == 0) != desc);
+      if (p0 + u * kBlock < half) {
+        k[lo[u]] = sw ? vb[u] : va[u];
+        k[lo[u] + jj] = sw ? va[u] : vb[u];
+      }
+    }
+  }
+}
+
+// Sort of k[0..P), P a power of two >= 512, ascending or descending.
+__device__ __forceinline__ void sort_block(int64_t* k, int P, bool desc) {
+  seg_pass(k, P, 2, min(P, kSortSeg), desc);
+  __syncthreads();
+  for (int kk = 2 * kSortSeg; kk <= P; kk <<= 1) {
+    for (int jj = kk >> 1; jj >= kSortSeg; jj >>= 1) {
+      pair_phase(k, P, kk, jj, desc);
+      __syncthreads();
+    }
+    seg_pass(k, P, kk, kk, desc);
+    __syncthreads();
+  }
+}
+
+// Ascending sort of a bitonic sequence k[0..P) (the last stage of a sort).
+__device__ __forceinline__ void merge_block(int64_t* k, int P) {
+  for (int jj = P >> 1; jj >= kSortSeg; jj >>= 1) {
+    pair_phase(k, P, 2 * P, jj, false);
+    __syncthreads();
+  }
+  seg_pass(k, P, 2 * P, 2 * P, false);
+  __syncthreads();
+}
+
+// Ascending sort of keys[0..M) (keys[M..P) padded with KEY_NONE, P the
+// power of two at or above max(M, 512)).
+__device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
+  int P = kSortSeg;
+  while (P < M) P <<= 1;
+  for (int t = M + threadIdx.x; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
+  __syncthreads();
+  sort_block(keys, P, false);
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
+// n keys from global memory into LDS, eight loads in flight per thread;
+// those from index `valid` on read as KEY_NONE.
+__device__ __forceinline__ void keys_to_lds(int64_t* sb, const int64_t* g, int n, int valid) {
+  for (int t0 = threadIdx.x; t0 < n; t0 += 8 * kBlock) {
+    int64_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * kBlock;
+      v[u] = (t < n && t < valid) ? ldg(g + t) : MLP_KEY_NONE;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u * kBlock < n) sb[t0 + u * kBlock] = v[u];
+  }
+}
+
+__device__ __forceinline__ void keys_to_global(int64_t* g, const int64_t* sb, int n) {
+  for (int t = threadIdx.x; t < n; t += kBlock) stg(g + t, sb[t]);
 }
 
 // A merge whose working arrays are in global memory still sorts up to
@@ -879,6 +1009,8 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
                                            int64_t* keys, int64_t* lds, const uint16_t* ctxL,
                                            uint32_t* bits, MergeShared& S, int64_t*& sk) {
   const int tid = threadIdx.x;
+  const int64_t* gtail = nullptr;  // sorted keys beyond the LDS buffer
+  int tail = INT32_MAX;
   // Up to one key per lane of a wave, keys in LDS: rank sort on wave 0, which
   // also runs the greedy next (the keys are distinct; a key's rank is the
   // number of smaller keys; every lane reads all keys before any lane
@@ -908,9 +1040,50 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
     // fit the LDS sort buffer: sort them there, and let the greedy and the
     // apply read them from LDS
     int64_t* sb = lds;
-    for (int t = tid; t < M; t += kBlock) sb[t] = keys[t];
+    keys_to_lds(sb, keys, M, M);
     bitonic_keys(sb, M);
     sk = sb;
+  } else if (GMEM && M <= 2 * kGmemSortCap) {
+    // twice the LDS buffer (keys[M..2 cap) in the slot's memory are padded):
+    // the lower half sorted ascending and the upper half descending in LDS,
+    // the last stage's first phase in global memory, then each half's
+    // remaining phases in LDS, the upper half first: the lower one stays in
+    // LDS for the greedy (the kept moves, at most N/2, fit below it)
+    int64_t* sb = lds;
+    constexpr int H = kGmemSortCap;
+    for (int h = 0; h < 2; ++h) {
+      keys_to_lds(sb, keys + h * H, H, M - h * H);
+      __syncthreads();
+      sort_block(sb, H, h == 1);
+      keys_to_global(keys + h * H, sb, H);
+      __syncthreads();
+    }
+    for (int t0 = tid; t0 < H; t0 += 4 * kBlock) {
+      int64_t a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = ldg(keys + t0 + u * kBlock);
+        b[u] = ldg(keys + t0 + u * kBlock + H);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        stg(keys + t0 + u * kBlock, min(a[u], b[u]));
+        stg(keys + t0 + u * kBlock + H, max(a[u], b[u]));
+      }
+    }
+    __syncthreads();
+    for (int h = 1; h >= 0; --h) {
+      keys_to_lds(sb, keys + h * H, H, H);
+      __syncthreads();
+      merge_block(sb, H);
+      if (h == 1) {
+        keys_to_global(keys + H, sb, H);
+        __syncthreads();
+      }
+    }
+    sk = sb;
+    gtail = keys;
+    tail = H;
   } else {
     bitonic_keys(keys, M);
   }
@@ -918,7 +1091,7 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
   if (tid < kWave) {
     int kept = 0;
 #define MLP_GREEDY(NB) \
-  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, sk, ctxL, bits, tid); break
+  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, sk, ctxL, bits, tid, gtail, tail); break
     switch (nb) {
       MLP_GREEDY(0); MLP_GREEDY(1); MLP_GREEDY(2); MLP_GREEDY(3); MLP_GREEDY(4);
       MLP_GREEDY(5); MLP_GREEDY(6); MLP_GREEDY(7); MLP_GREEDY(8); MLP_GREEDY(9);
@@ -1024,6 +1197,9 @@ __device__ __forceinline__ void merge_slot(const SlotDev& s, int nb, int rowmap,
   }
 }
 
+// One workgroup per slot.  Dynamic LDS: keys[P] int64 | told[N] | tnew[N] | bits | ctx[P].
+// tail != 0: this is the last merge of a DVND round, and it also ends the
+// round for each active slot (round_tail, max_rounds = tail - 1).
 __global__ __launch_bounds__(kBlock) void k_merge_apply(const SlotDev* __restrict__ table,
                                                         SlotListInline list,
                                                         const int32_t* list_dev, int nb,
