@@ -46,18 +46,23 @@ __host__ __device__ inline bool full_word(const uint32_t* bits, int n, int w) {
   return (full[w >> 5] >> (w & 31)) & 1u;
 }
 
-// Is position p claimed?
+// Is position p claimed?  (both words read before either is tested: no
+// short-circuit, so the two loads are in flight together)
 __host__ __device__ inline bool claimed(const uint32_t* bits, int n, int p) {
-  return ((bits[p >> 5] >> (p & 31)) & 1u) || full_word(bits, n, p >> 5);
+  const uint32_t w = bits[p >> 5];
+  const bool f = full_word(bits, n, p >> 5);
+  return (((w >> (p & 31)) & 1u) != 0) | f;
 }
 
-// Is any position of [a, b] claimed?  (n: the tour's positions)
+// Is any position of [a, b] claimed?  (n: the tour's positions; the four
+// end words are read before any is tested)
 __host__ __device__ inline bool claims_any(const uint32_t* bits, int n, int a, int b) {
   const int wa = a >> 5, wb = b >> 5;
   const uint32_t ma = 0xffffffffu << (a & 31), mb = 0xffffffffu >> (31 - (b & 31));
-  if (wa == wb) return (bits[wa] & ma & mb) || full_word(bits, n, wa);
-  if ((bits[wa] & ma) || (bits[wb] & mb) || full_word(bits, n, wa) || full_word(bits, n, wb))
-    return true;
+  const uint32_t xa = bits[wa], xb = bits[wb];
+  const bool fa = full_word(bits, n, wa), fb = full_word(bits, n, wb);
+  if (wa == wb) return ((xa & ma & mb) != 0) | fa;
+  if (((xa & ma) != 0) | ((xb & mb) != 0) | fa | fb) return true;
   // the words between: any claimed position at all
   return wb - wa > 1 && range_any(bits + claim_words0(n), wa + 1, wb - 1);
 }

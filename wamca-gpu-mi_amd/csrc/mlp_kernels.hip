@@ -713,6 +713,14 @@ __device__ __forceinline__ bool conflicts_nb(int i1, int j1, int i2, int j2) {
   return ((int)(max(i1, j1) + K < min(i2, j2)) | (int)(min(i1, j1) > max(i2, j2) + K)) == 0;
 }
 
+// The conflict predicates by kind (0 swap, 1 2-opt, 2 or-opt-k with k
+// given at run time).
+template <int KIND>
+__device__ __forceinline__ bool conflicts_kind(int k, int i1, int j1, int i2, int j2) {
+  if constexpr (KIND < 2) return conflicts_nb<KIND>(i1, j1, i2, j2);
+  return ((int)(max(i1, j1) + k < min(i2, j2)) | (int)(min(i1, j1) > max(i2, j2) + k)) == 0;
+}
+
 // Greedy over the M candidates sorted by (cost, row), on one wave, 64 at a
 // time.  The bitmap of positions claimed by kept moves is an exact conflict
 // test against every move kept so far (swap: positions i-1..i+1, j-1..j+1;
@@ -721,30 +729,43 @@ __device__ __forceinline__ bool conflicts_nb(int i1, int j1, int i2, int j2) {
 // kept ones.  Kept moves go to keys[q]
 // as (i << 32 | j) and to the slot's kept arrays.  Returns their count.
 // Candidates from index `tail` on are read from gtail (a sort whose keys
-// exceed the LDS buffer); kept moves are written below `tail`.
-template <int NB>
-__device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* keys,
-                           const uint16_t* ctxL, uint32_t* bits, int lane,
-                           const int64_t* gtail = nullptr, int tail = INT32_MAX) {
-  constexpr int K = NB >= 2 ? NB - 1 : 0;
+// exceed the LDS buffer); kept moves are written below `tail`.  The next
+// batch's keys and ctx and the key after it are read while a batch is
+// decided (kept moves overwrite only keys of decided batches).
+template <int KIND>
+__device__ __forceinline__ int greedy_wave(const SlotDev& s, int nb, int N, int rowmap, int M,
+                                           int64_t* keys, const uint16_t* ctxL, uint32_t* bits,
+                                           int lane, const int64_t* gtail, int tail) {
+  const int K = KIND == 2 ? nb - 1 : 0;
+  auto key_at = [&](int t) -> int64_t {
+    return t < M ? (t < tail ? keys[t] : ldg(gtail + t)) : MLP_KEY_NONE;
+  };
+  auto ctx_at = [&](int t, int64_t key) -> int {
+    return t < M ? (int)ctxL[(int)(key & MLP_KEY_MASK)] : 0;
+  };
+  // slot fields in registers: the loop's global stores would otherwise make
+  // the compiler reload them (a global load on every batch)
+  int64_t* const kcost = s.kept_cost[nb];
+  int32_t* const kij = s.kept_ij[nb];
+  int64_t skc = key_at(lane);
+  int ctxc = ctx_at(lane, skc);
+  int64_t skn = key_at(kWave + lane);
   int kept_total = 0;
   for (int base = 0; base < M; base += kWave) {
     const int t = base + lane;
     const bool valid = t < M;
-    int i = 0, j = 0, row = 0;
-    int64_t cost = 0;
-    if (valid) {
-      const int64_t sk = t < tail ? keys[t] : ldg(gtail + t);
-      cost = sk >> MLP_KEY_SHIFT;
-      row = (int)(sk & MLP_KEY_MASK);
-      move_ij(NB, rowmap, N, row, (int)ctxL[row], i, j);
-    }
-    const int a = NB == 1 ? i : min(i, j);
-    const int b = NB == 1 ? j + 1 : max(i, j) + K;
+    const int ctxn = ctx_at(t + kWave, skn);
+    const int64_t sknn = key_at(t + 2 * kWave);
+    int i = 0, j = 0;
+    const int row = (int)(skc & MLP_KEY_MASK);
+    const int64_t cost = skc >> MLP_KEY_SHIFT;
+    if (valid) move_ij(nb, rowmap, N, row, ctxc, i, j);
+    const int a = KIND == 1 ? i : min(i, j);
+    const int b = KIND == 1 ? j + 1 : max(i, j) + K;
     bool pre = valid;
     if (valid) {
-      if (NB == 0) pre = !claimed(bits, s.N, i) && !claimed(bits, s.N, j);
-      else pre = !claims_any(bits, s.N, a, b);
+      if (KIND == 0) pre = !(claimed(bits, N, i) | claimed(bits, N, j));
+      else pre = !claims_any(bits, N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
     // conflicts with leaves the batch (the predicates are symmetric), so the
@@ -754,24 +775,27 @@ __device__ int greedy_wave(const SlotDev& s, int N, int rowmap, int M, int64_t* 
       const int q = __builtin_ctzll(m);
       kept |= 1ull << q;
       const int iq = __builtin_amdgcn_readlane(i, q), jq = __builtin_amdgcn_readlane(j, q);
-      const bool c = conflicts_nb<NB>(iq, jq, i, j);
+      const bool c = conflicts_kind<KIND>(K, iq, jq, i, j);
       m &= ~(__ballot(c) | (1ull << q));
     }
     if ((kept >> lane) & 1ull) {
-      if (NB == 0) {
-        claims_set(bits, s.N, i - 1, i + 1);
-        claims_set(bits, s.N, j - 1, j + 1);
+      if (KIND == 0) {
+        claims_set(bits, N, i - 1, i + 1);
+        claims_set(bits, N, j - 1, j + 1);
       } else {
-        claims_set(bits, s.N, a, b);
+        claims_set(bits, N, a, b);
       }
       const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
       keys[q] = ((int64_t)i << 32) | (uint32_t)j;  // q <= t: already consumed
-      s.kept_cost[NB][q] = cost;
-      s.kept_ij[NB][3 * q + 0] = i;
-      s.kept_ij[NB][3 * q + 1] = j;
-      s.kept_ij[NB][3 * q + 2] = row;
+      kcost[q] = cost;
+      kij[3 * q + 0] = i;
+      kij[3 * q + 1] = j;
+      kij[3 * q + 2] = row;
     }
     kept_total += __popcll(kept);
+    skc = skn;
+    ctxc = ctxn;
+    skn = sknn;
   }
   return kept_total;
 }
@@ -788,9 +812,10 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
                                            int32_t* tnew, int& s_long, int* s_longq,
                                            bool loaded = false, bool copy_back = true) {
   const int N = s.N, tid = threadIdx.x, k = nb_k(nb);
+  int32_t* const tour = s.tour;  // in a register across the loops' stores
   if (!loaded)
     for (int p = tid; p < N; p += kBlock) {
-      const int c = s.tour[p];
+      const int c = tour[p];
       told[p] = c;
       tnew[p] = c;
     }
@@ -814,7 +839,7 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
   }
   __syncthreads();
   if (copy_back)
-    for (int p = tid; p < N; p += kBlock) s.tour[p] = tnew[p];
+    for (int p = tid; p < N; p += kBlock) tour[p] = tnew[p];
 }
 
 }  // namespace
@@ -1090,15 +1115,9 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
   // 3. greedy filter on wave 0
   if (tid < kWave) {
     int kept = 0;
-#define MLP_GREEDY(NB) \
-  case NB: kept = greedy_wave<NB>(s, N, rowmap, M, sk, ctxL, bits, tid, gtail, tail); break
-    switch (nb) {
-      MLP_GREEDY(0); MLP_GREEDY(1); MLP_GREEDY(2); MLP_GREEDY(3); MLP_GREEDY(4);
-      MLP_GREEDY(5); MLP_GREEDY(6); MLP_GREEDY(7); MLP_GREEDY(8); MLP_GREEDY(9);
-      MLP_GREEDY(10); MLP_GREEDY(11); MLP_GREEDY(12); MLP_GREEDY(13); MLP_GREEDY(14);
-      default: break;
-    }
-#undef MLP_GREEDY
+    if (nb == 0) kept = greedy_wave<0>(s, nb, N, rowmap, M, sk, ctxL, bits, tid, gtail, tail);
+    else if (nb == 1) kept = greedy_wave<1>(s, nb, N, rowmap, M, sk, ctxL, bits, tid, gtail, tail);
+    else if (nb < MLP_NB_MAX) kept = greedy_wave<2>(s, nb, N, rowmap, M, sk, ctxL, bits, tid, gtail, tail);
     if (tid == 0) S.kept = kept;
   }
   __syncthreads();
@@ -1115,6 +1134,8 @@ __device__ __forceinline__ void merge_rows(const SlotDev& s, const int64_t* rk, 
                                            uint16_t* ctxL, Put put) {
   constexpr int P = 8;
   const int N = s.N, tid = threadIdx.x;
+  const int32_t* const tour = s.tour;  // in registers across the loop's stores
+  const int32_t* const xy = s.xy;
   const int rend = min(R, q1), nend = min(N, q1);
   for (int base = q0; base < q1; base += kBlock * P) {
     int64_t key[P];
@@ -1123,8 +1144,8 @@ __device__ __forceinline__ void merge_rows(const SlotDev& s, const int64_t* rk, 
     for (int e = 0; e < P; ++e) {
       const int q = base + e * kBlock + tid;
       key[e] = q < rend ? rk[q] : MLP_KEY_NONE;
-      c[e] = q < nend ? s.tour[q] : 0;
-      xq[e] = q < nend ? s.xy[q] : 0;
+      c[e] = q < nend ? tour[q] : 0;
+      xq[e] = q < nend ? xy[q] : 0;
     }
 #pragma unroll
     for (int e = 0; e < P; ++e) {
@@ -1488,6 +1509,10 @@ template <int NB>
 __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t* bits, int lane,
                            int base_kept) {
   constexpr int K = NB >= 2 ? NB - 1 : 0;
+  // slot fields in registers (the loop's global stores would reload them)
+  const int n = s.N;
+  int64_t* const kcost = s.kept_cost[NB];
+  int32_t* const kij = s.kept_ij[NB];
   int kept_total = base_kept;
   for (int base = 0; base < S; base += kWave) {
     const int t = base + lane;
@@ -1497,8 +1522,8 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
     const int b = NB == 1 ? j + 1 : max(i, j) + K;
     bool pre = valid;
     if (valid) {
-      if (NB == 0) pre = !claimed(bits, s.N, i) && !claimed(bits, s.N, j);
-      else pre = !claims_any(bits, s.N, a, b);
+      if (NB == 0) pre = !claimed(bits, n, i) && !claimed(bits, n, j);
+      else pre = !claims_any(bits, n, a, b);
     }
     // the first remaining candidate is kept and every candidate it
     // conflicts with leaves the batch (the predicates are symmetric), so the
@@ -1513,16 +1538,16 @@ __device__ int greedy_list(const SlotDev& s, int S, const TotalLds& T, uint32_t*
     }
     if ((kept >> lane) & 1ull) {
       if (NB == 0) {
-        claims_set(bits, s.N, i - 1, i + 1);
-        claims_set(bits, s.N, j - 1, j + 1);
+        claims_set(bits, n, i - 1, i + 1);
+        claims_set(bits, n, j - 1, j + 1);
       } else {
-        claims_set(bits, s.N, a, b);
+        claims_set(bits, n, a, b);
       }
       const int q = kept_total + __popcll(kept & ((1ull << lane) - 1ull));
-      s.kept_cost[NB][q] = T.cc[t];
-      s.kept_ij[NB][3 * q + 0] = i;
-      s.kept_ij[NB][3 * q + 1] = j;
-      s.kept_ij[NB][3 * q + 2] = T.ce[t];
+      kcost[q] = T.cc[t];
+      kij[3 * q + 0] = i;
+      kij[3 * q + 1] = j;
+      kij[3 * q + 2] = T.ce[t];
     }
     kept_total += __popcll(kept);
   }
