@@ -55,13 +55,18 @@ def _run(eng, jobs, rowmap, max_rounds=0, rnd=1):
 @pytest.mark.parametrize("chunk,graph,persistent", [(1, False, False), (4, True, False),
                                                     (3, True, True), (8, False, True)])
 @pytest.mark.parametrize("rowmap", [0, 1])
-def test_dvnd_forms_match_oracle(eng, chunk, graph, persistent, rowmap):
+@pytest.mark.parametrize("groups", [1, 2])
+def test_dvnd_forms_match_oracle(eng, chunk, graph, persistent, rowmap, groups):
+    # groups: one stream, or the slots split over two streams (the persistent
+    # form keeps one)
     jobs = _jobs([40, 97, 180, 333, 611], 900 + rowmap)
     eng.dvnd_config(chunk, graph, persistent)
+    eng.dvnd_groups(groups)
     try:
         got = _run(eng, jobs, rowmap)
     finally:
         eng.dvnd_config(4, True, False)
+        eng.dvnd_groups(2)
     for q, (xs, ys, W, t0) in enumerate(jobs):
         tref, Lref, rref, _ = O.dvnd(W, t0, rowmap)
         assert got[q][1] == Lref, (q, chunk, graph, persistent)

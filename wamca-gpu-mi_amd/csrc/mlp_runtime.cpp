@@ -228,9 +228,14 @@ struct mlp_ctx {
   };
   std::vector<std::vector<GraphOp>> user_graph_ops;
   std::vector<GraphOp> capture_ops;
-  hipGraphExec_t graph_exec = nullptr;
-  std::string graph_key;
-  int graph_uses = 0;
+  // cached DVND chunk graphs, one per slot group of a DVND call
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  std::string graph_key[2];
+  // DVND slot groups: the largest slots on the context stream, the rest on
+  // a second stream, so one group's merges overlap the other's sweeps
+  int dvnd_groups = 2;
+  hipStream_t stream_main = nullptr, stream_alt = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void* scratch = nullptr;  // device address of this GPU's copy of the static scratch array
   // caller-owned device buffers (mlp_dev_alloc) and event-pair timers
   std::vector<void*> dev_bufs;
@@ -277,8 +282,9 @@ bool slot_ok(mlp_ctx* ctx, int s) {
 // area, reused (after the stream drains) when it is full.
 int stage_alloc(mlp_ctx* ctx, size_t bytes, char** h) {
   if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
-    // every copy out of the staging area is done once the stream drains
-    HCK(hipStreamSynchronize(ctx->stream));
+    // every copy out of the staging area is done once the streams drain
+    HCK(hipStreamSynchronize(ctx->stream_main));
+    if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
     ctx->h_lists_off = 0;
     if (bytes > ctx->h_lists_cap) {
       if (ctx->h_lists) hipHostFree(ctx->h_lists);
@@ -309,12 +315,14 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     return MLP_OK;
   }
   if (ctx->lists.size() >= 8192 && !ctx->capturing) {
-    HCK(hipStreamSynchronize(ctx->stream));
-    if (ctx->graph_exec) {  // its launches point into the lists
-      HCK(hipGraphExecDestroy(ctx->graph_exec));
-      ctx->graph_exec = nullptr;
-      ctx->graph_key.clear();
-    }
+    HCK(hipStreamSynchronize(ctx->stream_main));
+    if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
+    for (int g = 0; g < 2; ++g)
+      if (ctx->graph_exec[g]) {  // its launches point into the lists
+        HCK(hipGraphExecDestroy(ctx->graph_exec[g]));
+        ctx->graph_exec[g] = nullptr;
+        ctx->graph_key[g].clear();
+      }
     bool user_graphs = false;
     for (hipGraphExec_t g : ctx->user_graphs) user_graphs = user_graphs || g;
     if (!user_graphs) {  // recorded graphs may point into the lists: keep them
@@ -464,6 +472,7 @@ int mlp_ctx_create(int device, mlp_ctx** out) {
     delete ctx;
     return MLP_EDEVICE;
   }
+  ctx->stream_main = ctx->stream;
   ctx->scratch = scratch_address(device);  // device is current
   ctx->cu_count = prop.multiProcessorCount;
   ctx->hbm = (int64_t)prop.totalGlobalMem;
@@ -497,7 +506,9 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   for (void* p : ctx->pool_all) hipFree(p);
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
-  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->stream_alt) hipStreamSynchronize(ctx->stream_alt);
+  for (hipGraphExec_t g : ctx->graph_exec)
+    if (g) hipGraphExecDestroy(g);
   for (hipGraphExec_t g : ctx->user_graphs)
     if (g) hipGraphExecDestroy(g);
   if (ctx->d_counter) hipFree(ctx->d_counter);
@@ -515,8 +526,11 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
       hipEventDestroy(t.first);
       hipEventDestroy(t.second);
     }
-  hipStreamDestroy(ctx->stream);
+  hipStreamDestroy(ctx->stream_main);
   if (ctx->side) hipStreamDestroy(ctx->side);
+  if (ctx->stream_alt) hipStreamDestroy(ctx->stream_alt);
+  if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
   delete ctx;
   return MLP_OK;
 }
@@ -1010,12 +1024,17 @@ int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask
 // Forget the cached DVND chunk graph (its launches embed the neighborhood
 // set, tile choices and merge mode of the moment it was captured).
 int drop_dvnd_graph(mlp_ctx* ctx) {
-  if (ctx->graph_exec) {
-    if (!ctx->capturing) HCK(hipStreamSynchronize(ctx->stream));
-    HCK(hipGraphExecDestroy(ctx->graph_exec));
-    ctx->graph_exec = nullptr;
+  for (int g = 0; g < 2; ++g) {
+    if (ctx->graph_exec[g]) {
+      if (!ctx->capturing) {
+        HCK(hipStreamSynchronize(ctx->stream_main));
+        if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
+      }
+      HCK(hipGraphExecDestroy(ctx->graph_exec[g]));
+      ctx->graph_exec[g] = nullptr;
+    }
+    ctx->graph_key[g].clear();
   }
-  ctx->graph_key.clear();
   return MLP_OK;
 }
 
@@ -1913,7 +1932,7 @@ int enqueue_rounds(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, in
 }
 
 int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
-              int rounds) {
+              int rounds, int gc = 0) {
   if (!ctx->dvnd_graph || ctx->user_capture)
     return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   int tile = ctx->tile_override;  // (the per-neighborhood shapes follow from the slots)
@@ -1928,8 +1947,8 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
     const int32_t Nn[2] = {ctx->slots[slots[q]].N, ctx->slots[slots[q]].nbs};
     key.append((const char*)Nn, sizeof Nn);
   }
-  if (key == ctx->graph_key && ctx->graph_exec) {
-    HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  if (key == ctx->graph_key[gc] && ctx->graph_exec[gc]) {
+    HCK(hipGraphLaunch(ctx->graph_exec[gc], ctx->stream));
     // the launches' host bookkeeping (evaluated / merged flags)
     for (int q = 0; q < count; ++q) {
       Slot& s = ctx->slots[slots[q]];
@@ -1941,12 +1960,12 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
     }
     return MLP_OK;
   }
-  if (key != ctx->graph_key) {  // first sight: run eagerly, remember the key
-    if (ctx->graph_exec) {
-      HCK(hipGraphExecDestroy(ctx->graph_exec));
-      ctx->graph_exec = nullptr;
+  if (key != ctx->graph_key[gc]) {  // first sight: run eagerly, remember the key
+    if (ctx->graph_exec[gc]) {
+      HCK(hipGraphExecDestroy(ctx->graph_exec[gc]));
+      ctx->graph_exec[gc] = nullptr;
     }
-    ctx->graph_key = key;
+    ctx->graph_key[gc] = key;
     return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   }
   hipGraph_t g = nullptr;
@@ -1957,20 +1976,60 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
   if (st || ec != hipSuccess) {  // nothing captured ran: run the chunk eagerly instead
     if (g) hipGraphDestroy(g);
-    ctx->graph_key.clear();
+    ctx->graph_key[gc].clear();
     return enqueue_rounds(ctx, slots, count, rowmap, max_rounds, rounds);
   }
-  const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec[gc], g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (ei != hipSuccess) {
-    ctx->graph_exec = nullptr;
+    ctx->graph_exec[gc] = nullptr;
     return fail(ctx, MLP_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
   }
-  HCK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  HCK(hipGraphLaunch(ctx->graph_exec[gc], ctx->stream));
   return MLP_OK;
 }
 
 }  // namespace
+
+namespace {
+
+// The largest slots (by N^2, the work of a round) holding at least 40% of
+// the active set's work in g0, the others in g1; slot order is kept inside
+// each group.
+int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, std::vector<int32_t>& g0,
+                 std::vector<int32_t>& g1) {
+  const int c = (int)active.size();
+  std::vector<int> order((size_t)c);
+  double total = 0;
+  for (int q = 0; q < c; ++q) {
+    order[(size_t)q] = q;
+    const double N = ctx->slots[active[(size_t)q]].N;
+    total += N * N;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return ctx->slots[active[(size_t)a]].N > ctx->slots[active[(size_t)b]].N;
+  });
+  std::vector<char> first((size_t)c, 0);
+  double acc = 0;
+  for (int k = 0; k < c - 1 && acc < 0.4 * total; ++k) {
+    const double N = ctx->slots[active[(size_t)order[(size_t)k]]].N;
+    acc += N * N;
+    first[(size_t)order[(size_t)k]] = 1;
+  }
+  g0.clear();
+  g1.clear();
+  for (int q = 0; q < c; ++q) (first[(size_t)q] ? g0 : g1).push_back(active[(size_t)q]);
+  return MLP_OK;
+}
+
+}  // namespace
+
+int mlp_dvnd_groups(mlp_ctx* ctx, int32_t groups) {
+  if (!ctx) return MLP_EINVAL;
+  if (groups < 1 || groups > 2) return fail(ctx, MLP_EINVAL, "groups must be 1 or 2");
+  ctx->dvnd_groups = groups;
+  return MLP_OK;
+}
 
 int mlp_dvnd_start(mlp_ctx* ctx, const int32_t* slots, int32_t count) {
   int st = check_slots(ctx, slots, count);
@@ -2046,11 +2105,37 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
   if (st) return st;
   st = mlp_dvnd_start(ctx, slots, count);
   if (st) return st;
+  // two slot groups on two streams: the fused and persistent forms share
+  // device counters across slots, so they keep one stream
+  const bool grouped = ctx->dvnd_groups > 1 && !ctx->dvnd_fused && !ctx->dvnd_persistent;
+  if (grouped && !ctx->stream_alt) {
+    if (hipStreamCreateWithFlags(&ctx->stream_alt, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+      return fail(ctx, MLP_EDEVICE, "second DVND stream: %s", hipGetErrorString(hipGetLastError()));
+  }
   std::vector<int32_t> active(slots, slots + count), flag((size_t)count);
+  std::vector<int32_t> g0, g1;
   while (!active.empty()) {
     const int c = (int)active.size();
-    st = run_chunk(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);
-    if (st) return st;
+    if (grouped && c >= 2) {
+      st = split_groups(ctx, active, g0, g1);
+      if (st) return st;
+      // the second stream starts after the work queued so far on the first
+      HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
+      HCK(hipStreamWaitEvent(ctx->stream_alt, ctx->ev_fork, 0));
+      st = run_chunk(ctx, g0.data(), (int)g0.size(), rowmap, max_rounds, ctx->dvnd_chunk, 0);
+      if (st) return st;
+      ctx->stream = ctx->stream_alt;
+      st = run_chunk(ctx, g1.data(), (int)g1.size(), rowmap, max_rounds, ctx->dvnd_chunk, 1);
+      ctx->stream = ctx->stream_main;
+      if (st) return st;
+      HCK(hipEventRecord(ctx->ev_join, ctx->stream_alt));
+      HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join, 0));
+    } else {
+      st = run_chunk(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);
+      if (st) return st;
+    }
     st = mlp_dvnd_status(ctx, active.data(), c, flag.data(), nullptr);
     if (st) return st;
     std::vector<int32_t> next;

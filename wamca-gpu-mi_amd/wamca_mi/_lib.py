@@ -66,6 +66,7 @@ _SIGS = {
     "mlp_mem_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlp_launch_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mlp_dvnd_fused": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mlp_dvnd_groups": (C.c_int, [C.c_void_p, C.c_int32]),
     "mlp_dev_alloc": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "mlp_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mlp_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),

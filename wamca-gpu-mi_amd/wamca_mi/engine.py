@@ -242,6 +242,11 @@ class Engine:
         queue of every slot's evaluation tiles and merges)."""
         self._ck(self._L.mlp_dvnd_fused(self.h, int(on)))
 
+    def dvnd_groups(self, groups=2):
+        """mlp_dvnd over two slot groups on two streams (the largest slots,
+        and the rest), or one group (1)."""
+        self._ck(self._L.mlp_dvnd_groups(self.h, int(groups)))
+
     def dvnd_start(self, slots):
         slots = np.ascontiguousarray(slots, np.int32)
         self._ck(self._L.mlp_dvnd_start(self.h, slots, len(slots)))
