@@ -2022,6 +2022,38 @@ int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, std::vector<i
   return MLP_OK;
 }
 
+// `rounds` DVND rounds over the slots, split into two groups on two streams
+// when enabled (the fused and persistent forms share device counters across
+// slots, so they keep one stream; so does a caller's capture).  Work queued
+// after the call on the context stream follows both groups.
+int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
+                int rounds) {
+  const bool grouped = ctx->dvnd_groups > 1 && !ctx->dvnd_fused && !ctx->dvnd_persistent &&
+                       !ctx->user_capture && count >= 2;
+  if (!grouped) return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
+  if (!ctx->stream_alt) {
+    if (hipStreamCreateWithFlags(&ctx->stream_alt, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+      return fail(ctx, MLP_EDEVICE, "second DVND stream: %s", hipGetErrorString(hipGetLastError()));
+  }
+  std::vector<int32_t> active(slots, slots + count), g0, g1;
+  int st = split_groups(ctx, active, g0, g1);
+  if (st) return st;
+  // the second stream starts after the work queued so far on the first
+  HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
+  HCK(hipStreamWaitEvent(ctx->stream_alt, ctx->ev_fork, 0));
+  st = run_chunk(ctx, g0.data(), (int)g0.size(), rowmap, max_rounds, rounds, 0);
+  if (st) return st;
+  ctx->stream = ctx->stream_alt;
+  st = run_chunk(ctx, g1.data(), (int)g1.size(), rowmap, max_rounds, rounds, 1);
+  ctx->stream = ctx->stream_main;
+  if (st) return st;
+  HCK(hipEventRecord(ctx->ev_join, ctx->stream_alt));
+  HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join, 0));
+  return MLP_OK;
+}
+
 }  // namespace
 
 int mlp_dvnd_groups(mlp_ctx* ctx, int32_t groups) {
@@ -2054,7 +2086,7 @@ int mlp_dvnd_rounds(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t r
   if (rowmap != 0 && rowmap != 1) return fail(ctx, MLP_EINVAL, "rowmap must be 0 or 1");
   if (rounds < 0) return fail(ctx, MLP_EINVAL, "rounds must be >= 0");
   if (count == 0 || rounds == 0) return MLP_OK;
-  return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
+  return run_grouped(ctx, slots, count, rowmap, max_rounds, rounds);
 }
 
 int mlp_dvnd_status(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t* active,
@@ -2105,37 +2137,11 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
   if (st) return st;
   st = mlp_dvnd_start(ctx, slots, count);
   if (st) return st;
-  // two slot groups on two streams: the fused and persistent forms share
-  // device counters across slots, so they keep one stream
-  const bool grouped = ctx->dvnd_groups > 1 && !ctx->dvnd_fused && !ctx->dvnd_persistent;
-  if (grouped && !ctx->stream_alt) {
-    if (hipStreamCreateWithFlags(&ctx->stream_alt, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
-      return fail(ctx, MLP_EDEVICE, "second DVND stream: %s", hipGetErrorString(hipGetLastError()));
-  }
   std::vector<int32_t> active(slots, slots + count), flag((size_t)count);
-  std::vector<int32_t> g0, g1;
   while (!active.empty()) {
     const int c = (int)active.size();
-    if (grouped && c >= 2) {
-      st = split_groups(ctx, active, g0, g1);
-      if (st) return st;
-      // the second stream starts after the work queued so far on the first
-      HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
-      HCK(hipStreamWaitEvent(ctx->stream_alt, ctx->ev_fork, 0));
-      st = run_chunk(ctx, g0.data(), (int)g0.size(), rowmap, max_rounds, ctx->dvnd_chunk, 0);
-      if (st) return st;
-      ctx->stream = ctx->stream_alt;
-      st = run_chunk(ctx, g1.data(), (int)g1.size(), rowmap, max_rounds, ctx->dvnd_chunk, 1);
-      ctx->stream = ctx->stream_main;
-      if (st) return st;
-      HCK(hipEventRecord(ctx->ev_join, ctx->stream_alt));
-      HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join, 0));
-    } else {
-      st = run_chunk(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);
-      if (st) return st;
-    }
+    st = run_grouped(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);
+    if (st) return st;
     st = mlp_dvnd_status(ctx, active.data(), c, flag.data(), nullptr);
     if (st) return st;
     std::vector<int32_t> next;
