@@ -229,13 +229,13 @@ struct mlp_ctx {
   std::vector<std::vector<GraphOp>> user_graph_ops;
   std::vector<GraphOp> capture_ops;
   // cached DVND chunk graphs, one per slot group of a DVND call
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  std::string graph_key[2];
-  // DVND slot groups: the largest slots on the context stream, the rest on
-  // a second stream, so one group's merges overlap the other's sweeps
+  hipGraphExec_t graph_exec[MLP_DVND_GROUPS] = {};
+  std::string graph_key[MLP_DVND_GROUPS];
+  // DVND slot groups: the largest slots on the context stream, the others
+  // on streams of their own, so one group's merges overlap another's sweeps
   int dvnd_groups = 2;
-  hipStream_t stream_main = nullptr, stream_alt = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream_main = nullptr, stream_grp[MLP_DVND_GROUPS] = {};  // [0] unused
+  hipEvent_t ev_fork = nullptr, ev_join[MLP_DVND_GROUPS] = {};
   void* scratch = nullptr;  // device address of this GPU's copy of the static scratch array
   // caller-owned device buffers (mlp_dev_alloc) and event-pair timers
   std::vector<void*> dev_bufs;
@@ -280,11 +280,18 @@ bool slot_ok(mlp_ctx* ctx, int s) {
 
 // Pinned staging space for an upload ordered on the context stream: a bump
 // area, reused (after the stream drains) when it is full.
+// The context stream and the DVND group streams drained.
+hipError_t sync_streams(mlp_ctx* ctx) {
+  hipError_t e = hipStreamSynchronize(ctx->stream_main);
+  for (int g = 1; g < MLP_DVND_GROUPS && e == hipSuccess; ++g)
+    if (ctx->stream_grp[g]) e = hipStreamSynchronize(ctx->stream_grp[g]);
+  return e;
+}
+
 int stage_alloc(mlp_ctx* ctx, size_t bytes, char** h) {
   if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
     // every copy out of the staging area is done once the streams drain
-    HCK(hipStreamSynchronize(ctx->stream_main));
-    if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
+    HCK(sync_streams(ctx));
     ctx->h_lists_off = 0;
     if (bytes > ctx->h_lists_cap) {
       if (ctx->h_lists) hipHostFree(ctx->h_lists);
@@ -315,9 +322,8 @@ int device_list(mlp_ctx* ctx, const int32_t* v, int count, const int32_t** out) 
     return MLP_OK;
   }
   if (ctx->lists.size() >= 8192 && !ctx->capturing) {
-    HCK(hipStreamSynchronize(ctx->stream_main));
-    if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
-    for (int g = 0; g < 2; ++g)
+    HCK(sync_streams(ctx));
+    for (int g = 0; g < MLP_DVND_GROUPS; ++g)
       if (ctx->graph_exec[g]) {  // its launches point into the lists
         HCK(hipGraphExecDestroy(ctx->graph_exec[g]));
         ctx->graph_exec[g] = nullptr;
@@ -506,7 +512,7 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   for (void* p : ctx->pool_all) hipFree(p);
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
-  if (ctx->stream_alt) hipStreamSynchronize(ctx->stream_alt);
+  (void)sync_streams(ctx);
   for (hipGraphExec_t g : ctx->graph_exec)
     if (g) hipGraphExecDestroy(g);
   for (hipGraphExec_t g : ctx->user_graphs)
@@ -528,9 +534,11 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
     }
   hipStreamDestroy(ctx->stream_main);
   if (ctx->side) hipStreamDestroy(ctx->side);
-  if (ctx->stream_alt) hipStreamDestroy(ctx->stream_alt);
+  for (int g = 1; g < MLP_DVND_GROUPS; ++g) {
+    if (ctx->stream_grp[g]) hipStreamDestroy(ctx->stream_grp[g]);
+    if (ctx->ev_join[g]) hipEventDestroy(ctx->ev_join[g]);
+  }
   if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
   delete ctx;
   return MLP_OK;
 }
@@ -1024,12 +1032,9 @@ int auto_tile(const mlp_ctx* ctx, const int32_t* slots, int count, uint32_t mask
 // Forget the cached DVND chunk graph (its launches embed the neighborhood
 // set, tile choices and merge mode of the moment it was captured).
 int drop_dvnd_graph(mlp_ctx* ctx) {
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < MLP_DVND_GROUPS; ++g) {
     if (ctx->graph_exec[g]) {
-      if (!ctx->capturing) {
-        HCK(hipStreamSynchronize(ctx->stream_main));
-        if (ctx->stream_alt) HCK(hipStreamSynchronize(ctx->stream_alt));
-      }
+      if (!ctx->capturing) HCK(sync_streams(ctx));
       HCK(hipGraphExecDestroy(ctx->graph_exec[g]));
       ctx->graph_exec[g] = nullptr;
     }
@@ -1993,11 +1998,11 @@ int run_chunk(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max
 
 namespace {
 
-// The largest slots (by N^2, the work of a round) holding at least 40% of
-// the active set's work in g0, the others in g1; slot order is kept inside
-// each group.
-int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, std::vector<int32_t>& g0,
-                 std::vector<int32_t>& g1) {
+// Slots sorted by N (N^2 is the work of a round) from the largest, cut into
+// `groups` groups at 40% of the work and then at even shares of the rest;
+// slot order is kept inside each group, and every group has a slot.
+int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, int groups,
+                 std::vector<int32_t>* g) {
   const int c = (int)active.size();
   std::vector<int> order((size_t)c);
   double total = 0;
@@ -2009,48 +2014,60 @@ int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, std::vector<i
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return ctx->slots[active[(size_t)a]].N > ctx->slots[active[(size_t)b]].N;
   });
-  std::vector<char> first((size_t)c, 0);
+  std::vector<int> of((size_t)c, groups - 1);
   double acc = 0;
-  for (int k = 0; k < c - 1 && acc < 0.4 * total; ++k) {
-    const double N = ctx->slots[active[(size_t)order[(size_t)k]]].N;
-    acc += N * N;
-    first[(size_t)order[(size_t)k]] = 1;
+  int k = 0;
+  for (int grp = 0; grp < groups - 1; ++grp) {
+    const double cut = (0.4 + 0.6 * grp / (groups - 1)) * total;
+    // at least one slot per group, and one left for each later group
+    const int kmax = c - (groups - 1 - grp);
+    const int k0 = k;
+    while (k < kmax && (k == k0 || acc < cut)) {
+      const double N = ctx->slots[active[(size_t)order[(size_t)k]]].N;
+      acc += N * N;
+      of[(size_t)order[(size_t)k]] = grp;
+      ++k;
+    }
   }
-  g0.clear();
-  g1.clear();
-  for (int q = 0; q < c; ++q) (first[(size_t)q] ? g0 : g1).push_back(active[(size_t)q]);
+  for (int grp = 0; grp < groups; ++grp) g[grp].clear();
+  for (int q = 0; q < c; ++q) g[of[(size_t)q]].push_back(active[(size_t)q]);
   return MLP_OK;
 }
 
-// `rounds` DVND rounds over the slots, split into two groups on two streams
-// when enabled (the fused and persistent forms share device counters across
-// slots, so they keep one stream; so does a caller's capture).  Work queued
-// after the call on the context stream follows both groups.
+// `rounds` DVND rounds over the slots, split into groups on streams of
+// their own when enabled (the fused and persistent forms share device
+// counters across slots, so they keep one stream; so does a caller's
+// capture).  Work queued after the call on the context stream follows every
+// group.
 int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int max_rounds,
                 int rounds) {
-  const bool grouped = ctx->dvnd_groups > 1 && !ctx->dvnd_fused && !ctx->dvnd_persistent &&
-                       !ctx->user_capture && count >= 2;
-  if (!grouped) return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
-  if (!ctx->stream_alt) {
-    if (hipStreamCreateWithFlags(&ctx->stream_alt, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
-      return fail(ctx, MLP_EDEVICE, "second DVND stream: %s", hipGetErrorString(hipGetLastError()));
-  }
-  std::vector<int32_t> active(slots, slots + count), g0, g1;
-  int st = split_groups(ctx, active, g0, g1);
+  const int groups = std::min(ctx->dvnd_groups, count);
+  if (groups < 2 || ctx->dvnd_fused || ctx->dvnd_persistent || ctx->user_capture)
+    return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
+  if (!ctx->ev_fork &&
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail(ctx, MLP_EDEVICE, "DVND group event: %s", hipGetErrorString(hipGetLastError()));
+  for (int g = 1; g < groups; ++g)
+    if (!ctx->stream_grp[g] &&
+        (hipStreamCreateWithFlags(&ctx->stream_grp[g], hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_join[g], hipEventDisableTiming) != hipSuccess))
+      return fail(ctx, MLP_EDEVICE, "DVND group stream: %s", hipGetErrorString(hipGetLastError()));
+  std::vector<int32_t> active(slots, slots + count), part[MLP_DVND_GROUPS];
+  int st = split_groups(ctx, active, groups, part);
   if (st) return st;
-  // the second stream starts after the work queued so far on the first
+  // the group streams start after the work queued so far on the context stream
   HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
-  HCK(hipStreamWaitEvent(ctx->stream_alt, ctx->ev_fork, 0));
-  st = run_chunk(ctx, g0.data(), (int)g0.size(), rowmap, max_rounds, rounds, 0);
-  if (st) return st;
-  ctx->stream = ctx->stream_alt;
-  st = run_chunk(ctx, g1.data(), (int)g1.size(), rowmap, max_rounds, rounds, 1);
+  for (int g = 1; g < groups; ++g) HCK(hipStreamWaitEvent(ctx->stream_grp[g], ctx->ev_fork, 0));
+  for (int g = 0; g < groups && !st; ++g) {
+    ctx->stream = g ? ctx->stream_grp[g] : ctx->stream_main;
+    st = run_chunk(ctx, part[g].data(), (int)part[g].size(), rowmap, max_rounds, rounds, g);
+  }
   ctx->stream = ctx->stream_main;
   if (st) return st;
-  HCK(hipEventRecord(ctx->ev_join, ctx->stream_alt));
-  HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join, 0));
+  for (int g = 1; g < groups; ++g) {
+    HCK(hipEventRecord(ctx->ev_join[g], ctx->stream_grp[g]));
+    HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join[g], 0));
+  }
   return MLP_OK;
 }
 
@@ -2058,7 +2075,8 @@ int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int m
 
 int mlp_dvnd_groups(mlp_ctx* ctx, int32_t groups) {
   if (!ctx) return MLP_EINVAL;
-  if (groups < 1 || groups > 2) return fail(ctx, MLP_EINVAL, "groups must be 1 or 2");
+  if (groups < 1 || groups > MLP_DVND_GROUPS)
+    return fail(ctx, MLP_EINVAL, "groups must be in [1, %d]", MLP_DVND_GROUPS);
   ctx->dvnd_groups = groups;
   return MLP_OK;
 }
