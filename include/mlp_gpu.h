@@ -212,7 +212,7 @@ int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t pers
  * dependency costs an agent-scope release and acquire (a write-back and an
  * invalidation of the XCD's L2), so on gfx950 the per-neighborhood launches
  * replayed as a graph, which get coherence at kernel boundaries, are faster
- * (64 instances, n in [50, 2000]: 0.050 s against 0.082 s). */
+ * (64 instances, n in [50, 2000]: 0.037 s against 0.074 s). */
 int mlp_dvnd_fused(mlp_ctx* ctx, int32_t on);
 /* groups = 2 (default): mlp_dvnd and mlp_dvnd_rounds split the slots into
  * two groups (the largest slots holding 40% of the work of a round, and the
