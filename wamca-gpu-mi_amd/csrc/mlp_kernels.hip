@@ -847,13 +847,14 @@ __device__ __forceinline__ void apply_kept(const SlotDev& s, int nb, int K, IJ i
 // Bitonic sorts of 64-bit keys by the workgroup.  Stage kk (a power of two)
 // merges blocks of kk keys, element i ascending when (i & kk) == 0 (the
 // whole array flipped for a descending sort); its phase jj compares i with
-// i + jj for (i & jj) == 0.  Phases with jj >= 512 go pair by pair through
-// memory between barriers; the phases below 512 run on 512-key segments in
+// i + jj for (i & jj) == 0.  The array is cut into segments of 64 E keys
+// (E = 2, 4 or 8: every wave of the workgroup holds at least one), held in
 // the registers of one wave, key base + 64 e + lane in v[e] of that lane:
-// jj of 64..256 pairs keys inside a lane, jj of 1..32 pairs lanes
-// (lane ^ jj).  The stages up to 512 then take one segment pass, and each
+// phases with jj of 64 .. 32 E pair keys inside a lane, jj of 1..32 pair
+// lanes (lane ^ jj); phases with jj >= 64 E go pair by pair through memory
+// between barriers.  The stages up to 64 E take one segment pass, and each
 // later stage one pass after its memory phases.
-constexpr int kSortSeg = 8 * kWave;
+constexpr int kSortSeg = 8 * kWave;  // the largest segment
 
 __device__ __forceinline__ int64_t xor_lane64(int64_t v, int m) {
   const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, m, kWave);
@@ -864,12 +865,12 @@ __device__ __forceinline__ int64_t xor_lane64(int64_t v, int m) {
 // The direction bit (i & kk) of key base + 64 e + lane: the lane's bit when
 // kk < 64 (the same for every e), otherwise a bit of base + 64 e (uniform
 // over the wave).
-template <int JJ>
-__device__ __forceinline__ void seg_phase(int64_t (&v)[8], int base, int lane, int kk, bool desc) {
+template <int E, int JJ>
+__device__ __forceinline__ void seg_phase(int64_t (&v)[E], int base, int lane, int kk, bool desc) {
   if constexpr (JJ >= kWave) {
     constexpr int D = JJ / kWave;  // kk > JJ >= 64
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < E; ++e) {
       if (e & D) continue;
       const bool asc = (((base + kWave * e) & kk) == 0) != desc;
       const int64_t a = v[e], b = v[e + D];
@@ -881,17 +882,17 @@ __device__ __forceinline__ void seg_phase(int64_t (&v)[8], int base, int lane, i
     }
   } else {
     const bool lower = (lane & JJ) == 0;
-    int64_t q[8];
+    int64_t q[E];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) q[e] = xor_lane64(v[e], JJ);
+    for (int e = 0; e < E; ++e) q[e] = xor_lane64(v[e], JJ);
     // the lower key of a pair keeps the minimum in an ascending block
     if (kk < kWave) {
       const bool tmin = lower == (((lane & kk) == 0) != desc);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = ((q[e] < v[e]) == tmin) ? q[e] : v[e];
+      for (int e = 0; e < E; ++e) v[e] = ((q[e] < v[e]) == tmin) ? q[e] : v[e];
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < E; ++e) {
         const bool asc = (((base + kWave * e) & kk) == 0) != desc;
         v[e] = ((q[e] < v[e]) == (lower == asc)) ? q[e] : v[e];
       }
@@ -899,37 +900,43 @@ __device__ __forceinline__ void seg_phase(int64_t (&v)[8], int base, int lane, i
   }
 }
 
-// phases min(jj_hi, 256) .. 1 of stage kk on one segment (jj_hi uniform)
-__device__ __forceinline__ void seg_phases(int64_t (&v)[8], int base, int lane, int kk, int jj_hi,
+// phases min(jj_hi, 32 E) .. 1 of stage kk on one segment (jj_hi uniform)
+template <int E>
+__device__ __forceinline__ void seg_phases(int64_t (&v)[E], int base, int lane, int kk, int jj_hi,
                                            bool desc) {
-  if (jj_hi >= 256) seg_phase<256>(v, base, lane, kk, desc);
-  if (jj_hi >= 128) seg_phase<128>(v, base, lane, kk, desc);
-  if (jj_hi >= 64) seg_phase<64>(v, base, lane, kk, desc);
-  if (jj_hi >= 32) seg_phase<32>(v, base, lane, kk, desc);
-  if (jj_hi >= 16) seg_phase<16>(v, base, lane, kk, desc);
-  if (jj_hi >= 8) seg_phase<8>(v, base, lane, kk, desc);
-  if (jj_hi >= 4) seg_phase<4>(v, base, lane, kk, desc);
-  if (jj_hi >= 2) seg_phase<2>(v, base, lane, kk, desc);
-  seg_phase<1>(v, base, lane, kk, desc);
+  if constexpr (E >= 8) {
+    if (jj_hi >= 256) seg_phase<E, 256>(v, base, lane, kk, desc);
+  }
+  if constexpr (E >= 4) {
+    if (jj_hi >= 128) seg_phase<E, 128>(v, base, lane, kk, desc);
+  }
+  if (jj_hi >= 64) seg_phase<E, 64>(v, base, lane, kk, desc);
+  if (jj_hi >= 32) seg_phase<E, 32>(v, base, lane, kk, desc);
+  if (jj_hi >= 16) seg_phase<E, 16>(v, base, lane, kk, desc);
+  if (jj_hi >= 8) seg_phase<E, 8>(v, base, lane, kk, desc);
+  if (jj_hi >= 4) seg_phase<E, 4>(v, base, lane, kk, desc);
+  if (jj_hi >= 2) seg_phase<E, 2>(v, base, lane, kk, desc);
+  seg_phase<E, 1>(v, base, lane, kk, desc);
 }
 
 // Stages kk0 .. kk1 (powers of two) on every segment of k[0..P), phases
-// below 512 only (those of a stage above 512 follow its memory phases).
+// below 64 E only (those of a larger stage follow its memory phases).
+template <int E>
 __device__ __forceinline__ void seg_pass(int64_t* k, int P, int kk0, int kk1, bool desc) {
   const int lane = threadIdx.x % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // uniform segment bases
-  for (int base = wave * kSortSeg; base < P; base += (kBlock / kWave) * kSortSeg) {
-    int64_t v[8];
+  for (int base = wave * kWave * E; base < P; base += kBlock * E) {
+    int64_t v[E];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = k[base + kWave * e + lane];
-    for (int kk = kk0; kk <= kk1; kk <<= 1) seg_phases(v, base, lane, kk, kk >> 1, desc);
+    for (int e = 0; e < E; ++e) v[e] = k[base + kWave * e + lane];
+    for (int kk = kk0; kk <= kk1; kk <<= 1) seg_phases<E>(v, base, lane, kk, kk >> 1, desc);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) k[base + kWave * e + lane] = v[e];
+    for (int e = 0; e < E; ++e) k[base + kWave * e + lane] = v[e];
   }
 }
 
-// Phase jj (>= 512) of stage kk through memory: P/2 compare-swaps, up to four
-// pairs per thread loaded before any is compared.
+// Phase jj (>= 64 E) of stage kk through memory: P/2 compare-swaps, up to
+// four pairs per thread loaded before any is compared.
 __device__ __forceinline__ void pair_phase(int64_t* k, int P, int kk, int jj, bool desc) {
   const int half = P >> 1, lj = 31 - __clz(jj);
   for (int p0 = threadIdx.x; p0 < half; p0 += 4 * kBlock) {
@@ -953,38 +960,44 @@ __device__ __forceinline__ void pair_phase(int64_t* k, int P, int kk, int jj, bo
   }
 }
 
-// Sort of k[0..P), P a power of two >= 512, ascending or descending.
+// Sort of k[0..P), P a power of two >= 256 E, ascending or descending.
+template <int E>
 __device__ __forceinline__ void sort_block(int64_t* k, int P, bool desc) {
-  seg_pass(k, P, 2, min(P, kSortSeg), desc);
+  constexpr int SEG = kWave * E;
+  seg_pass<E>(k, P, 2, min(P, SEG), desc);
   __syncthreads();
-  for (int kk = 2 * kSortSeg; kk <= P; kk <<= 1) {
-    for (int jj = kk >> 1; jj >= kSortSeg; jj >>= 1) {
+  for (int kk = 2 * SEG; kk <= P; kk <<= 1) {
+    for (int jj = kk >> 1; jj >= SEG; jj >>= 1) {
       pair_phase(k, P, kk, jj, desc);
       __syncthreads();
     }
-    seg_pass(k, P, kk, kk, desc);
+    seg_pass<E>(k, P, kk, kk, desc);
     __syncthreads();
   }
 }
 
-// Ascending sort of a bitonic sequence k[0..P) (the last stage of a sort).
+// Ascending sort of a bitonic sequence k[0..P) (the last stage of a sort),
+// P >= 2048.
 __device__ __forceinline__ void merge_block(int64_t* k, int P) {
   for (int jj = P >> 1; jj >= kSortSeg; jj >>= 1) {
     pair_phase(k, P, 2 * P, jj, false);
     __syncthreads();
   }
-  seg_pass(k, P, 2 * P, 2 * P, false);
+  seg_pass<8>(k, P, 2 * P, 2 * P, false);
   __syncthreads();
 }
 
 // Ascending sort of keys[0..M) (keys[M..P) padded with KEY_NONE, P the
-// power of two at or above max(M, 512)).
+// power of two at or above max(M, 512)), in segments of 128, 256 or 512
+// keys so that each of the four waves holds one.
 __device__ __forceinline__ void bitonic_keys(int64_t* keys, int M) {
-  int P = kSortSeg;
+  int P = 2 * kBlock;
   while (P < M) P <<= 1;
   for (int t = M + threadIdx.x; t < P; t += kBlock) keys[t] = MLP_KEY_NONE;
   __syncthreads();
-  sort_block(keys, P, false);
+  if (P <= 2 * kBlock) sort_block<2>(keys, P, false);
+  else if (P <= 4 * kBlock) sort_block<4>(keys, P, false);
+  else sort_block<8>(keys, P, false);
 }
 
 template <typename T>
@@ -1079,7 +1092,7 @@ __device__ __forceinline__ int sort_greedy(const SlotDev& s, int nb, int rowmap,
     for (int h = 0; h < 2; ++h) {
       keys_to_lds(sb, keys + h * H, H, M - h * H);
       __syncthreads();
-      sort_block(sb, H, h == 1);
+      sort_block<8>(sb, H, h == 1);
       keys_to_global(keys + h * H, sb, H);
       __syncthreads();
     }
