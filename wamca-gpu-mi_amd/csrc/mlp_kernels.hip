@@ -2663,7 +2663,7 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
   constexpr int FT = (C + 2 + kWave - 1) / kWave;  // e entries per lane, both parts
   // Wide tiles hold both parts of every row of a tour up to C positions:
   // their two parts' entries are filled as one flat range (one distance per
-  // entry; 200 tours n=1000, 16x1024: 283 -> 270 us), narrow tiles fill each
+  // entry; 200 tours n=1000, 16x1024: 281 -> 265 us), narrow tiles fill each
   // part on its own (the flat form measured 3% slower at 4x256)
   constexpr bool FLAT = C >= 1024;
   constexpr int FP = FLAT ? 1 : 2, FI = FLAT ? FT : FE;

@@ -278,8 +278,6 @@ bool slot_ok(mlp_ctx* ctx, int s) {
   return s >= 0 && s < (int)ctx->slots.size() && ctx->slots[s].alive;
 }
 
-// Pinned staging space for an upload ordered on the context stream: a bump
-// area, reused (after the stream drains) when it is full.
 // The context stream and the DVND group streams drained.
 hipError_t sync_streams(mlp_ctx* ctx) {
   hipError_t e = hipStreamSynchronize(ctx->stream_main);
@@ -288,6 +286,8 @@ hipError_t sync_streams(mlp_ctx* ctx) {
   return e;
 }
 
+// Pinned staging space for an upload ordered on the context stream: a bump
+// area, reused (after the streams drain) when it is full.
 int stage_alloc(mlp_ctx* ctx, size_t bytes, char** h) {
   if (ctx->h_lists_off + bytes > ctx->h_lists_cap) {
     // every copy out of the staging area is done once the streams drain
@@ -508,11 +508,11 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
   }
   if (!ctx) return MLP_EINVAL;
   hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  ctx->stream = ctx->stream_main;
+  (void)sync_streams(ctx);  // every stream's work done before its memory goes
   for (void* p : ctx->pool_all) hipFree(p);
   for (auto& kv : ctx->lists) hipFree(kv.second.p);
   if (ctx->d_table) hipFree(ctx->d_table);
-  (void)sync_streams(ctx);
   for (hipGraphExec_t g : ctx->graph_exec)
     if (g) hipGraphExecDestroy(g);
   for (hipGraphExec_t g : ctx->user_graphs)
@@ -2063,12 +2063,12 @@ int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int m
     st = run_chunk(ctx, part[g].data(), (int)part[g].size(), rowmap, max_rounds, rounds, g);
   }
   ctx->stream = ctx->stream_main;
-  if (st) return st;
+  // joined also after a failed enqueue, so later work still follows it
   for (int g = 1; g < groups; ++g) {
     HCK(hipEventRecord(ctx->ev_join[g], ctx->stream_grp[g]));
     HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join[g], 0));
   }
-  return MLP_OK;
+  return st;
 }
 
 }  // namespace
