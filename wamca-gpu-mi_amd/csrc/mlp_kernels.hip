@@ -69,10 +69,13 @@ __device__ __forceinline__ T ldg(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
 
+// Packed coordinates (x << 16 | y, x and y in [0, 32767]) as two signed
+// 16-bit lanes: one packed subtraction gives dx and dy exactly (|d| <= 32767)
+// and one 16-bit dot product dx^2 + dy^2 < 2^31 (v_pk_sub_i16, v_dot2).
+typedef short short2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int dist_xy(int a, int b, float dr) {
-  const int dx = (a >> 16) - (b >> 16);
-  const int dy = (a & 0xffff) - (b & 0xffff);
-  return dist_s(__mul24(dx, dx) + __mul24(dy, dy), dr);
+  const short2v d = __builtin_bit_cast(short2v, a) - __builtin_bit_cast(short2v, b);
+  return dist_s(__builtin_amdgcn_sdot2(d, d, 0, false), dr);
 }
 
 // ---------------------------------------------------------------------------
