@@ -212,14 +212,16 @@ int mlp_dvnd_config(mlp_ctx* ctx, int32_t chunk, int32_t use_graph, int32_t pers
  * dependency costs an agent-scope release and acquire (a write-back and an
  * invalidation of the XCD's L2), so on gfx950 the per-neighborhood launches
  * replayed as a graph, which get coherence at kernel boundaries, are faster
- * (64 instances, n in [50, 2000]: 0.037 s against 0.074 s). */
+ * (64 instances, n in [50, 2000]: 0.036 s against 0.072 s). */
 int mlp_dvnd_fused(mlp_ctx* ctx, int32_t on);
 /* groups = 2 (default): mlp_dvnd and mlp_dvnd_rounds split the slots into
  * two groups (the largest slots holding 40% of the work of a round, and the
  * rest) and run the second group's rounds on a second stream, so the merges
  * of one group (one workgroup per slot) overlap the evaluation of the other;
- * work queued afterwards on the context stream follows both groups (64
- * instances, n in [50, 2000]: 42.6 ms with one group, 38.0 ms with two).
+ * in mlp_dvnd each group is checked for convergence as soon as its own chunk
+ * of rounds ends; work queued afterwards on the context stream follows both
+ * groups (64 instances, n in [50, 2000]: 41.5 ms with one group, 35.5 ms
+ * with two).
  * groups = 1: one stream.  Results are the same either way.  Not used with
  * the fused or persistent forms or inside a caller's graph capture. */
 int mlp_dvnd_groups(mlp_ctx* ctx, int32_t groups);

@@ -236,6 +236,7 @@ struct mlp_ctx {
   int dvnd_groups = 2;
   hipStream_t stream_main = nullptr, stream_grp[MLP_DVND_GROUPS] = {};  // [0] unused
   hipEvent_t ev_fork = nullptr, ev_join[MLP_DVND_GROUPS] = {};
+  hipEvent_t ev_chunk[MLP_DVND_GROUPS] = {};  // end of a group's chunk (mlp_dvnd)
   void* scratch = nullptr;  // device address of this GPU's copy of the static scratch array
   // caller-owned device buffers (mlp_dev_alloc) and event-pair timers
   std::vector<void*> dev_bufs;
@@ -534,9 +535,10 @@ int mlp_ctx_destroy(mlp_ctx* ctx) {
     }
   hipStreamDestroy(ctx->stream_main);
   if (ctx->side) hipStreamDestroy(ctx->side);
-  for (int g = 1; g < MLP_DVND_GROUPS; ++g) {
+  for (int g = 0; g < MLP_DVND_GROUPS; ++g) {
     if (ctx->stream_grp[g]) hipStreamDestroy(ctx->stream_grp[g]);
     if (ctx->ev_join[g]) hipEventDestroy(ctx->ev_join[g]);
+    if (ctx->ev_chunk[g]) hipEventDestroy(ctx->ev_chunk[g]);
   }
   if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
   delete ctx;
@@ -2034,6 +2036,23 @@ int split_groups(mlp_ctx* ctx, const std::vector<int32_t>& active, int groups,
   return MLP_OK;
 }
 
+// The group streams and events of `groups` slot groups, created once.
+int group_streams(mlp_ctx* ctx, int groups) {
+  if (!ctx->ev_fork &&
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail(ctx, MLP_EDEVICE, "DVND group event: %s", hipGetErrorString(hipGetLastError()));
+  for (int g = 0; g < groups; ++g) {
+    if (g && !ctx->stream_grp[g] &&
+        (hipStreamCreateWithFlags(&ctx->stream_grp[g], hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_join[g], hipEventDisableTiming) != hipSuccess))
+      return fail(ctx, MLP_EDEVICE, "DVND group stream: %s", hipGetErrorString(hipGetLastError()));
+    if (!ctx->ev_chunk[g] &&
+        hipEventCreateWithFlags(&ctx->ev_chunk[g], hipEventDisableTiming) != hipSuccess)
+      return fail(ctx, MLP_EDEVICE, "DVND group event: %s", hipGetErrorString(hipGetLastError()));
+  }
+  return MLP_OK;
+}
+
 // `rounds` DVND rounds over the slots, split into groups on streams of
 // their own when enabled (the fused and persistent forms share device
 // counters across slots, so they keep one stream; so does a caller's
@@ -2044,16 +2063,10 @@ int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int m
   const int groups = std::min(ctx->dvnd_groups, count);
   if (groups < 2 || ctx->dvnd_fused || ctx->dvnd_persistent || ctx->user_capture)
     return run_chunk(ctx, slots, count, rowmap, max_rounds, rounds);
-  if (!ctx->ev_fork &&
-      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess)
-    return fail(ctx, MLP_EDEVICE, "DVND group event: %s", hipGetErrorString(hipGetLastError()));
-  for (int g = 1; g < groups; ++g)
-    if (!ctx->stream_grp[g] &&
-        (hipStreamCreateWithFlags(&ctx->stream_grp[g], hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&ctx->ev_join[g], hipEventDisableTiming) != hipSuccess))
-      return fail(ctx, MLP_EDEVICE, "DVND group stream: %s", hipGetErrorString(hipGetLastError()));
+  int st = group_streams(ctx, groups);
+  if (st) return st;
   std::vector<int32_t> active(slots, slots + count), part[MLP_DVND_GROUPS];
-  int st = split_groups(ctx, active, groups, part);
+  st = split_groups(ctx, active, groups, part);
   if (st) return st;
   // the group streams start after the work queued so far on the context stream
   HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
@@ -2064,6 +2077,65 @@ int run_grouped(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap, int m
   }
   ctx->stream = ctx->stream_main;
   // joined also after a failed enqueue, so later work still follows it
+  for (int g = 1; g < groups; ++g) {
+    HCK(hipEventRecord(ctx->ev_join[g], ctx->stream_grp[g]));
+    HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join[g], 0));
+  }
+  return st;
+}
+
+// mlp_dvnd over slot groups that advance independently: each group's chunk
+// of rounds is followed by its own convergence check as soon as that chunk
+// is done (the host polls the groups' chunk events), so a group of small
+// slots does not wait at every check for the group of large ones.  The
+// groups are fixed for the call; the context stream waits for every group
+// at the end.
+int dvnd_independent(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap,
+                     int max_rounds) {
+  const int groups = std::min(ctx->dvnd_groups, count);
+  int st = group_streams(ctx, groups);
+  if (st) return st;
+  std::vector<int32_t> active(slots, slots + count), part[MLP_DVND_GROUPS], flag;
+  st = split_groups(ctx, active, groups, part);
+  if (st) return st;
+  HCK(hipEventRecord(ctx->ev_fork, ctx->stream_main));
+  for (int g = 1; g < groups; ++g) HCK(hipStreamWaitEvent(ctx->stream_grp[g], ctx->ev_fork, 0));
+  bool pending[MLP_DVND_GROUPS] = {};
+  for (;;) {
+    for (int g = 0; g < groups && !st; ++g)
+      if (!pending[g] && !part[g].empty()) {
+        ctx->stream = g ? ctx->stream_grp[g] : ctx->stream_main;
+        st = run_chunk(ctx, part[g].data(), (int)part[g].size(), rowmap, max_rounds,
+                       ctx->dvnd_chunk, g);
+        if (!st && hipEventRecord(ctx->ev_chunk[g], ctx->stream) != hipSuccess)
+          st = fail(ctx, MLP_EDEVICE, "DVND chunk event");
+        pending[g] = !st;
+      }
+    int g = -1;
+    for (int h = 0; h < groups && !st; ++h) g = pending[h] ? h : g;
+    if (st || g < 0) break;  // an error, or every group converged
+    // the first group whose chunk is done (a spin on the chunks' events)
+    for (g = -1; g < 0;)
+      for (int h = 0; h < groups && g < 0; ++h)
+        if (pending[h]) {
+          const hipError_t q = hipEventQuery(ctx->ev_chunk[h]);
+          if (q == hipSuccess) g = h;
+          else if (q != hipErrorNotReady) {
+            ctx->stream = ctx->stream_main;
+            return fail(ctx, MLP_EDEVICE, "DVND chunk: %s", hipGetErrorString(q));
+          }
+        }
+    pending[g] = false;
+    ctx->stream = g ? ctx->stream_grp[g] : ctx->stream_main;
+    flag.resize(part[g].size());
+    st = mlp_dvnd_status(ctx, part[g].data(), (int)part[g].size(), flag.data(), nullptr);
+    std::vector<int32_t> next;
+    for (size_t q = 0; q < part[g].size() && !st; ++q)
+      if (flag[q]) next.push_back(part[g][q]);
+    if (!st) part[g].swap(next);
+  }
+  ctx->stream = ctx->stream_main;
+  // joined also after an error, so later work still follows every group
   for (int g = 1; g < groups; ++g) {
     HCK(hipEventRecord(ctx->ev_join[g], ctx->stream_grp[g]));
     HCK(hipStreamWaitEvent(ctx->stream_main, ctx->ev_join[g], 0));
@@ -2156,6 +2228,11 @@ int mlp_dvnd(mlp_ctx* ctx, const int32_t* slots, int32_t count, int32_t rowmap,
   st = mlp_dvnd_start(ctx, slots, count);
   if (st) return st;
   std::vector<int32_t> active(slots, slots + count), flag((size_t)count);
+  if (count >= 2 && ctx->dvnd_groups > 1 && !ctx->dvnd_fused && !ctx->dvnd_persistent) {
+    st = dvnd_independent(ctx, slots, count, rowmap, max_rounds);
+    if (st) return st;
+    active.clear();
+  }
   while (!active.empty()) {
     const int c = (int)active.size();
     st = run_grouped(ctx, active.data(), c, rowmap, max_rounds, ctx->dvnd_chunk);

@@ -244,7 +244,8 @@ class Engine:
 
     def dvnd_groups(self, groups=2):
         """mlp_dvnd over two slot groups on two streams (the largest slots,
-        and the rest), or one group (1)."""
+        and the rest), each checked for convergence when its own chunk of
+        rounds ends, or one group (1)."""
         self._ck(self._L.mlp_dvnd_groups(self.h, int(groups)))
 
     def dvnd_start(self, slots):
