@@ -2852,9 +2852,11 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
   // 0 and ctx = i + by - 1 rises in part 1 (above every part-0 ctx), so
   // the lowest ctx among equal costs is the last equal one in part 0
   // (<=) and the first in part 1 (<).
-  auto eval_batch = [&](int k, const uint32_t* g, const uint32_t* h, const uint32_t* a,
-                        const uint32_t* b) {
-    const int part = k < nb0 ? 0 : 1;
+  // (the part is a template constant: as a run-time value the tie-break's
+  // two compares, the ctx select and the batch offset were computed per lane)
+  auto eval_part = [&](auto pc, int k, const uint32_t* g, const uint32_t* h, const uint32_t* a,
+                       const uint32_t* b) {
+    constexpr int part = decltype(pc)::value ? 1 : 0;
     const uint16_t* e = L.e[wave][part];
     const int m = mm[part], lo = ilo[part], hi = ihi[part];
     const int i0 = lo + (k - (part ? nb0 : 0)) * CU * kWave;
@@ -2872,6 +2874,11 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
       bd = take ? (int32_t)d : bd;
       bc = take ? ctx : bc;
     }
+  };
+  auto eval_batch = [&](int k, const uint32_t* g, const uint32_t* h, const uint32_t* a,
+                        const uint32_t* b) {
+    if (k < nb0) eval_part(Flag<false>{}, k, g, h, a, b);
+    else eval_part(Flag<true>{}, k, g, h, a, b);
   };
   // two register sets, alternating: batch k + 1 loads while batch k runs
   const int nbat = nb0 + nb1;

@@ -14,6 +14,7 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <map>
 #include <mutex>
@@ -2114,8 +2115,9 @@ int dvnd_independent(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap,
     int g = -1;
     for (int h = 0; h < groups && !st; ++h) g = pending[h] ? h : g;
     if (st || g < 0) break;  // an error, or every group converged
-    // the first group whose chunk is done (a spin on the chunks' events)
-    for (g = -1; g < 0;)
+    // the first group whose chunk is done (a spin on the chunks' events that
+    // yields the core between polls)
+    for (g = -1; g < 0;) {
       for (int h = 0; h < groups && g < 0; ++h)
         if (pending[h]) {
           const hipError_t q = hipEventQuery(ctx->ev_chunk[h]);
@@ -2125,6 +2127,8 @@ int dvnd_independent(mlp_ctx* ctx, const int32_t* slots, int count, int rowmap,
             return fail(ctx, MLP_EDEVICE, "DVND chunk: %s", hipGetErrorString(q));
           }
         }
+      if (g < 0) std::this_thread::yield();
+    }
     pending[g] = false;
     ctx->stream = g ? ctx->stream_grp[g] : ctx->stream_main;
     flag.resize(part[g].size());
