@@ -767,7 +767,7 @@ __device__ __forceinline__ int greedy_wave(const SlotDev& s, int nb, int N, int 
     const int b = KIND == 1 ? j + 1 : max(i, j) + K;
     bool pre = valid;
     if (valid) {
-      if (KIND == 0) pre = !(claimed(bits, N, i) | claimed(bits, N, j));
+      if (KIND == 0) pre = !((int)claimed(bits, N, i) | (int)claimed(bits, N, j));
       else pre = !claims_any(bits, N, a, b);
     }
     // the first remaining candidate is kept and every candidate it
@@ -2000,7 +2000,10 @@ struct TwoLds {
 struct NoTwoLds {};
 
 // ODC row map inside a fixed-row tile: per-diagonal minimum keys of or-opt-k
-// (k = 1..3) at slot (j - i) - (q0 - b0 - R - 1).
+// (k = 1..3) at slot (j - i) - (q0 - b0 - R - 1).  With packed keys (|delta|
+// < 2^26) the slots are int32 at the start of each array: delta * 32 + the
+// move's tile row (forward moves) or R - 1 - that row (backward moves), so
+// the minimum is the lowest delta and then the lowest ODC index.
 template <int R, int C>
 struct OdcLds {
   int64_t dg[3][C + R];
@@ -2318,7 +2321,8 @@ __device__ __forceinline__ void tile_swap(const SlotDev& s, int b0, int q0, int 
 // move (j > i) belongs to row j - i - 1 with index j - 2, a backward one with
 // i - j >= 2 to row N - K - 1 - (i - j) with index N - K - 1 - i (backward
 // moves by one position are not in that row map); each move's key is folded
-// into the tile's per-diagonal minimum dg[(j - i) - (q0 - b0 - R - 1)] in LDS.
+// into the tile's per-diagonal minimum dg[(j - i) - (q0 - b0 - R - 1)] in LDS
+// (an int32 packed key when pk).
 template <int R, int C, int RW, int K, bool ODC = false>
 __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int q1, int cbase, TileLds<R, C>& L,
                            int parity, int wave, int lane, bool pk, int64_t* dg = nullptr) {
@@ -2365,8 +2369,13 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
   const int jhi = min(q1, N - K);
   const int slo = q0 - b0 - R - 1;
   // PK: packed keys in bd (bc unused)
+  int32_t* dg32 = reinterpret_cast<int32_t*>(dg);
   auto upd = [&](auto pkc, int r, uint32_t d, int it, int j, bool ok, bool fwd) {
-    if constexpr (ODC) {
+    if constexpr (ODC && decltype(pkc)::value) {
+      const int S = j - bb - r - (fwd ? 1 : 2);  // j - i
+      if (ok && S != -1)
+        atomicMin(&dg32[S - slo], pack_key(d, fwd ? r0 + r : R - 1 - (r0 + r)));
+    } else if constexpr (ODC) {
       const int i = fwd ? bb + r + 1 : bb + r + 2;
       const int S = j - i;
       if (ok && S != -1)
@@ -2431,8 +2440,13 @@ __device__ __forceinline__ void tile_oropt(const SlotDev& s, int b0, int q0, int
   };
   int64_t* out = row_buf(s, K + 1, (parity >> (K + 1)) & 1) + bb;
   if constexpr (ODC) {  // the tile flushes the diagonals afterwards
-    if (rv == RW) cols(Flag<true>{}, Flag<false>{});
-    else cols(Flag<false>{}, Flag<false>{});
+    if (pk) {
+      if (rv == RW) cols(Flag<true>{}, Flag<true>{});
+      else cols(Flag<false>{}, Flag<true>{});
+    } else {
+      if (rv == RW) cols(Flag<true>{}, Flag<false>{});
+      else cols(Flag<false>{}, Flag<false>{});
+    }
     return;
   }
   if (pk) {
@@ -2576,7 +2590,11 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     if (tid < 4) F.two.dgx[tid] = MLP_KEY_NONE;
   }
   if constexpr (ODC) {
-    for (int q = tid; q < 3 * (C + R); q += kBlock) F.odc.dg[q / (C + R)][q % (C + R)] = MLP_KEY_NONE;
+    if (pk)
+      for (int q = tid; q < 3 * (C + R); q += kBlock)
+        reinterpret_cast<int32_t*>(F.odc.dg[q / (C + R)])[q % (C + R)] = INT32_MAX;
+    else
+      for (int q = tid; q < 3 * (C + R); q += kBlock) F.odc.dg[q / (C + R)][q % (C + R)] = MLP_KEY_NONE;
   }
   // the or-opt diagonals of an ODC tile (unused otherwise)
   int64_t* dg1 = nullptr;
@@ -2622,9 +2640,20 @@ __device__ __forceinline__ void sweep_fixed(const SlotDev& s, int rb, int ct, in
     for (int q = tid; q < 3 * (C + R); q += kBlock) {
       const int k = q / (C + R) + 1, sl = q % (C + R);
       if (!((mask >> (k + 1)) & 1)) continue;
-      const int64_t key = F.odc.dg[k - 1][sl];
-      if (key == MLP_KEY_NONE) continue;
       const int S = slo + sl;  // j - i
+      int64_t key;
+      if (pk) {
+        // packed: delta * 32 + tile row (forward) or R - 1 - tile row
+        // (backward); ODC index j - 2 = S + i - 2 or N - k - 1 - i
+        const int32_t kp = reinterpret_cast<const int32_t*>(F.odc.dg[k - 1])[sl];
+        if (kp == INT32_MAX) continue;
+        const int o = kp & 31;
+        const int idx = S >= 1 ? S + b0 + o - 1 : N - k - 1 - (b0 + (R - 1 - o) + 2);
+        key = (int64_t)(kp >> 5) * (int64_t)(1 << MLP_KEY_SHIFT) + idx;
+      } else {
+        key = F.odc.dg[k - 1][sl];
+        if (key == MLP_KEY_NONE) continue;
+      }
       int64_t* p = row_buf(s, k + 1, (parity >> (k + 1)) & 1) + (S >= 1 ? S - 1 : N - k - 1 + S);
       if (filtered) fold_key_filtered(p, key);
       else fold_key(p, key);
@@ -2670,7 +2699,10 @@ __device__ __forceinline__ void sweep_2opt(const SlotDev& s, int rb, int ct, int
   // part on its own (the flat form measured 3% slower at 4x256)
   constexpr bool FLAT = C >= 1024;
   constexpr int FP = FLAT ? 1 : 2, FI = FLAT ? FT : FE;
-  constexpr int CU = 4;                            // compute iterations per load batch
+  // compute iterations per load batch (a part of a tile holds up to C + 1
+  // entries: two for the 128-column tiles, whose batches would otherwise be
+  // half empty)
+  constexpr int CU = C >= 4 * kWave ? 4 : 2;
   const int N = s.N, tid = threadIdx.x, wave = wave_id(), lane = tid % kWave;
   const float dr = round_dr(s.round);
   const int rows = (N + 1) >> 1;
