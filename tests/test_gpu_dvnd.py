@@ -74,6 +74,24 @@ def test_dvnd_forms_match_oracle(eng, chunk, graph, persistent, rowmap, groups):
         assert got[q][2] == rref
 
 
+@pytest.mark.parametrize("rowmap", [0, 1])
+def test_dvnd_groups_uneven_and_converged(eng, rowmap):
+    """Two slot groups that advance independently (each group's convergence
+    check follows its own chunk of rounds): one large slot in the first group,
+    small ones in the second, one of which starts at its local optimum (one
+    round that applies nothing retires it at its group's first check)."""
+    jobs = _jobs([700, 12, 35, 60], 1300 + rowmap)
+    xs, ys, W, t0 = jobs[3]
+    jobs[3] = (xs, ys, W, O.dvnd(W, t0, rowmap)[0])
+    eng.dvnd_groups(2)
+    got = _run(eng, jobs, rowmap)
+    for q, (xs, ys, W, t0) in enumerate(jobs):
+        tref, Lref, rref, _ = O.dvnd(W, t0, rowmap)
+        assert (got[q][1], got[q][2]) == (Lref, rref), q
+        assert np.array_equal(got[q][0], tref), q
+    assert got[3][2] == 1 and np.array_equal(got[3][0], jobs[3][3])
+
+
 @pytest.mark.parametrize("max_rounds", [1, 2, 5])
 def test_dvnd_max_rounds(eng, max_rounds):
     jobs = _jobs([120, 260], 77)
